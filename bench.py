@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark of the integration hot path on one MI355X.
+
+Workload (one step = one adaptive Cash-Karp RK5(4) step, tolerance 1e-10, with its
+rejected attempts): 1 star + 2 giant planets + 65536 planetesimals (m log-uniform in
+[1e-12, 1e-9] M_sun, non-self-interacting as the reference types planetesimals) +
+65533 massless test particles, a uniform in [0.4, 4] AU; N = 131072.  Collisions at
+1 x (R_i + R_j) with merging, ejection at 100 AU.  Synthetic, seeded (303).
+
+Reported: pair-interactions/s of the mixed-precision path (FP64 differences, FP32
+r^-3, compensated FP64 sums), simulated years per hour, the relative energy change
+over the timed steps, the same figures for the plain FP64 path on the same state,
+the roofline position of the dominant kernel (the test-particle force sweep) from
+HIP events around its launches, and the CPU oracle's pair rate on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "red.cuda.closeencounter_amd"))
+
+GAUSS = 1.720209895e-2
+PEAK_FP32_TFLOPS = 157.3     # MI355X vector FP32 (packed), vendor figure
+PEAK_FP64_TFLOPS = 78.6      # MI355X vector FP64
+FLOP_PER_PAIR = 20           # 3 sub, 6 r^2, 1 rsqrt, 2 r^-3, 1 mass, 6 accumulate (+1 select)
+
+
+def pair_counts(counts):
+    ns, ngp, nrp, npp, nspl, npl, ntp = counts
+    n_si = ns + ngp + nrp + npp
+    n_nsi = nspl + npl
+    n_m = n_si + n_nsi
+    si = n_si * n_m - n_si          # self pairs excluded
+    nsi = n_nsi * n_si
+    ni = ntp * n_m
+    return si + nsi + ni, dict(si=si, nsi=nsi, ni=ni)
+
+
+def run_path(native, bodies, precision, steps, warmup, kahan=False):
+    d = native.Disk(bodies, integrator=3, adaptive=True, tolerance=1e-10, dt_day=1.0, precision=precision,
+                    radii_enhance_factor=1.0, ejection=100.0, kahan_pairs=kahan, use_graph=False)
+    for _ in range(warmup):
+        d.step()
+        d.check_for_ejection_hit_centrum()
+        d.check_for_collision()
+        d.clear_event_counter()
+    E0, _ = d.energy()
+    d.sync()
+    d.set_profiling(True)
+    d.profile(reset=True)
+    tries = 0
+    sim_t = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim_t += d.step()
+        tries += d.last_step.tries
+        d.check_for_ejection_hit_centrum()
+        d.check_for_collision()
+        d.clear_event_counter()
+    d.sync()
+    t1 = time.perf_counter()
+    prof = d.profile(reset=True)
+    d.set_profiling(False)
+    E1, _ = d.energy()
+    return dict(seconds=t1 - t0, tries=tries, sim_t=sim_t, E0=E0, E1=E1, prof=prof,
+                counts=d.counts(), t_end=d.t)
+
+
+def cpu_baseline(bodies_full, target_s=10.0):
+    """CPU oracle force evaluations on a bounded sample of the same disk."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+    from rednative import files
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    m = sum(bodies_full.counts[:6])
+
+    def sample(ntp):
+        b = bodies_full
+        keep = np.r_[0:m, m:m + ntp]
+        c = list(b.counts[:6]) + [ntp]
+        return files.BodySet(c, b.ids[keep], [b.names[k] for k in keep], b.types[keep], b.epoch[keep],
+                             b.mass[keep], b.radius[keep], b.density[keep], b.cd[keep], b.mig_type[keep],
+                             b.mig_stop_at[keep], b.r[keep], b.v[keep])
+
+    def rate(ntp):
+        s = sample(ntp)
+        o = pyoracle.Oracle(s, n_threads=threads, radii_enhance_factor=1.0)
+        t0 = time.perf_counter()
+        o.accel(0.0, s.r, s.v)
+        dt = time.perf_counter() - t0
+        return pair_counts(s.counts)[0], dt
+
+    p, dt = rate(256)
+    ntp = int(max(256, min(bodies_full.counts[6], 256 * target_s / max(dt, 1e-6))))
+    p, dt = rate(ntp)
+    return dict(value=p / dt, unit="pair-interactions/s", cores=threads, kind="port",
+                sample=f"one FP64 force evaluation (oracle, OpenMP over sinks) of the same disk with "
+                       f"{ntp} of its 65533 test particles: {p} pairs in {dt:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--fp64-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kahan", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the force sweep would shard by sinks with an all-gather of source positions).", file=sys.stderr)
+        sys.exit(2)
+
+    from rednative import generate, native
+    native.load_library()
+
+    bodies = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
+    pairs_eval, parts = pair_counts(bodies.counts)
+
+    mixed = run_path(native, bodies, native.PREC_MIXED, args.steps, args.warmup, kahan=args.kahan)
+    fp64 = run_path(native, bodies, native.PREC_FP64, args.fp64_steps, max(1, args.warmup // 2))
+
+    def summarize(r, steps):
+        evals = steps + 5 * r["tries"]            # f0 once per step, 5 stages per attempt
+        pairs = pairs_eval * evals
+        yrs = r["sim_t"] / GAUSS / 365.25
+        return dict(pairs_per_s=pairs / r["seconds"], sim_yr_per_hr=yrs / (r["seconds"] / 3600.0),
+                    ms_per_step=1e3 * r["seconds"] / steps, attempts=r["tries"], steps=steps,
+                    dE_over_E=abs((r["E1"] - r["E0"]) / r["E0"]), sim_days=r["sim_t"] / GAUSS)
+
+    sm = summarize(mixed, args.steps)
+    s64 = summarize(fp64, args.fp64_steps)
+
+    ni_ms, ni_n = mixed["prof"]["ni"]
+    avg_ni_s = (ni_ms / max(ni_n, 1)) * 1e-3
+    ni_pairs = bodies.counts[6] * sum(bodies.counts[:6])
+    achieved = FLOP_PER_PAIR * ni_pairs / avg_ni_s / 1e12
+    ni64_ms, ni64_n = fp64["prof"]["ni"]
+    achieved64 = FLOP_PER_PAIR * ni_pairs / ((ni64_ms / max(ni64_n, 1)) * 1e-3) / 1e12
+
+    cpu = None if args.no_cpu else cpu_baseline(bodies)
+
+    line = {
+        "metric": "pair-interactions/s & sim-yr/hr at N=131072, mixed vs FP64, ΔE/E≤1e-9",
+        "value": sm["pairs_per_s"],
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sm["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 differences + f32 r^-3 (mixed)",
+        "data": "synthetic (seeded disk, 1 star + 2 giants + 65536 planetesimals + 65533 test particles)",
+        "config": {"workload": "disk N=131072 (3 SI + 65536 NSI + 65533 NI), Cash-Karp RK5(4) tol 1e-10, "
+                               "collisions 1x(R_i+R_j) merged, ejection 100 AU",
+                   "pairs_per_eval": pairs_eval, "pairs_by_sweep": parts, "integrator": "rk5(4)"},
+        "sim_yr_per_hr": sm["sim_yr_per_hr"],
+        "dE_over_E_timed": sm["dE_over_E"],
+        "sim_days_timed": sm["sim_days"],
+        "attempts_timed": sm["attempts"],
+        "fp64": {"pairs_per_s": s64["pairs_per_s"], "sim_yr_per_hr": s64["sim_yr_per_hr"],
+                 "ms_per_step": s64["ms_per_step"], "dE_over_E_timed": s64["dE_over_E"],
+                 "steps": args.fp64_steps, "ni_sweep_tflops_20flop": achieved64,
+                 "ni_sweep_frac_fp64_peak": achieved64 / PEAK_FP64_TFLOPS},
+        "speedup_sim_yr_per_hr_vs_fp64": sm["sim_yr_per_hr"] / s64["sim_yr_per_hr"],
+        "speedup_pairs_per_s_vs_fp64": sm["pairs_per_s"] / s64["pairs_per_s"],
+        "roofline": {"bound": "valu", "kernel": "k_grav_mixed_bcast (test-particle sweep)",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "launch_ms": avg_ni_s * 1e3, "pairs_per_launch": ni_pairs,
+                     "flop_per_pair": FLOP_PER_PAIR},
+        "profile_ms": {k: v[0] for k, v in mixed["prof"].items() if v[1]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,219 @@
+/*
+ * rednative.h -- C ABI of the MI355X-native red.cuda integration path.
+ *
+ * Plain C: pointers, sizes and return codes only.  Every function returns
+ * RED_OK (0) or a positive RED_ERR_* code and never throws; the message of the
+ * last failure on a context is available from red_last_error().
+ *
+ * The reference is a monolithic C++/CUDA program; its "plugin" seam is the class
+ * surface that red.cuda/src/red.cuda/main.cu drives.  Each entry point below names
+ * the reference function it replaces.  Units are the reference's internal ones:
+ * lengths in AU, masses in solar masses, time t_int = k * t_day and velocity
+ * v_int = v_day / k with k = 0.01720209895 (Gaussian units, G = 1;
+ * red.cuda/src/red.cuda/pp_disk.cu:1512-1551, integrator.cu:29).
+ *
+ * Body arrays are in body-file order: star, giant, rocky, proto (self-interacting,
+ * SI), super-planetesimal, planetesimal (non-self-interacting, NSI), test particle
+ * (non-interacting, NI) (red.cuda/src/red.cuda/number_of_bodies.cpp:106-258).
+ * Positions and velocities cross the boundary as n*3 interleaved doubles.
+ *
+ * Ownership: a context owns all device memory it allocates and one HIP stream.
+ * Host arrays passed in are copied; host arrays passed out are caller-owned.
+ * Threading: one context per host thread; calls on one context must not overlap.
+ */
+#ifndef REDNATIVE_H
+#define REDNATIVE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------- */
+#define RED_OK                 0
+#define RED_ERR_ARG            1   /* invalid argument or state                       */
+#define RED_ERR_HIP            2   /* HIP runtime failure (launch, copy, allocation)  */
+#define RED_ERR_EVENT_OVERFLOW 3   /* more pair events in one attempt than capacity   */
+#define RED_ERR_NAN            4   /* non-finite error estimate                       */
+#define RED_ERR_UNSUPPORTED    5   /* what the reference throws "not implemented" for */
+#define RED_ERR_NO_STAR        6   /* no body of type STAR among the massive bodies   */
+#define RED_ERR_IO             7   /* file could not be read or parsed                */
+#define RED_ERR_STEP_LIMIT     8   /* attempt loop exceeded its cap                   */
+
+/* ---- enumerations (values match red.cuda/red.type/red_type.h:45-101) ---- */
+#define RED_EULER   0
+#define RED_RK2     1
+#define RED_RK4     2
+#define RED_RK5     3   /* Cash-Karp RK5(4) */
+#define RED_RKF78   4   /* Runge-Kutta-Fehlberg 7(8), the reference's "rkf8" */
+
+#define RED_FRAME_ASTRO 0
+#define RED_FRAME_BARY  1
+
+#define RED_PREC_FP64   0   /* FP64 everywhere, same operation order as the reference */
+#define RED_PREC_MIXED  1   /* FP64 differences, FP32 1/r^3, compensated FP64 sums    */
+
+#define RED_EV_NONE            0
+#define RED_EV_HIT_CENTRUM     1
+#define RED_EV_EJECTION        2
+#define RED_EV_CLOSE_ENCOUNTER 3
+#define RED_EV_COLLISION       4
+
+/* One event record: the reference's event_data_t (red.cuda/red.type/red_type.h:230-275)
+ * as a flat C struct, plus the detection key (attempt, stage, sink, source) that
+ * fixes a canonical order for records appended concurrently on the GPU. */
+typedef struct red_event {
+    int    name;                  /* RED_EV_*                                       */
+    int    id1, idx1;             /* survivor id / index                            */
+    int    id2, idx2;             /* merger id / index                              */
+    int    attempt, stage;        /* attempt within the step, stage index r         */
+    int    sink, source;          /* (i, j) of the detecting pair                   */
+    int    pad;
+    double t, d;                  /* event time (internal units), separation        */
+    double p1[4], p2[4], ps[4];   /* mass radius density cd: survivor, merger, after */
+    double r1[3], v1[3], r2[3], v2[3], rs[3], vs[3];
+    double oe1[6], oe2[6], oes[6];/* sma ecc inc peri node mean                     */
+} red_event;
+
+/* Run parameters: the reference's parameter file (red.cuda/src/red.cuda/parameter.cpp:32-292)
+ * plus the switches this implementation adds. */
+typedef struct red_params {
+    int    integrator;            /* RED_EULER .. RED_RKF78                         */
+    int    adaptive;              /* embedded-error step control (set by tolerance) */
+    double tolerance;
+    int    error_check_for_tp;    /* include test particles in the error norm       */
+    int    close_encounter;       /* report pair events instead of merging          */
+    int    inner_steps;           /* detect pair events inside the force sweep      */
+    int    frame_center;          /* RED_FRAME_BARY (RED_FRAME_ASTRO: not supported) */
+    double hit_centrum;           /* AU, 0 = off                                    */
+    double ejection;              /* AU, 0 = off                                    */
+    double radii_enhance_factor;  /* collision when d < f (R_i + R_j), 0 = off      */
+    double encounter_hill_factor; /* close encounter when d < f R_Hill, 0 = off     */
+    double dt_day;                /* initial step in days (reference: 0.001)        */
+    int    precision;             /* RED_PREC_FP64 or RED_PREC_MIXED                */
+    int    event_capacity;        /* pair events per attempt, 0 = max(4096, n)      */
+    int    use_graph;             /* replay each attempt from a captured HIP graph  */
+    int    kahan_pairs;           /* mixed: compensate every pair, not every tile   */
+    int    device;                /* HIP device ordinal                             */
+    int    block;                 /* gravity launch shape, 0 = autotuned/default    */
+    int    sinks_per_thread;      /* 0 = default                                    */
+    int    strategy;              /* 0 = auto, 1 = scalar-broadcast, 2 = LDS tiles  */
+    int    reserved[4];
+} red_params;
+
+/* Bodies in body-file order (host arrays, read during the call only). */
+typedef struct red_bodies {
+    int           counts[7];      /* ns ngp nrp npp nspl npl ntp                    */
+    const int*    id;
+    const int*    type;
+    const int*    mig_type;       /* may be NULL */
+    const double* mig_stop_at;    /* may be NULL */
+    const double* epoch;          /* may be NULL */
+    const double* mass;
+    const double* radius;
+    const double* density;
+    const double* cd;
+    const double* r;              /* n*3, AU                                        */
+    const double* v;              /* n*3, internal units                            */
+} red_bodies;
+
+typedef struct red_step_info {
+    double dt_did;                /* step taken (internal units)                    */
+    double dt_next;               /* next trial step                                */
+    double err;                   /* last error estimate (scaled as the reference)  */
+    int    tries;                 /* attempts in this step                          */
+    int    n_events;              /* raw pair events recorded in the step           */
+} red_step_info;
+
+typedef struct red_ctx red_ctx;
+
+/* Context life cycle.  Replaces pp_disk::pp_disk + options::create_integrator
+ * (red.cuda/src/red.cuda/pp_disk.cu:1204-1235, options.cpp:197-274): allocates device
+ * storage, uploads the state and thresholds (pp_disk.cu:1411-1452).  The state must
+ * already be in internal units and in the wanted frame (see red_host_* below). */
+int red_ctx_create(const red_bodies* bodies, const red_params* params, red_ctx** out);
+int red_ctx_destroy(red_ctx* ctx);
+/* Message of the last failure on ctx ("" if none); static text when ctx is NULL. */
+const char* red_last_error(const red_ctx* ctx);
+
+/* Counts after removals (number_of_bodies::update_numbers, number_of_bodies.cpp:27-104). */
+int red_get_counts(red_ctx* ctx, int counts[7]);
+/* Time and trial step, internal units (pp_disk::t, integrator::dt_try). */
+int red_get_time(red_ctx* ctx, double* t);
+int red_set_time(red_ctx* ctx, double t);
+int red_get_dt(red_ctx* ctx, double* dt_try);
+int red_set_dt(red_ctx* ctx, double dt_try);
+
+/* State transfer, body-file order.  Replaces pp_disk::copy_to_host / copy_to_device
+ * (pp_disk.cu:1411-1439).  Any output pointer may be NULL. */
+int red_get_state(red_ctx* ctx, double* r, double* v, double* mass, double* radius,
+                  double* density, double* cd, int* id, int* type);
+int red_set_state(red_ctx* ctx, const double* r, const double* v);
+
+/* Force evaluation on caller-owned device arrays.  Replaces pp_disk::calc_dydx(1, ...)
+ * -> call_kernel_calc_grav_accel (pp_disk.cu:609-681, 998-1041): the SI, NSI and NI
+ * sweeps with the fused pair tests.  r_dev, v_dev, a_dev are device pointers to n*4
+ * doubles (x, y, z, w per body; w of r_dev carries the mass, w of a_dev is written 0).
+ * t is the stage time stamped on events; stage tags them.  Asynchronous on the
+ * context stream; red_sync() waits. */
+int red_accel(red_ctx* ctx, double t, int stage, const double* r_dev, const double* v_dev, double* a_dev);
+/* Device arrays of the current state (n*4 doubles each, as above); valid until the
+ * next red_step, red_check_rebuild or red_ctx_destroy. */
+int red_state_device(red_ctx* ctx, double** r_dev, double** v_dev);
+int red_sync(red_ctx* ctx);
+
+/* One integrator step with step control.  Replaces integrator::step()
+ * (int_euler.cu:91-141, int_rungekutta2.cu:112-152, int_rungekutta4.cu:189-273,
+ * int_rungekutta5.cu:202-282, int_rungekutta8.cu:704-797). */
+int red_step(red_ctx* ctx, red_step_info* info);
+
+/* Event checks of the driver loop (main.cu:201-255).  *handled = 1 when events were
+ * found and handled.  Replace pp_disk::check_for_ejection_hit_centrum (pp_disk.cu:683-715),
+ * check_for_collision (:717-738), check_for_close_encounter(true) (:740-770) and
+ * check_for_rebuild_vectors (:893-907). */
+int red_check_ejection_hit_centrum(red_ctx* ctx, int* handled);
+int red_check_collision(red_ctx* ctx, int* handled);
+int red_check_close_encounter(red_ctx* ctx, int* handled);
+int red_check_rebuild(red_ctx* ctx, int threshold, int* rebuilt);
+/* pp_disk::clear_event_counter (pp_disk.cu:1469-1476). */
+int red_clear_events(red_ctx* ctx);
+/* Raw events since the last clear, canonical order (attempt, stage, sink, source).
+ * Writes min(n, cap) records; *n = total, *overflow = 1 if records were dropped. */
+int red_raw_events(red_ctx* ctx, red_event* buf, int cap, int* n, int* overflow);
+/* Events of the last handling pass as pp_disk::sp_events (pp_disk.cu:1111-1149). */
+int red_sp_events(red_ctx* ctx, red_event* buf, int cap, int* n);
+/* Index map of the last compaction: old index of each new body (n entries). */
+int red_last_rebuild_map(red_ctx* ctx, int* old_index, int cap, int* n);
+
+/* out[8]: tried failed passed, total collisions ejections hit-centrum close-encounters,
+ * events since the last rebuild (integrator.cu:258-263, pp_disk.h:129-133). */
+int red_counters(red_ctx* ctx, int out[8]);
+
+/* Not in the reference: total energy E = sum 1/2 m v^2 - sum m_i m_j / r_ij over
+ * interacting pairs (NSI-NSI excluded) and angular momentum L = sum m r x v, G = 1. */
+int red_energy(red_ctx* ctx, double* E, double L[3]);
+
+/* Host-side transforms (tools.cpp:122-178, pp_disk.cu:1512-1551): barycentric shift
+ * (sums run from the last body down, as the reference), and t*k / v/k scaling. */
+int red_host_transform_to_bc(int n, const double* mass, double* r, double* v);
+int red_host_transform_velocity(int n, double* v);
+
+/* Kernel timing (not in the reference, which has none beyond nvprof): with profiling on,
+ * HIP events bracket every launch group on the context stream; red_get_profile returns
+ * the accumulated milliseconds and launch counts per class.  Needs use_graph = 0. */
+#define RED_PROF_SI     0   /* force sweep: self-interacting sinks x massive sources */
+#define RED_PROF_NSI    1   /* force sweep: non-self-interacting sinks x SI sources  */
+#define RED_PROF_NI     2   /* force sweep: test particles x massive sources         */
+#define RED_PROF_STAGE  3   /* stage combinations and y_(n+1)                        */
+#define RED_PROF_ERR    4   /* error estimate + max-reduction                        */
+#define RED_PROF_PACK   5   /* mixed-precision source packing                        */
+#define RED_PROF_CLASSES 8
+int red_set_profiling(red_ctx* ctx, int on);
+int red_get_profile(red_ctx* ctx, double ms[RED_PROF_CLASSES], int launches[RED_PROF_CLASSES], int reset);
+
+/* Library version string. */
+const char* red_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* REDNATIVE_H */

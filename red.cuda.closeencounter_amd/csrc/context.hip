@@ -1,0 +1,1099 @@
+// context.hip -- the C ABI: context life cycle, integrator step control, event pipeline.
+//
+// Host-side counterpart of pp_disk + integrator (red.cuda/src/red.cuda/pp_disk.cu,
+// integrator.cu, int_*.cu).  The per-attempt work runs on one HIP stream, optionally
+// replayed from a captured HIP graph; the host reads back one small control block per
+// attempt (error maximum, NaN flag, event count), which is what the reference reads with
+// its two blocking copies in integrator::get_max_error (integrator.cu:211-256).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host_tools.hpp"
+#include "red_internal.hpp"
+
+using namespace red;
+
+namespace {
+
+const double GAUSS = tools::GAUSS;
+const double GAUSS2 = tools::GAUSS2;
+
+enum { BT_STAR = 0, BT_GIANT = 1, BT_ROCKY = 2, BT_PROTO = 3, BT_SUPERPL = 4, BT_PL = 5, BT_TP = 6, BT_PAD = 7 };
+enum { CNT_TOTAL = 0, CNT_LAST_CLEAR = 1, CNT_LAST_STEP = 2 };
+
+// One stage combination: which form, which stages feed it with which coefficient.
+struct Row {
+    int form = SF_AXPY;
+    int nterms = 0;
+    int stage[MAX_TERMS] = {0};
+    double coef[MAX_TERMS] = {0.0};
+    double c1 = 0.0, c2 = 0.0;
+    void add(int s, double c) { stage[nterms] = s; coef[nterms] = c; nterms++; }
+};
+
+// An embedded Runge-Kutta scheme in the shape the reference steps it.
+struct Scheme {
+    int n_stage = 1;            // evaluations per attempt including f0
+    int n_main = 1;             // stages before y_(n+1)
+    std::vector<double> c;      // stage times
+    std::vector<Row> rows;      // rows[r] forms stage state r (r >= 1)
+    Row np1;                    // y_(n+1)
+    int err_form = -1;
+    Row err;
+    double lambda = 1.0;
+    double exponent = 1.0;
+    bool clear_events_each_attempt = false;   // RKF78 (int_rungekutta8.cu:730-733)
+    bool accept_on_event = false;             // RK4 / RK5 (int_rungekutta4.cu:255-262)
+    bool extra_on_yout = false;               // RK4: f5 = f(t + dt, y_(n+1))
+};
+
+Scheme make_scheme(int integrator, bool adaptive)
+{
+    Scheme S;
+    switch (integrator) {
+    case RED_EULER: {   // int_euler.cu:57-141
+        S.n_stage = 1; S.n_main = 1; S.c = {0.0};
+        S.np1.form = SF_AXPY; S.np1.add(0, 1.0);
+        break;
+    }
+    case RED_RK2: {     // int_rungekutta2.cu:13-152
+        const double a[2] = {0.0, 1.0 / 2.0}, b[2] = {0.0, 1.0};
+        S.n_stage = 2; S.n_main = 2; S.c = {0.0, 1.0 / 2.0};
+        S.rows.resize(2);
+        S.rows[1].form = SF_AXPY; S.rows[1].add(0, a[1]);
+        S.np1.form = SF_AXPY; S.np1.add(1, b[1]);
+        break;
+    }
+    case RED_RK4: {     // int_rungekutta4.cu:27-273
+        const double a[4] = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0};
+        const double b[2] = {1.0 / 6.0, 1.0 / 3.0};
+        S.c = {0.0, 1.0 / 2.0, 1.0 / 2.0, 1.0, 1.0};
+        S.n_main = 4; S.n_stage = adaptive ? 5 : 4;
+        S.rows.resize(4);
+        for (int r = 1; r < 4; r++) { S.rows[r].form = SF_AXPY; S.rows[r].add(r - 1, a[r]); }
+        S.np1.form = SF_RK4NP1; for (int k = 0; k < 4; k++) S.np1.add(k, 0.0);
+        S.np1.c1 = b[0]; S.np1.c2 = b[1];
+        S.err_form = EF_RK4; S.err.add(3, 0.0); S.err.add(4, 0.0);
+        S.lambda = 1.0 / 10.0; S.exponent = 1.0 / 4.0;
+        S.accept_on_event = true; S.extra_on_yout = true;
+        break;
+    }
+    case RED_RK5: {     // Cash-Karp, int_rungekutta5.cu:24-32 (tableau), restated as a working scheme
+        const double a[16] = {0.0,
+                              1.0 / 5.0,
+                              3.0 / 40.0, 9.0 / 40.0,
+                              3.0 / 10.0, -9.0 / 10.0, 6.0 / 5.0,
+                              -11.0 / 54.0, 5.0 / 2.0, -70.0 / 27.0, 35.0 / 27.0,
+                              1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0};
+        const double bh[6] = {37.0 / 378.0, 0.0, 250.0 / 621.0, 125.0 / 594.0, 0.0, 512.0 / 1771.0};
+        const double b[6] = {2825.0 / 27648.0, 0.0, 18575.0 / 48384.0, 13525.0 / 55296.0, 277.0 / 14336.0, 1.0 / 4.0};
+        const int idx[6] = {0, 1, 2, 4, 7, 11};
+        S.c = {0.0, 1.0 / 5.0, 3.0 / 10.0, 3.0 / 5.0, 1.0, 7.0 / 8.0};
+        S.n_stage = 6; S.n_main = 6;
+        S.rows.resize(6);
+        for (int r = 1; r < 6; r++) {
+            S.rows[r].form = SF_ACCUM;
+            for (int s = 0; s < r; s++) if (a[idx[r] + s] != 0.0) S.rows[r].add(s, a[idx[r] + s]);
+        }
+        S.np1.form = SF_ACCUM;
+        for (int s = 0; s < 6; s++) if (bh[s] != 0.0) S.np1.add(s, bh[s]);
+        S.err_form = EF_CK;
+        for (int s = 0; s < 6; s++) S.err.add(s, bh[s] - b[s]);
+        S.lambda = 1.0; S.exponent = 1.0 / 5.0;
+        S.accept_on_event = true;
+        break;
+    }
+    case RED_RKF78: {   // int_rungekutta8.cu:27-47, 476-797
+        const double a[79] = {
+            0.0,
+            2.0 / 27.0,
+            1.0 / 36.0, 1.0 / 12.0,
+            1.0 / 24.0, 0.0, 1.0 / 8.0,
+            5.0 / 12.0, 0.0, -25.0 / 16.0, 25.0 / 16.0,
+            1.0 / 20.0, 0.0, 0.0, 1.0 / 4.0, 1.0 / 5.0,
+            -25.0 / 108.0, 0.0, 0.0, 125.0 / 108.0, -65.0 / 27.0, 125.0 / 54.0,
+            31.0 / 300.0, 0.0, 0.0, 0.0, 61.0 / 225.0, -2.0 / 9.0, 13.0 / 900.0,
+            2.0, 0.0, 0.0, -53.0 / 6.0, 704.0 / 45.0, -107.0 / 9.0, 67.0 / 90.0, 3.0,
+            -91.0 / 108.0, 0.0, 0.0, 23.0 / 108.0, -976.0 / 135.0, 311.0 / 54.0, -19.0 / 60.0, 17.0 / 6.0, -1.0 / 12.0,
+            2383.0 / 4100.0, 0.0, 0.0, -341.0 / 164.0, 4496.0 / 1025.0, -301.0 / 82.0, 2133.0 / 4100.0, 45.0 / 82.0, 45.0 / 164.0, 18.0 / 41.0,
+            3.0 / 205.0, 0.0, 0.0, 0.0, 0.0, -6.0 / 41.0, -3.0 / 205.0, -3.0 / 41.0, 3.0 / 41.0, 6.0 / 41.0, 0.0,
+            -1777.0 / 4100.0, 0.0, 0.0, -341.0 / 164.0, 4496.0 / 1025.0, -289.0 / 82.0, 2193.0 / 4100.0, 51.0 / 82.0, 33.0 / 164.0, 12.0 / 41.0, 0.0, 1.0};
+        const double b[11] = {41.0 / 840.0, 0.0, 0.0, 0.0, 0.0, 34.0 / 105.0, 9.0 / 35.0, 9.0 / 35.0, 9.0 / 280.0, 9.0 / 280.0, 41.0 / 840.0};
+        const int idx[13] = {0, 1, 2, 4, 7, 11, 16, 22, 29, 37, 46, 56, 67};
+        S.c = {0.0, 2.0 / 27.0, 1.0 / 9.0, 1.0 / 6.0, 5.0 / 12.0, 1.0 / 2.0, 5.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0, 1.0 / 3.0, 1.0, 0.0, 1.0};
+        S.n_main = 11; S.n_stage = adaptive ? 13 : 11;
+        S.rows.resize(13);
+        for (int r = 1; r < 13; r++) {
+            S.rows[r].form = SF_DTSUM;
+            for (int s = 0; s < r; s++) if (a[idx[r] + s] != 0.0) S.rows[r].add(s, a[idx[r] + s]);
+        }
+        S.np1.form = SF_RKF78NP1;
+        S.np1.add(0, b[0]); S.np1.add(5, b[5]); S.np1.add(6, b[6]); S.np1.add(7, b[7]);
+        S.np1.add(8, b[8]); S.np1.add(9, b[9]); S.np1.add(10, b[10]);
+        S.err_form = EF_RKF78;
+        S.err.add(0, 0.0); S.err.add(10, 0.0); S.err.add(11, 0.0); S.err.add(12, 0.0);
+        S.lambda = 41.0 / 840.0; S.exponent = 1.0 / (7 + 1);
+        S.clear_events_each_attempt = true;
+        break;
+    }
+    }
+    return S;
+}
+
+__global__ void k_set_ctl(Ctl* ctl, double t, double dt, int attempt, int reset_err, int clear_events, int clear_hc)
+{
+    ctl->t = t;
+    ctl->dt = dt;
+    ctl->attempt = attempt;
+    if (reset_err) { ctl->err_bits = 0ull; ctl->nan_flag = 0; }
+    if (clear_events) ctl->n_events = 0;
+    if (clear_hc) ctl->n_events_hc = 0;
+}
+
+template <typename T>
+hipError_t dmalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+bool event_key_less(const red_event& a, const red_event& b)
+{
+    if (a.attempt != b.attempt) return a.attempt < b.attempt;
+    if (a.stage != b.stage) return a.stage < b.stage;
+    if (a.sink != b.sink) return a.sink < b.sink;
+    return a.source < b.source;
+}
+
+}  // namespace
+
+struct red_ctx {
+    red_params prm{};
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cnt[7] = {0};
+    int n_inactive[7] = {0};
+    int n = 0, cap = 0;
+    // host mirror of the per-body parameters (pp_disk::h_p, h_body_md)
+    std::vector<double> h_mass, h_radius, h_density, h_cd, h_epoch, h_mig_stop;
+    std::vector<int> h_id, h_type, h_mig_type;
+    // device storage
+    d4* d_r[2] = {nullptr, nullptr};
+    d4* d_v[2] = {nullptr, nullptr};
+    int cur = 0;
+    std::vector<d4*> d_sr, d_sv, d_a;
+    double* d_mass = nullptr;
+    double* d_radius = nullptr;
+    int* d_id = nullptr;
+    int* d_type = nullptr;
+    Ctl* d_ctl = nullptr;
+    Ctl* h_ctl = nullptr;                 // pinned
+    red_event* d_events = nullptr;
+    int ev_cap = 0;
+    red_event* d_events_hc = nullptr;
+    void* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    SrcMixed* d_src_mixed = nullptr;
+    BodyUpdate* d_updates = nullptr;
+    int updates_cap = 0;
+    double* d_energy = nullptr;
+    // integrator state (integrator.h:359-383)
+    Scheme S;
+    double t = 0.0, dt_try = 0.0, dt_did = 0.0, last_err = 0.0;
+    int n_tried = 0, n_failed = 0, n_passed = 0;
+    int attempts_cap = 1000;
+    // event counters (pp_disk.h:129-133)
+    int n_hit_centrum[3] = {0}, n_ejection[3] = {0}, n_collision[3] = {0}, n_close_encounter[3] = {0};
+    int n_event[3] = {0};
+    std::vector<red_event> raw;           // pair events, canonical order
+    bool raw_valid = true;
+    std::vector<red_event> sp;
+    std::vector<int> rebuild_map;
+    // launch configuration
+    GravConfig gcfg{};
+    PairTest ptest{};
+    // optional kernel timing: HIP events around every launch group, resolved on demand
+    struct ProfRec { int cls; hipEvent_t a, b; };
+    bool profiling = false;
+    std::vector<ProfRec> prof;
+    double prof_ms[RED_PROF_CLASSES] = {0.0};
+    int prof_n[RED_PROF_CLASSES] = {0};
+    // captured graphs, per state-buffer parity
+    hipGraphExec_t g_attempt[2] = {nullptr, nullptr};
+    hipGraphExec_t g_f0[2] = {nullptr, nullptr};
+
+    int nSI() const { return cnt[0] + cnt[1] + cnt[2] + cnt[3]; }
+    int nNSI() const { return cnt[4] + cnt[5]; }
+    int nNI() const { return cnt[6]; }
+    int nMassive() const { return nSI() + nNSI(); }
+
+    int fail(int code, const std::string& msg) { err = msg; return code; }
+    int hipfail(hipError_t e, const char* where)
+    {
+        err = std::string(where) + ": " + hipGetErrorString(e);
+        return RED_ERR_HIP;
+    }
+};
+
+#define CTX_HIP(ctx, expr)                                       \
+    do {                                                         \
+        hipError_t _e = (expr);                                  \
+        if (_e != hipSuccess) return (ctx)->hipfail(_e, #expr);  \
+    } while (0)
+
+namespace {
+
+void destroy_graphs(red_ctx* c)
+{
+    for (int k = 0; k < 2; k++) {
+        if (c->g_attempt[k]) (void)hipGraphExecDestroy(c->g_attempt[k]);
+        if (c->g_f0[k]) (void)hipGraphExecDestroy(c->g_f0[k]);
+        c->g_attempt[k] = c->g_f0[k] = nullptr;
+    }
+}
+
+double mass_of_star(const red_ctx* c)
+{
+    for (int j = 0; j < c->nMassive(); j++)
+        if (c->h_type[j] == BT_STAR) return c->h_mass[j];
+    return -1.0;
+}
+
+void refresh_pair_test(red_ctx* c)
+{
+    PairTest& P = c->ptest;
+    P.radii_f = c->prm.radii_enhance_factor;
+    P.hill_f = c->prm.encounter_hill_factor;
+    const double ms = mass_of_star(c);
+    P.three_mstar = 3.0 * (ms > 0 ? ms : 1.0);
+    double rmax = 0.0;
+    for (int i = 0; i < c->n; i++) rmax = std::max(rmax, c->h_radius[i]);
+    P.rmax = rmax;
+    P.close_encounter = c->prm.close_encounter;
+    P.enabled = c->prm.inner_steps && (P.radii_f > 0.0 || P.hill_f > 0.0);
+}
+
+// Stage state records: stage 0 = current state, stage k >= 1 = S_k, RK4's stage 4 = y_(n+1).
+const d4* stage_r(red_ctx* c, int k)
+{
+    if (k == 0) return c->d_r[c->cur];
+    if (c->S.extra_on_yout && k == 4) return c->d_r[1 - c->cur];
+    return c->d_sr[k];
+}
+const d4* stage_v(red_ctx* c, int k)
+{
+    if (k == 0) return c->d_v[c->cur];
+    if (c->S.extra_on_yout && k == 4) return c->d_v[1 - c->cur];
+    return c->d_sv[k];
+}
+
+// Records HIP events around fn() on the context stream when profiling is on.
+template <typename F>
+hipError_t timed(red_ctx* c, int cls, F fn)
+{
+    if (!c->profiling) return fn();
+    red_ctx::ProfRec rec{cls, nullptr, nullptr};
+    RED_HIP_TRY(hipEventCreate(&rec.a));
+    RED_HIP_TRY(hipEventCreate(&rec.b));
+    RED_HIP_TRY(hipEventRecord(rec.a, c->stream));
+    hipError_t e = fn();
+    RED_HIP_TRY(hipEventRecord(rec.b, c->stream));
+    c->prof.push_back(rec);
+    return e;
+}
+
+// Force evaluation of stage k on state (r, v) into a: the SI, NSI and NI sweeps
+// (pp_disk.cu:609-681).
+hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const d4* v, d4* a)
+{
+    GravBuffers B;
+    B.r = r; B.v = v; B.a = a;
+    B.mass = c->d_mass; B.radius = c->d_radius; B.id = c->d_id; B.ctl = c->d_ctl;
+    B.scratch = c->d_scratch; B.scratch_bytes = c->scratch_bytes;
+    B.src_mixed = c->d_src_mixed;
+    EventSink E;
+    E.buf = c->d_events; E.cap = c->ev_cap; E.stage = stage; E.c_stage = c_stage;
+    if (c->gcfg.precision == RED_PREC_MIXED)
+        RED_HIP_TRY(timed(c, RED_PROF_PACK, [&] { return launch_pack_mixed(r, c->d_id, c->d_src_mixed, c->nMassive(), c->stream); }));
+    const int nsi = c->nSI(), nm = c->nMassive(), n = c->n;
+    const Sweep sw[3] = {{0, nsi, 0, nm}, {nsi, nm, 0, nsi}, {nm, n, 0, nm}};
+    for (int k = 0; k < 3; k++) {
+        if (sw[k].sink1 <= sw[k].sink0) continue;
+        RED_HIP_TRY(timed(c, RED_PROF_SI + k, [&] { return launch_sweep(sw[k], B, c->gcfg, c->ptest, E, c->stream); }));
+    }
+    return hipSuccess;
+}
+
+StageArgs make_args(red_ctx* c, const Row& row, const d4* yr, const d4* yv, d4* outr, d4* outv, int n)
+{
+    StageArgs A;
+    memset(&A, 0, sizeof(A));
+    A.yr = yr; A.yv = yv; A.outr = outr; A.outv = outv; A.n = n;
+    A.nterms = row.nterms; A.c1 = row.c1; A.c2 = row.c2; A.ctl = c->d_ctl;
+    for (int k = 0; k < row.nterms; k++) {
+        A.t[k].coef = row.coef[k];
+        A.t[k].fv = stage_v(c, row.stage[k]);
+        A.t[k].fa = c->d_a[row.stage[k]];
+    }
+    return A;
+}
+
+int n_err_bodies(red_ctx* c) { return c->prm.error_check_for_tp ? c->n : c->nMassive(); }
+
+// The work of one attempt after f0 (int_rungekutta8.cu:735-777 and its siblings).
+hipError_t enqueue_attempt(red_ctx* c)
+{
+    const Scheme& S = c->S;
+    const d4* yr = c->d_r[c->cur];
+    const d4* yv = c->d_v[c->cur];
+    d4* outr = c->d_r[1 - c->cur];
+    d4* outv = c->d_v[1 - c->cur];
+    for (int r = 1; r < S.n_main; r++) {
+        StageArgs A = make_args(c, S.rows[r], yr, yv, c->d_sr[r], c->d_sv[r], c->n);
+        RED_HIP_TRY(timed(c, RED_PROF_STAGE, [&] { return launch_stage(S.rows[r].form, A, c->stream); }));
+        RED_HIP_TRY(eval_forces(c, r, S.c[r], c->d_sr[r], c->d_sv[r], c->d_a[r]));
+    }
+    StageArgs A = make_args(c, S.np1, yr, yv, outr, outv, c->n);
+    RED_HIP_TRY(timed(c, RED_PROF_STAGE, [&] { return launch_stage(S.np1.form, A, c->stream); }));
+    if (c->prm.adaptive && S.err_form >= 0) {
+        for (int r = S.n_main; r < S.n_stage; r++) {
+            if (S.extra_on_yout) {
+                RED_HIP_TRY(eval_forces(c, r, S.c[r], outr, outv, c->d_a[r]));
+            } else {
+                StageArgs B = make_args(c, S.rows[r], yr, yv, c->d_sr[r], c->d_sv[r], c->n);
+                RED_HIP_TRY(timed(c, RED_PROF_STAGE, [&] { return launch_stage(S.rows[r].form, B, c->stream); }));
+                RED_HIP_TRY(eval_forces(c, r, S.c[r], c->d_sr[r], c->d_sv[r], c->d_a[r]));
+            }
+        }
+        StageArgs E = make_args(c, S.err, yr, yv, nullptr, nullptr, n_err_bodies(c));
+        RED_HIP_TRY(timed(c, RED_PROF_ERR, [&] { return launch_err(S.err_form, E, c->stream); }));
+    }
+    return hipSuccess;
+}
+
+hipError_t enqueue_f0(red_ctx* c)
+{
+    return eval_forces(c, 0, c->S.c[0], c->d_r[c->cur], c->d_v[c->cur], c->d_a[0]);
+}
+
+// Captures fn into a graph for the current buffer parity on first use, then replays it.
+template <typename F>
+hipError_t run_maybe_graph(red_ctx* c, hipGraphExec_t* slot, F fn)
+{
+    if (!c->prm.use_graph) return fn(c);
+    if (!*slot) {
+        hipGraph_t g = nullptr;
+        RED_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = fn(c);
+        hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return e; }
+        RED_HIP_TRY(e2);
+        e = hipGraphInstantiate(slot, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        RED_HIP_TRY(e);
+    }
+    return hipGraphLaunch(*slot, c->stream);
+}
+
+int read_ctl(red_ctx* c)
+{
+    CTX_HIP(c, hipMemcpyAsync(c->h_ctl, c->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    return RED_OK;
+}
+
+double bits_to_double(unsigned long long b)
+{
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+}
+
+// Copies the pair events of the device buffer to the host in canonical order.
+int fetch_raw_events(red_ctx* c)
+{
+    if (c->raw_valid) return RED_OK;
+    int rc = read_ctl(c);
+    if (rc) return rc;
+    const int n = c->h_ctl->n_events;
+    if (n > c->ev_cap)
+        return c->fail(RED_ERR_EVENT_OVERFLOW, "pair events exceed the event capacity (" + std::to_string(n) + " > " +
+                                                   std::to_string(c->ev_cap) + ")");
+    c->raw.resize(n);
+    if (n > 0) {
+        CTX_HIP(c, hipMemcpy(c->raw.data(), c->d_events, sizeof(red_event) * n, hipMemcpyDeviceToHost));
+        std::stable_sort(c->raw.begin(), c->raw.end(), event_key_less);
+    }
+    c->raw_valid = true;
+    return RED_OK;
+}
+
+void increment_event_counter(red_ctx* c, int* cnt)  // pp_disk.cu:1059-1067
+{
+    for (int i = 0; i < 3; i++) { cnt[i]++; c->n_event[i]++; }
+}
+
+void set_event_counter(red_ctx* c, int field, int value)  // pp_disk.cu:1069-1076
+{
+    c->n_hit_centrum[field] = c->n_ejection[field] = c->n_collision[field] = c->n_close_encounter[field] =
+        c->n_event[field] = value;
+}
+
+// pp_disk.cu:1151-1202 on the host mirror; the device copy is updated from 'ups'.
+int handle_collision_pair(red_ctx* c, red_event* e, bool kernel_negated, std::vector<BodyUpdate>& ups)
+{
+    const int surv = e->idx1, merg = e->idx2;
+    if (c->h_type[merg] == BT_SUPERPL)
+        return c->fail(RED_ERR_UNSUPPORTED, "Collision between a massive body and a super-planetesimal is not yet implemented.");
+    const double p1[4] = {c->h_mass[surv], c->h_radius[surv], c->h_density[surv], c->h_cd[surv]};
+    const double p2[4] = {c->h_mass[merg], c->h_radius[merg], c->h_density[merg], c->h_cd[merg]};
+    memcpy(e->p1, p1, sizeof(p1));
+    memcpy(e->p2, p2, sizeof(p2));
+    tools::mass_weighted_mean(p1[0], p2[0], e->r1, e->r2, e->rs);
+    tools::mass_weighted_mean(p1[0], p2[0], e->v1, e->v2, e->vs);
+    tools::calc_physical_properties(p1, p2, e->ps);
+    c->h_mass[surv] = e->ps[0]; c->h_radius[surv] = e->ps[1]; c->h_density[surv] = e->ps[2]; c->h_cd[surv] = e->ps[3];
+    // merger: inactive, massless, far away (pp_disk.cu:1177-1189)
+    if (kernel_negated) c->h_id[merg] = -std::abs(c->h_id[merg]);
+    else c->h_id[merg] *= -1;
+    c->h_mass[merg] = 0.0; c->h_density[merg] = 0.0; c->h_radius[merg] = 0.0;
+    BodyUpdate us;
+    us.idx = surv; us.id = c->h_id[surv];
+    us.r[0] = e->rs[0]; us.r[1] = e->rs[1]; us.r[2] = e->rs[2]; us.r[3] = c->h_mass[surv];
+    us.v[0] = e->vs[0]; us.v[1] = e->vs[1]; us.v[2] = e->vs[2]; us.v[3] = 0.0;
+    us.mass = c->h_mass[surv]; us.radius = c->h_radius[surv];
+    BodyUpdate um;
+    um.idx = merg; um.id = c->h_id[merg];
+    um.r[0] = 1.0e20; um.r[1] = 0.0; um.r[2] = 0.0; um.r[3] = 0.0;
+    um.v[0] = um.v[1] = um.v[2] = um.v[3] = 0.0;
+    um.mass = 0.0; um.radius = 0.0;
+    ups.push_back(us);
+    ups.push_back(um);
+    return RED_OK;
+}
+
+int apply_updates(red_ctx* c, const std::vector<BodyUpdate>& ups)
+{
+    if (ups.empty()) return RED_OK;
+    if ((int)ups.size() > c->updates_cap) {
+        if (c->d_updates) (void)hipFree(c->d_updates);
+        c->updates_cap = (int)ups.size() * 2;
+        CTX_HIP(c, dmalloc(&c->d_updates, c->updates_cap));
+    }
+    CTX_HIP(c, hipMemcpyAsync(c->d_updates, ups.data(), sizeof(BodyUpdate) * ups.size(), hipMemcpyHostToDevice, c->stream));
+    CTX_HIP(c, launch_apply_updates(c->d_updates, (int)ups.size(), c->d_r[c->cur], c->d_v[c->cur], c->d_mass,
+                                    c->d_radius, c->d_id, c->stream));
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    refresh_pair_test(c);
+    return RED_OK;
+}
+
+// Astro-centric elements of a close encounter from the recorded stage states
+// (pp_disk.cu:526-561 with tools::transform_to_ac on the stage state).
+void close_encounter_elements(red_ctx* c, red_event* e)
+{
+    const double ms = mass_of_star(c);
+    const double mu1 = GAUSS2 * (ms + c->h_mass[e->idx1]);
+    const double mu2 = GAUSS2 * (ms + c->h_mass[e->idx2]);
+    double r1[3], v1[3], r2[3], v2[3];
+    for (int k = 0; k < 3; k++) {
+        r1[k] = e->r1[k] - e->rs[k];
+        r2[k] = e->r2[k] - e->rs[k];
+        v1[k] = (e->v1[k] - e->vs[k]) * GAUSS;
+        v2[k] = (e->v2[k] - e->vs[k]) * GAUSS;
+    }
+    tools::calculate_orbital_element(mu1, r1, v1, e->oe1);
+    tools::calculate_orbital_element(mu2, r2, v2, e->oe2);
+    const double p1[4] = {c->h_mass[e->idx1], c->h_radius[e->idx1], c->h_density[e->idx1], c->h_cd[e->idx1]};
+    const double p2[4] = {c->h_mass[e->idx2], c->h_radius[e->idx2], c->h_density[e->idx2], c->h_cd[e->idx2]};
+    memcpy(e->p1, p1, sizeof(p1));
+    memcpy(e->p2, p2, sizeof(p2));
+    for (int k = 0; k < 3; k++) { e->rs[k] = 0.0; e->vs[k] = 0.0; }
+}
+
+int upload_all(red_ctx* c)
+{
+    std::vector<d4> r(c->n), v(c->n);
+    // keep the caller's values: only the parameters and ids are re-uploaded here
+    (void)r; (void)v;
+    CTX_HIP(c, hipMemcpy(c->d_mass, c->h_mass.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
+    CTX_HIP(c, hipMemcpy(c->d_radius, c->h_radius.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
+    CTX_HIP(c, hipMemcpy(c->d_id, c->h_id.data(), sizeof(int) * c->n, hipMemcpyHostToDevice));
+    CTX_HIP(c, hipMemcpy(c->d_type, c->h_type.data(), sizeof(int) * c->n, hipMemcpyHostToDevice));
+    return RED_OK;
+}
+
+int download_state(red_ctx* c, std::vector<d4>& r, std::vector<d4>& v)
+{
+    r.resize(c->n); v.resize(c->n);
+    if (c->n == 0) return RED_OK;
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    CTX_HIP(c, hipMemcpy(r.data(), c->d_r[c->cur], sizeof(d4) * c->n, hipMemcpyDeviceToHost));
+    CTX_HIP(c, hipMemcpy(v.data(), c->d_v[c->cur], sizeof(d4) * c->n, hipMemcpyDeviceToHost));
+    return RED_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* red_version(void) { return "rednative 0.1 (gfx950)"; }
+
+const char* red_last_error(const red_ctx* ctx)
+{
+    if (!ctx) return "no context";
+    return ctx->err.c_str();
+}
+
+int red_ctx_destroy(red_ctx* c)
+{
+    if (!c) return RED_OK;
+    destroy_graphs(c);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 2; k++) { (void)hipFree(c->d_r[k]); (void)hipFree(c->d_v[k]); }
+    for (auto p : c->d_sr) (void)hipFree(p);
+    for (auto p : c->d_sv) (void)hipFree(p);
+    for (auto p : c->d_a) (void)hipFree(p);
+    (void)hipFree(c->d_mass); (void)hipFree(c->d_radius); (void)hipFree(c->d_id); (void)hipFree(c->d_type);
+    (void)hipFree(c->d_ctl); (void)hipFree(c->d_events); (void)hipFree(c->d_events_hc); (void)hipFree(c->d_scratch);
+    (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RED_OK;
+}
+
+int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
+{
+    if (!b || !p || !out) return RED_ERR_ARG;
+    *out = nullptr;
+    red_ctx* c = new red_ctx();
+    c->prm = *p;
+    int n = 0;
+    for (int k = 0; k < 7; k++) {
+        if (b->counts[k] < 0) { delete c; return RED_ERR_ARG; }
+        c->cnt[k] = b->counts[k];
+        n += b->counts[k];
+    }
+    if (p->integrator < RED_EULER || p->integrator > RED_RKF78) { delete c; return RED_ERR_UNSUPPORTED; }
+    if (p->frame_center != RED_FRAME_BARY) { delete c; return RED_ERR_UNSUPPORTED; }
+    if (!p->inner_steps && (p->radii_enhance_factor > 0.0 || p->encounter_hill_factor > 0.0)) { delete c; return RED_ERR_UNSUPPORTED; }
+    if (p->precision == RED_PREC_MIXED && p->encounter_hill_factor > 0.0) { delete c; return RED_ERR_UNSUPPORTED; }
+    if (n > 0 && (!b->id || !b->type || !b->mass || !b->radius || !b->density || !b->cd || !b->r || !b->v)) {
+        delete c;
+        return RED_ERR_ARG;
+    }
+    c->n = n;
+    c->cap = std::max(64, (n + 63) / 64 * 64);
+    c->h_mass.assign(b->mass, b->mass + n);
+    c->h_radius.assign(b->radius, b->radius + n);
+    c->h_density.assign(b->density, b->density + n);
+    c->h_cd.assign(b->cd, b->cd + n);
+    c->h_id.assign(b->id, b->id + n);
+    c->h_type.assign(b->type, b->type + n);
+    c->h_mig_type.assign(n, 0);
+    c->h_mig_stop.assign(n, 0.0);
+    c->h_epoch.assign(n, 0.0);
+    if (b->mig_type) c->h_mig_type.assign(b->mig_type, b->mig_type + n);
+    if (b->mig_stop_at) c->h_mig_stop.assign(b->mig_stop_at, b->mig_stop_at + n);
+    if (b->epoch) c->h_epoch.assign(b->epoch, b->epoch + n);
+    if (n > 0 && mass_of_star(c) < 0.0 && (p->encounter_hill_factor > 0.0 || p->close_encounter)) {
+        delete c;
+        return RED_ERR_NO_STAR;
+    }
+    c->S = make_scheme(p->integrator, p->adaptive != 0);
+    c->dt_try = p->dt_day * GAUSS;   // integrator.cu:29
+    c->t = 0.0;
+    // launch configuration
+    c->gcfg.precision = p->precision;
+    c->gcfg.n_fp64_src = c->cnt[0] + c->cnt[1];
+    c->gcfg.block = p->block > 0 ? p->block : 256;
+    c->gcfg.spt = p->sinks_per_thread > 0 ? p->sinks_per_thread : (p->precision == RED_PREC_MIXED ? 2 : 1);
+    c->gcfg.strategy = p->strategy > 0 ? p->strategy : 1;
+    c->gcfg.kahan_pairs = p->kahan_pairs;
+    c->gcfg.split_max_sinks = 2048;
+    refresh_pair_test(c);
+
+    hipError_t e = hipSetDevice(p->device);
+    if (e != hipSuccess) { int rc = c->hipfail(e, "hipSetDevice"); *out = c; return rc; }
+    *out = c;
+    CTX_HIP(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const int cap = c->cap;
+    for (int k = 0; k < 2; k++) {
+        CTX_HIP(c, dmalloc(&c->d_r[k], cap));
+        CTX_HIP(c, dmalloc(&c->d_v[k], cap));
+        CTX_HIP(c, hipMemset(c->d_r[k], 0, sizeof(d4) * cap));
+        CTX_HIP(c, hipMemset(c->d_v[k], 0, sizeof(d4) * cap));
+    }
+    const int ns = c->S.n_stage;
+    c->d_sr.assign(ns, nullptr); c->d_sv.assign(ns, nullptr); c->d_a.assign(ns, nullptr);
+    for (int k = 0; k < ns; k++) {
+        CTX_HIP(c, dmalloc(&c->d_a[k], cap));
+        CTX_HIP(c, hipMemset(c->d_a[k], 0, sizeof(d4) * cap));
+        if (k >= 1) {
+            CTX_HIP(c, dmalloc(&c->d_sr[k], cap));
+            CTX_HIP(c, dmalloc(&c->d_sv[k], cap));
+        }
+    }
+    CTX_HIP(c, dmalloc(&c->d_mass, cap));
+    CTX_HIP(c, dmalloc(&c->d_radius, cap));
+    CTX_HIP(c, dmalloc(&c->d_id, cap));
+    CTX_HIP(c, dmalloc(&c->d_type, cap));
+    CTX_HIP(c, dmalloc(&c->d_ctl, 1));
+    CTX_HIP(c, hipMemset(c->d_ctl, 0, sizeof(Ctl)));
+    CTX_HIP(c, hipHostMalloc((void**)&c->h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    memset(c->h_ctl, 0, sizeof(Ctl));
+    c->ev_cap = p->event_capacity > 0 ? p->event_capacity : std::max(4096, n);
+    CTX_HIP(c, dmalloc(&c->d_events, c->ev_cap));
+    CTX_HIP(c, dmalloc(&c->d_events_hc, std::max(1, n)));
+    if (p->precision == RED_PREC_MIXED) CTX_HIP(c, dmalloc(&c->d_src_mixed, cap));
+    CTX_HIP(c, dmalloc(&c->d_energy, 6 * (size_t)cap));
+    // split-sweep scratch: the largest few-sink sweep
+    {
+        const int nsi = c->nSI(), nm = c->nMassive();
+        size_t need = 0;
+        Sweep sw[3] = {{0, nsi, 0, nm}, {nsi, nm, 0, nsi}, {nm, n, 0, nm}};
+        for (auto& s : sw) {
+            const int ns_ = s.sink1 - s.sink0;
+            if (ns_ > 0 && ns_ <= c->gcfg.split_max_sinks && s.src1 - s.src0 >= 1024)
+                need = std::max(need, split_scratch_bytes(s));
+        }
+        const size_t limit = (size_t)4 << 30;
+        if (need > limit) need = 0;   // too large: those sweeps use the broadcast path
+        if (need > 0) CTX_HIP(c, hipMalloc(&c->d_scratch, need));
+        c->scratch_bytes = need;
+    }
+    // upload the state (w of r = mass)
+    {
+        std::vector<d4> r(n), v(n);
+        for (int i = 0; i < n; i++) {
+            r[i] = d4{b->r[3 * i], b->r[3 * i + 1], b->r[3 * i + 2], b->mass[i]};
+            v[i] = d4{b->v[3 * i], b->v[3 * i + 1], b->v[3 * i + 2], 0.0};
+        }
+        if (n > 0) {
+            CTX_HIP(c, hipMemcpy(c->d_r[0], r.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+            CTX_HIP(c, hipMemcpy(c->d_v[0], v.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+        }
+    }
+    int rc = upload_all(c);
+    if (rc) return rc;
+    CTX_HIP(c, hipDeviceSynchronize());
+    return RED_OK;
+}
+
+int red_get_counts(red_ctx* c, int counts[7])
+{
+    if (!c || !counts) return RED_ERR_ARG;
+    for (int k = 0; k < 7; k++) counts[k] = c->cnt[k];
+    return RED_OK;
+}
+
+int red_get_time(red_ctx* c, double* t) { if (!c || !t) return RED_ERR_ARG; *t = c->t; return RED_OK; }
+int red_set_time(red_ctx* c, double t) { if (!c) return RED_ERR_ARG; c->t = t; return RED_OK; }
+int red_get_dt(red_ctx* c, double* dt) { if (!c || !dt) return RED_ERR_ARG; *dt = c->dt_try; return RED_OK; }
+int red_set_dt(red_ctx* c, double dt) { if (!c) return RED_ERR_ARG; c->dt_try = dt; return RED_OK; }
+
+int red_get_state(red_ctx* c, double* r, double* v, double* mass, double* radius, double* density, double* cd,
+                  int* id, int* type)
+{
+    if (!c) return RED_ERR_ARG;
+    std::vector<d4> R, V;
+    if (r || v) {
+        int rc = download_state(c, R, V);
+        if (rc) return rc;
+    }
+    for (int i = 0; i < c->n; i++) {
+        if (r) { r[3 * i] = R[i].x; r[3 * i + 1] = R[i].y; r[3 * i + 2] = R[i].z; }
+        if (v) { v[3 * i] = V[i].x; v[3 * i + 1] = V[i].y; v[3 * i + 2] = V[i].z; }
+        if (mass) mass[i] = c->h_mass[i];
+        if (radius) radius[i] = c->h_radius[i];
+        if (density) density[i] = c->h_density[i];
+        if (cd) cd[i] = c->h_cd[i];
+        if (id) id[i] = c->h_id[i];
+        if (type) type[i] = c->h_type[i];
+    }
+    return RED_OK;
+}
+
+int red_set_state(red_ctx* c, const double* r, const double* v)
+{
+    if (!c || !r || !v) return RED_ERR_ARG;
+    std::vector<d4> R(c->n), V(c->n);
+    for (int i = 0; i < c->n; i++) {
+        R[i] = d4{r[3 * i], r[3 * i + 1], r[3 * i + 2], c->h_mass[i]};
+        V[i] = d4{v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0};
+    }
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->n > 0) {
+        CTX_HIP(c, hipMemcpy(c->d_r[c->cur], R.data(), sizeof(d4) * c->n, hipMemcpyHostToDevice));
+        CTX_HIP(c, hipMemcpy(c->d_v[c->cur], V.data(), sizeof(d4) * c->n, hipMemcpyHostToDevice));
+    }
+    return RED_OK;
+}
+
+int red_state_device(red_ctx* c, double** r, double** v)
+{
+    if (!c) return RED_ERR_ARG;
+    if (r) *r = (double*)c->d_r[c->cur];
+    if (v) *v = (double*)c->d_v[c->cur];
+    return RED_OK;
+}
+
+int red_sync(red_ctx* c)
+{
+    if (!c) return RED_ERR_ARG;
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    return RED_OK;
+}
+
+int red_accel(red_ctx* c, double t, int stage, const double* r_dev, const double* v_dev, double* a_dev)
+{
+    if (!c || !r_dev || !v_dev || !a_dev) return RED_ERR_ARG;
+    hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, c->stream, c->d_ctl, t, 0.0, 0, 1, 0, 0);
+    CTX_HIP(c, hipGetLastError());
+    c->raw_valid = false;
+    CTX_HIP(c, eval_forces(c, stage, 0.0, (const d4*)r_dev, (const d4*)v_dev, (d4*)a_dev));
+    return RED_OK;
+}
+
+int red_step(red_ctx* c, red_step_info* info)
+{
+    if (!c) return RED_ERR_ARG;
+    const Scheme& S = c->S;
+    const bool adaptive = c->prm.adaptive && S.err_form >= 0;
+    // f0 = f(t, y_n) (int_rungekutta8.cu:715-724)
+    hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, c->stream, c->d_ctl, c->t, c->dt_try, 0, 1, 0, 0);
+    CTX_HIP(c, hipGetLastError());
+    CTX_HIP(c, run_maybe_graph(c, &c->g_f0[c->cur], enqueue_f0));
+    c->raw_valid = false;
+    double max_err = 0.0;
+    int iter = 0;
+    bool done = false;
+    while (!done) {
+        if (iter >= c->attempts_cap)
+            return c->fail(RED_ERR_STEP_LIMIT, "step-size control did not converge within " + std::to_string(iter) + " attempts");
+        c->dt_did = c->dt_try;
+        hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, c->stream, c->d_ctl, c->t, c->dt_try, iter, 1,
+                           S.clear_events_each_attempt ? 1 : 0, 0);
+        CTX_HIP(c, hipGetLastError());
+        CTX_HIP(c, run_maybe_graph(c, &c->g_attempt[c->cur], enqueue_attempt));
+        if (!adaptive) {
+            iter++;
+            break;
+        }
+        int rc = read_ctl(c);
+        if (rc) return rc;
+        if (c->h_ctl->nan_flag) return c->fail(RED_ERR_NAN, "non-finite error estimate");
+        if (c->h_ctl->n_events > c->ev_cap)
+            return c->fail(RED_ERR_EVENT_OVERFLOW, "pair events exceed the event capacity");
+        // integrator.cu:255 and the controllers (int_rungekutta8.cu:776-777 etc.)
+        max_err = std::fabs(c->dt_try * S.lambda * bits_to_double(c->h_ctl->err_bits));
+        c->last_err = max_err;
+        c->dt_try *= 0.9 * std::pow(c->prm.tolerance / max_err, S.exponent);
+        if (S.accept_on_event && c->h_ctl->n_events > 0) {
+            if (c->dt_try < c->dt_did) c->dt_try = c->dt_did;
+            break;   // leaves before iter++ as int_rungekutta4.cu:255-264 does
+        }
+        iter++;
+        done = !(max_err > c->prm.tolerance);
+    }
+    // integrator.cu:258-263
+    c->n_tried += iter;
+    c->n_failed += (iter - 1);
+    c->n_passed++;
+    c->t += c->dt_did;
+    c->cur = 1 - c->cur;   // pp_disk::swap (pp_disk.cu:1043-1057)
+    if (info) {
+        info->dt_did = c->dt_did;
+        info->dt_next = c->dt_try;
+        info->err = max_err;
+        info->tries = iter;
+        info->n_events = adaptive ? c->h_ctl->n_events : -1;
+    }
+    return RED_OK;
+}
+
+int red_clear_events(red_ctx* c)
+{
+    if (!c) return RED_ERR_ARG;
+    hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, c->stream, c->d_ctl, c->t, c->dt_try, 0, 0, 1, 1);
+    CTX_HIP(c, hipGetLastError());
+    c->raw.clear();
+    c->raw_valid = true;
+    return RED_OK;
+}
+
+int red_raw_events(red_ctx* c, red_event* buf, int cap, int* n, int* overflow)
+{
+    if (!c) return RED_ERR_ARG;
+    int rc = read_ctl(c);
+    if (rc) return rc;
+    const int nd = c->h_ctl->n_events;
+    if (overflow) *overflow = nd > c->ev_cap;
+    if (nd > c->ev_cap) {
+        if (n) *n = nd;
+        return RED_OK;
+    }
+    rc = fetch_raw_events(c);
+    if (rc) return rc;
+    if (n) *n = (int)c->raw.size();
+    for (int k = 0; k < (int)c->raw.size() && k < cap && buf; k++) buf[k] = c->raw[k];
+    return RED_OK;
+}
+
+int red_sp_events(red_ctx* c, red_event* buf, int cap, int* n)
+{
+    if (!c) return RED_ERR_ARG;
+    if (n) *n = (int)c->sp.size();
+    for (int k = 0; k < (int)c->sp.size() && k < cap && buf; k++) buf[k] = c->sp[k];
+    return RED_OK;
+}
+
+// pp_disk.cu:683-715, 934-996, 1091-1109
+int red_check_ejection_hit_centrum(red_ctx* c, int* handled)
+{
+    if (!c) return RED_ERR_ARG;
+    if (handled) *handled = 0;
+    const double ej = c->prm.ejection, hc = c->prm.hit_centrum;
+    if (!(ej > 0.0) && !(hc > 0.0)) return RED_OK;
+    hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, c->stream, c->d_ctl, c->t, c->dt_try, 0, 0, 0, 1);
+    CTX_HIP(c, hipGetLastError());
+    CTX_HIP(c, launch_ejection_scan(c->d_r[c->cur], c->d_v[c->cur], c->d_id, c->d_type, c->n, ej, hc, c->d_ctl,
+                                    c->d_events_hc, std::max(1, c->n), c->stream));
+    int rc = read_ctl(c);
+    if (rc) return rc;
+    const int ne = c->h_ctl->n_events_hc;
+    if (ne <= 0) return RED_OK;
+    std::vector<red_event> ev(ne);
+    CTX_HIP(c, hipMemcpy(ev.data(), c->d_events_hc, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
+    std::stable_sort(ev.begin(), ev.end(), [](const red_event& a, const red_event& b) { return a.sink < b.sink; });
+    for (auto& e : ev) c->h_id[e.idx2] = -c->h_id[e.idx2];   // mirror the kernel's negation
+    std::vector<BodyUpdate> ups;
+    c->sp = ev;
+    for (auto& e : c->sp) {
+        if (e.name == RED_EV_EJECTION) {
+            const int i = e.idx2;
+            const double p0[4] = {c->h_mass[0], c->h_radius[0], c->h_density[0], c->h_cd[0]};
+            const double pi[4] = {c->h_mass[i], c->h_radius[i], c->h_density[i], c->h_cd[i]};
+            memcpy(e.p1, p0, sizeof(p0)); memcpy(e.p2, pi, sizeof(pi));
+            memcpy(e.rs, e.r1, sizeof(e.rs)); memcpy(e.vs, e.v1, sizeof(e.vs)); memcpy(e.ps, e.p1, sizeof(e.ps));
+            increment_event_counter(c, c->n_ejection);
+        } else {
+            rc = handle_collision_pair(c, &e, true, ups);
+            if (rc) return rc;
+            increment_event_counter(c, c->n_hit_centrum);
+        }
+    }
+    // ids negated by the kernel are already on the device; updates carry the merge results
+    rc = apply_updates(c, ups);
+    if (rc) return rc;
+    c->n_ejection[CNT_LAST_STEP] = 0;
+    c->n_hit_centrum[CNT_LAST_STEP] = 0;
+    if (handled) *handled = 1;
+    return RED_OK;
+}
+
+// pp_disk.cu:717-738, 1078-1149 (close-encounter records of the Hill test are reported, not merged)
+int red_check_collision(red_ctx* c, int* handled)
+{
+    if (!c) return RED_ERR_ARG;
+    if (handled) *handled = 0;
+    int rc = fetch_raw_events(c);
+    if (rc) return rc;
+    if (c->raw.empty()) return RED_OK;
+    // create_sp_events: dedupe by (id1, id2), keep the earliest t, first on ties
+    std::vector<red_event>& ev = c->raw;
+    const int ec = (int)ev.size();
+    std::vector<char> processed(ec, 0);
+    c->sp.clear();
+    for (int k = 0; k < ec; k++) {
+        if (processed[k]) continue;
+        processed[k] = 1;
+        red_event cur = ev[k];
+        for (int i = k + 1; i < ec; i++) {
+            if (cur.id1 == ev[i].id1 && cur.id2 == ev[i].id2) {
+                processed[i] = 1;
+                if (cur.t > ev[i].t) cur = ev[i];
+            }
+        }
+        c->sp.push_back(cur);
+    }
+    std::vector<BodyUpdate> ups;
+    for (auto& e : c->sp) {
+        if (e.name == RED_EV_COLLISION) {
+            rc = handle_collision_pair(c, &e, false, ups);
+            if (rc) return rc;
+            increment_event_counter(c, c->n_collision);
+        } else {
+            close_encounter_elements(c, &e);
+            increment_event_counter(c, c->n_close_encounter);
+        }
+    }
+    rc = apply_updates(c, ups);
+    if (rc) return rc;
+    c->n_collision[CNT_LAST_STEP] = 0;
+    c->n_close_encounter[CNT_LAST_STEP] = 0;
+    if (handled) *handled = 1;
+    return RED_OK;
+}
+
+// pp_disk.cu:740-770 (inner_steps == true)
+int red_check_close_encounter(red_ctx* c, int* handled)
+{
+    if (!c) return RED_ERR_ARG;
+    if (handled) *handled = 0;
+    int rc = fetch_raw_events(c);
+    if (rc) return rc;
+    if (c->raw.empty()) return RED_OK;
+    c->sp = c->raw;
+    for (auto& e : c->sp) {
+        close_encounter_elements(c, &e);
+        increment_event_counter(c, c->n_close_encounter);
+    }
+    c->n_close_encounter[CNT_LAST_STEP] = 0;
+    if (handled) *handled = 1;
+    return RED_OK;
+}
+
+// pp_disk.cu:893-907, 1324-1409; number_of_bodies.cpp:27-104
+int red_check_rebuild(red_ctx* c, int threshold, int* rebuilt)
+{
+    if (!c) return RED_ERR_ARG;
+    if (rebuilt) *rebuilt = 0;
+    if (!(c->n_event[CNT_LAST_CLEAR] >= threshold)) return RED_OK;
+    std::vector<d4> R, V;
+    int rc = download_state(c, R, V);
+    if (rc) return rc;
+    int removed[7] = {0};
+    c->rebuild_map.clear();
+    for (int i = 0; i < c->n; i++) {
+        if (0 < c->h_id[i] && BT_PAD > c->h_type[i]) { c->rebuild_map.push_back(i); continue; }
+        const int t = c->h_type[i];
+        if (t < 0 || t > BT_PAD) return c->fail(RED_ERR_ARG, "Undefined body type!");
+        if (t == BT_PAD) continue;
+        removed[t]++;
+        c->n_inactive[t]++;
+    }
+    for (int k = 0; k < 7; k++) c->cnt[k] -= removed[k];
+    const int m = (int)c->rebuild_map.size();
+    int nn = 0;
+    for (int k = 0; k < 7; k++) nn += c->cnt[k];
+    if (nn != m) return c->fail(RED_ERR_ARG, "Error: number of copied bodies does not equal to the number of active bodies.");
+    std::vector<d4> R2(m), V2(m);
+    for (int k = 0; k < m; k++) {
+        const int i = c->rebuild_map[k];
+        R2[k] = R[i]; V2[k] = V[i];
+        c->h_mass[k] = c->h_mass[i]; c->h_radius[k] = c->h_radius[i]; c->h_density[k] = c->h_density[i];
+        c->h_cd[k] = c->h_cd[i]; c->h_id[k] = c->h_id[i]; c->h_type[k] = c->h_type[i];
+        c->h_epoch[k] = c->h_epoch[i]; c->h_mig_type[k] = c->h_mig_type[i]; c->h_mig_stop[k] = c->h_mig_stop[i];
+    }
+    c->n = m;
+    for (auto* vec : {&c->h_mass, &c->h_radius, &c->h_density, &c->h_cd, &c->h_epoch, &c->h_mig_stop}) vec->resize(m);
+    for (auto* vec : {&c->h_id, &c->h_type, &c->h_mig_type}) vec->resize(m);
+    if (m > 0) {
+        CTX_HIP(c, hipMemcpy(c->d_r[c->cur], R2.data(), sizeof(d4) * m, hipMemcpyHostToDevice));
+        CTX_HIP(c, hipMemcpy(c->d_v[c->cur], V2.data(), sizeof(d4) * m, hipMemcpyHostToDevice));
+    }
+    rc = upload_all(c);
+    if (rc) return rc;
+    destroy_graphs(c);
+    c->gcfg.n_fp64_src = c->cnt[0] + c->cnt[1];
+    refresh_pair_test(c);
+    set_event_counter(c, CNT_LAST_CLEAR, 0);
+    if (rebuilt) *rebuilt = 1;
+    return RED_OK;
+}
+
+int red_last_rebuild_map(red_ctx* c, int* old_index, int cap, int* n)
+{
+    if (!c) return RED_ERR_ARG;
+    if (n) *n = (int)c->rebuild_map.size();
+    for (int k = 0; k < (int)c->rebuild_map.size() && k < cap && old_index; k++) old_index[k] = c->rebuild_map[k];
+    return RED_OK;
+}
+
+int red_counters(red_ctx* c, int out[8])
+{
+    if (!c || !out) return RED_ERR_ARG;
+    out[0] = c->n_tried; out[1] = c->n_failed; out[2] = c->n_passed;
+    out[3] = c->n_collision[CNT_TOTAL]; out[4] = c->n_ejection[CNT_TOTAL]; out[5] = c->n_hit_centrum[CNT_TOTAL];
+    out[6] = c->n_close_encounter[CNT_TOTAL]; out[7] = c->n_event[CNT_LAST_CLEAR];
+    return RED_OK;
+}
+
+int red_energy(red_ctx* c, double* E, double L[3])
+{
+    if (!c) return RED_ERR_ARG;
+    const int nm = c->nMassive();
+    std::vector<double> o(6 * (size_t)std::max(nm, 1));
+    CTX_HIP(c, launch_energy(c->d_r[c->cur], c->d_v[c->cur], c->d_mass, c->d_id, nm, c->nSI(), c->d_energy, c->stream));
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    if (nm > 0) CTX_HIP(c, hipMemcpy(o.data(), c->d_energy, sizeof(double) * 6 * nm, hipMemcpyDeviceToHost));
+    double T = 0.0, U = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
+    for (int i = 0; i < nm; i++) {
+        T += o[6 * i]; U += o[6 * i + 1];
+        lx += o[6 * i + 2]; ly += o[6 * i + 3]; lz += o[6 * i + 4];
+    }
+    if (E) *E = T - U;
+    if (L) { L[0] = lx; L[1] = ly; L[2] = lz; }
+    return RED_OK;
+}
+
+int red_set_profiling(red_ctx* c, int on)
+{
+    if (!c) return RED_ERR_ARG;
+    if (on && c->prm.use_graph) return c->fail(RED_ERR_ARG, "profiling needs use_graph = 0");
+    c->profiling = on != 0;
+    return RED_OK;
+}
+
+int red_get_profile(red_ctx* c, double ms[RED_PROF_CLASSES], int launches[RED_PROF_CLASSES], int reset)
+{
+    if (!c) return RED_ERR_ARG;
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    for (auto& p : c->prof) {
+        float t = 0.0f;
+        CTX_HIP(c, hipEventElapsedTime(&t, p.a, p.b));
+        c->prof_ms[p.cls] += t;
+        c->prof_n[p.cls] += 1;
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    c->prof.clear();
+    for (int k = 0; k < RED_PROF_CLASSES; k++) {
+        if (ms) ms[k] = c->prof_ms[k];
+        if (launches) launches[k] = c->prof_n[k];
+        if (reset) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    }
+    return RED_OK;
+}
+
+int red_host_transform_to_bc(int n, const double* mass, double* r, double* v)
+{
+    if (n < 0 || (n > 0 && (!mass || !r || !v))) return RED_ERR_ARG;
+    double R0[3] = {0, 0, 0}, V0[3] = {0, 0, 0};
+    for (int j = n - 1; j >= 0; j--) {
+        const double m = mass[j];
+        for (int k = 0; k < 3; k++) { R0[k] += m * r[3 * j + k]; V0[k] += m * v[3 * j + k]; }
+    }
+    double M0 = 0.0;
+    for (int j = n - 1; j >= 0; j--) M0 += mass[j];
+    for (int k = 0; k < 3; k++) { R0[k] /= M0; V0[k] /= M0; }
+    for (int j = n - 1; j >= 0; j--)
+        for (int k = 0; k < 3; k++) { r[3 * j + k] -= R0[k]; v[3 * j + k] -= V0[k]; }
+    return RED_OK;
+}
+
+int red_host_transform_velocity(int n, double* v)
+{
+    if (n < 0 || (n > 0 && !v)) return RED_ERR_ARG;
+    for (int j = 0; j < 3 * n; j++) v[j] /= GAUSS;
+    return RED_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,126 @@
+// kern_events.hip -- ejection / hit-centrum scan, merge scatter, energy partials (gfx950).
+//
+// Ejection and hit-centrum replace kernel_pp_disk::check_for_ejection_hit_centrum
+// (red.cuda/src/red.cuda/pp_disk.cu:72-142): for every active non-star body,
+// |r|^2 > R_ej^2 (with R_ej > 0) is an EJECTION, otherwise |r|^2 < R_hc^2 (R_hc > 0) a
+// HIT_CENTRUM; the body's id is negated.  Distances are taken from the frame origin as
+// in the reference.  Events are appended atomically and put into body order by the host.
+//
+// The merge scatter applies the host-computed results of pp_disk::handle_collision_pair
+// (pp_disk.cu:1151-1202) in one launch instead of seven small copies per pair.
+//
+// The energy kernel is not in the reference: per massive body it writes
+// (1/2 m v^2, sum_{j>i, interacting} m_i m_j / r_ij, m r x v) for a fixed-order host sum.
+#include "red_internal.hpp"
+
+namespace red {
+namespace {
+
+__global__ __launch_bounds__(256) void k_ejection(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                  int* __restrict__ id, const int* __restrict__ type, int n,
+                                                  double ej, double hc, Ctl* __restrict__ ctl,
+                                                  red_event* __restrict__ buf, int cap)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!(id[i] > 0 && type[i] != 0)) return;
+    const d4 q = r[i];
+    const double r2 = (q.x * q.x) + (q.y * q.y) + (q.z * q.z);
+    int name = RED_EV_NONE;
+    if (0.0 < ej && ej * ej < r2) name = RED_EV_EJECTION;
+    else if (0.0 < hc && hc * hc > r2) name = RED_EV_HIT_CENTRUM;
+    if (name == RED_EV_NONE) return;
+    const int k = atomicAdd(&ctl->n_events_hc, 1);
+    if (k < cap) {
+        red_event* e = buf + k;
+        e->name = name;
+        e->id1 = id[0]; e->idx1 = 0;
+        e->id2 = id[i]; e->idx2 = i;
+        e->attempt = -1; e->stage = -1; e->sink = i; e->source = 0; e->pad = 0;
+        e->t = ctl->t;
+        e->d = sqrt(r2);
+        const d4 r0 = r[0], v0 = v[0], vi = v[i];
+        e->r1[0] = r0.x; e->r1[1] = r0.y; e->r1[2] = r0.z;
+        e->v1[0] = v0.x; e->v1[1] = v0.y; e->v1[2] = v0.z;
+        e->r2[0] = q.x; e->r2[1] = q.y; e->r2[2] = q.z;
+        e->v2[0] = vi.x; e->v2[1] = vi.y; e->v2[2] = vi.z;
+        for (int c = 0; c < 4; c++) { e->p1[c] = 0.0; e->p2[c] = 0.0; e->ps[c] = 0.0; }
+        for (int c = 0; c < 3; c++) { e->rs[c] = 0.0; e->vs[c] = 0.0; }
+        for (int c = 0; c < 6; c++) { e->oe1[c] = 0.0; e->oe2[c] = 0.0; e->oes[c] = 0.0; }
+    }
+    id[i] = -id[i];
+}
+
+__global__ void k_apply_updates(const BodyUpdate* __restrict__ u, int nu, d4* __restrict__ r, d4* __restrict__ v,
+                                double* __restrict__ mass, double* __restrict__ radius, int* __restrict__ id)
+{
+    // Updates are applied in list order by a single work-item: a body may appear twice
+    // (survivor of one pair, merger of the next) and the later write must win.
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int k = 0; k < nu; k++) {
+        const BodyUpdate& b = u[k];
+        r[b.idx] = d4{b.r[0], b.r[1], b.r[2], b.r[3]};
+        v[b.idx] = d4{b.v[0], b.v[1], b.v[2], 0.0};
+        mass[b.idx] = b.mass;
+        radius[b.idx] = b.radius;
+        id[b.idx] = b.id;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_energy(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                const double* __restrict__ mass, const int* __restrict__ id,
+                                                int n_massive, int n_si, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_massive) return;
+    double* o = out + 6 * (size_t)i;
+    if (id[i] <= 0) {
+        for (int c = 0; c < 6; c++) o[c] = 0.0;
+        return;
+    }
+    const d4 q = r[i], w = v[i];
+    const double m = mass[i];
+    double U = 0.0;
+    const int jend = i < n_si ? n_massive : n_si;   // NSI-NSI pairs do not interact
+    for (int j = i + 1; j < jend; j++) {
+        if (id[j] <= 0) continue;
+        const d4 p = r[j];
+        const double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        U += m * mass[j] / sqrt((dx * dx) + (dy * dy) + (dz * dz));
+    }
+    o[0] = 0.5 * m * ((w.x * w.x) + (w.y * w.y) + (w.z * w.z));
+    o[1] = U;
+    o[2] = m * (q.y * w.z - q.z * w.y);
+    o[3] = m * (q.z * w.x - q.x * w.z);
+    o[4] = m * (q.x * w.y - q.y * w.x);
+    o[5] = 0.0;
+}
+
+}  // namespace
+
+hipError_t launch_ejection_scan(const d4* r, const d4* v, int* id, const int* type, int n, double ej, double hc,
+                                Ctl* ctl, red_event* buf, int cap, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ejection, dim3((n + 255) / 256), dim3(256), 0, s, r, v, id, type, n, ej, hc, ctl, buf, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_apply_updates(const BodyUpdate* u, int nu, d4* r, d4* v, double* mass, double* radius, int* id,
+                                hipStream_t s)
+{
+    if (nu <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_apply_updates, dim3(1), dim3(64), 0, s, u, nu, r, v, mass, radius, id);
+    return hipGetLastError();
+}
+
+hipError_t launch_energy(const d4* r, const d4* v, const double* mass, const int* id, int n_massive, int n_si,
+                         double* out6, hipStream_t s)
+{
+    if (n_massive <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_energy, dim3((n_massive + 255) / 256), dim3(256), 0, s, r, v, mass, id, n_massive, n_si,
+                       out6);
+    return hipGetLastError();
+}
+
+}  // namespace red

@@ -1,0 +1,567 @@
+// kern_gravity.hip -- all-pairs gravitational acceleration with fused pair tests (gfx950).
+//
+// Replaces kernel_pp_disk::calc_grav_accel and its padded twin
+// (red.cuda/src/red.cuda/pp_disk.cu:144-296) and the launcher
+// pp_disk::call_kernel_calc_grav_accel (pp_disk.cu:609-681).
+//
+// Per sweep (SI x massive, NSI x SI, NI x massive) one of three paths runs:
+//
+//  * broadcast sweep: a work-item owns SPT sinks, the j loop is wave-uniform so every
+//    source record {x, y, z, m} arrives through one 32-byte scalar load into SGPRs and
+//    feeds all 64 lanes as a scalar operand -- no LDS traffic and no VGPRs for sources.
+//  * LDS-tile sweep: the classic i-block x j-tile scheme, sources staged in LDS per tile
+//    (kept as an autotuner alternative).
+//  * split sweep for sweeps with few sinks (the star and giants against the whole disk):
+//    pair terms for all (sink, source) pairs in parallel, then per-sink sums.  In FP64
+//    mode the sum runs sequentially in ascending j, so the result is bit-identical to
+//    the reference loop; in mixed mode it is chunked (deterministic, not sequential).
+//
+// FP64 mode keeps the reference's operation order with no fused multiply-adds
+// (build with -ffp-contract=off): dx = xj - xi; r2 = (dx*dx + dy*dy) + dz*dz;
+// d = sqrt(r2); s = m / (d*r2); a += s*dx (pp_disk.cu:247-260).  sqrt and '/' are
+// correctly rounded on gfx950, so the accelerations equal the CPU oracle's bit for bit.
+//
+// Mixed mode: FP64 position differences, FP32 r^-3 via v_rsq_f32, FP32 partial sums per
+// source chunk folded into FP64 accumulators with Kahan compensation (optionally a
+// Kahan-compensated FP32 sum per pair).  Sources [0, n_fp64_src) (the star and giant
+// planets) stay in the FP64 path.  Pair tests stay exact: a conservative FP32 screen on
+// r^2 selects candidates and the reference predicate d < f (R_i + R_j) is re-evaluated
+// in FP64 for them, so event lists equal the FP64 path's for a given state.
+#include "red_internal.hpp"
+
+namespace red {
+namespace {
+
+constexpr int CHUNK = 64;   // mixed sweep: sources per FP32 partial sum
+
+__device__ __forceinline__ void put3(double* d, const d4& q) { d[0] = q.x; d[1] = q.y; d[2] = q.z; }
+
+// Mutual Hill radius helper: cube root by bit-seeded Newton iterations using only
+// +, -, *, / so that host and device produce identical results.
+__device__ __forceinline__ double cbrt_ieee(double x)
+{
+    if (!(x > 0.0)) return 0.0;
+    unsigned long long b = __double_as_longlong(x);
+    unsigned long long e = (b >> 52) & 0x7ff;
+    long long eb = (long long)e - 1023;
+    long long q = eb >= 0 ? eb / 3 : -((-eb + 2) / 3);
+    double y = __longlong_as_double((unsigned long long)(q + 1023) << 52);
+    for (int k = 0; k < 60; k++) {
+        double yn = y - (y * y * y - x) / (3.0 * y * y);
+        if (yn == y) break;
+        y = yn;
+    }
+    return y;
+}
+
+// Appends one pair event (pp_disk.cu:32-67, 265-291).
+__device__ void append_pair_event(const PairTest& P, const EventSink& E, Ctl* ctl, int i, int j, double d,
+                                  bool collide, const d4* r, const d4* v, const double* mass, const int* id)
+{
+    int surv = i, merg = j;
+    int name;
+    if (collide && !P.close_encounter) {
+        if (mass[merg] > mass[surv]) { int t = surv; surv = merg; merg = t; }
+        name = RED_EV_COLLISION;
+    } else {
+        name = RED_EV_CLOSE_ENCOUNTER;
+    }
+    int k = atomicAdd(&ctl->n_events, 1);
+    if (k >= E.cap) return;   // counted but dropped: the host reports an overflow
+    red_event* e = E.buf + k;
+    e->name = name;
+    e->id1 = id[surv]; e->idx1 = surv;
+    e->id2 = id[merg]; e->idx2 = merg;
+    e->attempt = ctl->attempt; e->stage = E.stage;
+    e->sink = i; e->source = j; e->pad = 0;
+    e->t = ctl->t + E.c_stage * ctl->dt;
+    e->d = d;
+    for (int c = 0; c < 4; c++) { e->p1[c] = 0.0; e->p2[c] = 0.0; e->ps[c] = 0.0; }
+    put3(e->r1, r[surv]); put3(e->v1, v[surv]);
+    put3(e->r2, r[merg]); put3(e->v2, v[merg]);
+    if (name == RED_EV_CLOSE_ENCOUNTER) {
+        // the host derives astro-centric elements from the star's stage state
+        put3(e->rs, r[0]); put3(e->vs, v[0]);
+    } else {
+        for (int c = 0; c < 3; c++) { e->rs[c] = 0.0; e->vs[c] = 0.0; }
+    }
+    for (int c = 0; c < 6; c++) { e->oe1[c] = 0.0; e->oe2[c] = 0.0; e->oes[c] = 0.0; }
+}
+
+// The reference pair predicate (pp_disk.cu:265) plus the optional Hill criterion.
+__device__ __forceinline__ void pair_test(const PairTest& P, const EventSink& E, Ctl* ctl, int i, int j, double d,
+                                          double Ri, double Rj, const d4* r, const d4* v, const double* mass,
+                                          const int* id)
+{
+    if (!(0 < i && i > j)) return;
+    bool collide = d < P.radii_f * (Ri + Rj);
+    bool hill = false;
+    if (!collide && P.hill_f > 0.0) {
+        const d4 qi = r[i], qj = r[j];
+        double ri = sqrt((qi.x * qi.x) + (qi.y * qi.y) + (qi.z * qi.z));
+        double rj = sqrt((qj.x * qj.x) + (qj.y * qj.y) + (qj.z * qj.z));
+        double rh = cbrt_ieee((mass[i] + mass[j]) / P.three_mstar) * ((ri + rj) * 0.5);
+        hill = d < P.hill_f * rh;
+    }
+    if (collide || hill) append_pair_event(P, E, ctl, i, j, d, collide, r, v, mass, id);
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 broadcast sweep.
+template <int SPT, bool OVERLAP, bool EV>
+__global__ __launch_bounds__(256) void k_grav_fp64_bcast(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                         d4* __restrict__ a, const double* __restrict__ mass,
+                                                         const double* __restrict__ radius,
+                                                         const int* __restrict__ id, Sweep sw, PairTest P,
+                                                         EventSink E, Ctl* __restrict__ ctl)
+{
+    const int nt = blockDim.x;
+    const int base = sw.sink0 + blockIdx.x * nt * SPT + threadIdx.x;
+    int ii[SPT];
+    bool live[SPT];
+    double xi[SPT], yi[SPT], zi[SPT], Ri[SPT], ax[SPT], ay[SPT], az[SPT];
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        ii[s] = base + s * nt;
+        const bool in = ii[s] < sw.sink1;
+        const int k = in ? ii[s] : sw.sink0;
+        const d4 q = r[k];
+        xi[s] = q.x; yi[s] = q.y; zi[s] = q.z;
+        live[s] = in && id[k] > 0;
+        Ri[s] = EV ? radius[k] : 0.0;
+        ax[s] = ay[s] = az[s] = 0.0;
+    }
+    // Inactive sources (id < 0) and i == j are skipped in the reference (pp_disk.cu:242);
+    // here their term is replaced by +0, which leaves the sum bit-identical: the
+    // accumulator starts at +0 and x + (+-0) == x for every other value it can hold.
+#pragma unroll 2
+    for (int j = sw.src0; j < sw.src1; ++j) {
+        const bool act = id[j] >= 0;
+        const d4 q = r[j];
+#pragma unroll
+        for (int s = 0; s < SPT; s++) {
+            const double dx = q.x - xi[s];
+            const double dy = q.y - yi[s];
+            const double dz = q.z - zi[s];
+            const double r2 = (dx * dx + dy * dy) + dz * dz;
+            const double d = sqrt(r2);
+            const bool use = OVERLAP ? (act && j != ii[s]) : act;
+            const double sc = use ? q.w / (d * r2) : 0.0;
+            ax[s] += sc * dx;
+            ay[s] += sc * dy;
+            az[s] += sc * dz;
+            if (EV && use && live[s]) pair_test(P, E, ctl, ii[s], j, d, Ri[s], radius[j], r, v, mass, id);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        if (ii[s] < sw.sink1) {
+            d4 o;
+            o.x = live[s] ? ax[s] : 0.0;
+            o.y = live[s] ? ay[s] : 0.0;
+            o.z = live[s] ? az[s] : 0.0;
+            o.w = 0.0;
+            a[ii[s]] = o;
+        }
+    }
+}
+
+// FP64 LDS-tile sweep (same arithmetic, sources staged in LDS).
+template <int SPT, bool OVERLAP, bool EV, int TILE>
+__global__ __launch_bounds__(256) void k_grav_fp64_lds(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                       d4* __restrict__ a, const double* __restrict__ mass,
+                                                       const double* __restrict__ radius,
+                                                       const int* __restrict__ id, Sweep sw, PairTest P,
+                                                       EventSink E, Ctl* __restrict__ ctl)
+{
+    __shared__ d4 tile[TILE];
+    __shared__ int tact[TILE];
+    const int nt = blockDim.x;
+    const int base = sw.sink0 + blockIdx.x * nt * SPT + threadIdx.x;
+    int ii[SPT];
+    bool live[SPT];
+    double xi[SPT], yi[SPT], zi[SPT], Ri[SPT], ax[SPT], ay[SPT], az[SPT];
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        ii[s] = base + s * nt;
+        const bool in = ii[s] < sw.sink1;
+        const int k = in ? ii[s] : sw.sink0;
+        const d4 q = r[k];
+        xi[s] = q.x; yi[s] = q.y; zi[s] = q.z;
+        live[s] = in && id[k] > 0;
+        Ri[s] = EV ? radius[k] : 0.0;
+        ax[s] = ay[s] = az[s] = 0.0;
+    }
+    for (int j0 = sw.src0; j0 < sw.src1; j0 += TILE) {
+        const int nj = min(TILE, sw.src1 - j0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nj; k += nt) {
+            tile[k] = r[j0 + k];
+            tact[k] = id[j0 + k] >= 0;
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int k = 0; k < nj; ++k) {
+            const bool act = tact[k] != 0;
+            const d4 q = tile[k];
+            const int j = j0 + k;
+#pragma unroll
+            for (int s = 0; s < SPT; s++) {
+                const double dx = q.x - xi[s];
+                const double dy = q.y - yi[s];
+                const double dz = q.z - zi[s];
+                const double r2 = (dx * dx + dy * dy) + dz * dz;
+                const double d = sqrt(r2);
+                const bool use = OVERLAP ? (act && j != ii[s]) : act;
+                const double sc = use ? q.w / (d * r2) : 0.0;
+                ax[s] += sc * dx;
+                ay[s] += sc * dy;
+                az[s] += sc * dz;
+                if (EV && use && live[s]) pair_test(P, E, ctl, ii[s], j, d, Ri[s], radius[j], r, v, mass, id);
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        if (ii[s] < sw.sink1) {
+            d4 o;
+            o.x = live[s] ? ax[s] : 0.0;
+            o.y = live[s] ? ay[s] : 0.0;
+            o.z = live[s] ? az[s] : 0.0;
+            o.w = 0.0;
+            a[ii[s]] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Mixed-precision broadcast sweep.
+__device__ __forceinline__ void kahan_add(double& sum, double& comp, double x)
+{
+    const double y = x - comp;
+    const double t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+template <int SPT, bool OVERLAP, bool EV, bool KAHAN>
+__global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                          d4* __restrict__ a, const double* __restrict__ mass,
+                                                          const double* __restrict__ radius,
+                                                          const int* __restrict__ id,
+                                                          const SrcMixed* __restrict__ src, Sweep sw,
+                                                          int n_fp64_src, double rmax, PairTest P, EventSink E,
+                                                          Ctl* __restrict__ ctl)
+{
+    const int nt = blockDim.x;
+    const int base = sw.sink0 + blockIdx.x * nt * SPT + threadIdx.x;
+    int ii[SPT];
+    bool live[SPT];
+    double xi[SPT], yi[SPT], zi[SPT], Ri[SPT], ax[SPT], ay[SPT], az[SPT], cx[SPT], cy[SPT], cz[SPT];
+    float cut2[SPT];
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        ii[s] = base + s * nt;
+        const bool in = ii[s] < sw.sink1;
+        const int k = in ? ii[s] : sw.sink0;
+        const d4 q = r[k];
+        xi[s] = q.x; yi[s] = q.y; zi[s] = q.z;
+        live[s] = in && id[k] > 0;
+        Ri[s] = EV ? radius[k] : 0.0;
+        ax[s] = ay[s] = az[s] = 0.0;
+        cx[s] = cy[s] = cz[s] = 0.0;
+        if (EV) {
+            // conservative screen: FP32 r^2 is within ~5e-7 relative of the FP64 value
+            const double cut = P.radii_f * (Ri[s] + rmax) * 1.001;
+            cut2[s] = (float)(cut * cut);
+        }
+    }
+    // FP64 head: the star and giant planets (same arithmetic as the FP64 sweep)
+    const int jsplit = min(sw.src1, max(sw.src0, n_fp64_src));
+    for (int j = sw.src0; j < jsplit; ++j) {
+        const bool act = id[j] >= 0;
+        const d4 q = r[j];
+#pragma unroll
+        for (int s = 0; s < SPT; s++) {
+            const double dx = q.x - xi[s];
+            const double dy = q.y - yi[s];
+            const double dz = q.z - zi[s];
+            const double r2 = (dx * dx + dy * dy) + dz * dz;
+            const double d = sqrt(r2);
+            const bool use = OVERLAP ? (act && j != ii[s]) : act;
+            const double sc = use ? q.w / (d * r2) : 0.0;
+            ax[s] += sc * dx;
+            ay[s] += sc * dy;
+            az[s] += sc * dz;
+            if (EV && use && live[s]) pair_test(P, E, ctl, ii[s], j, d, Ri[s], radius[j], r, v, mass, id);
+        }
+    }
+    // FP32 body in chunks
+    for (int j0 = jsplit; j0 < sw.src1; j0 += CHUNK) {
+        const int j1 = min(sw.src1, j0 + CHUNK);
+        float sx[SPT], sy[SPT], sz[SPT], kx[SPT], ky[SPT], kz[SPT], mn[SPT];
+#pragma unroll
+        for (int s = 0; s < SPT; s++) {
+            sx[s] = sy[s] = sz[s] = 0.0f;
+            kx[s] = ky[s] = kz[s] = 0.0f;
+            mn[s] = 3.0e38f;
+        }
+#pragma unroll 4
+        for (int j = j0; j < j1; ++j) {
+            // one 32-byte scalar load; the low word of w is the FP32 mass (0 if inactive)
+            const d4 q = reinterpret_cast<const d4*>(src)[j];
+            const float qm = __uint_as_float((unsigned)((unsigned long long)__double_as_longlong(q.w) & 0xffffffffull));
+#pragma unroll
+            for (int s = 0; s < SPT; s++) {
+                const float dx = (float)(q.x - xi[s]);
+                const float dy = (float)(q.y - yi[s]);
+                const float dz = (float)(q.z - zi[s]);
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                float sc = (qm * inv) * (inv * inv);
+                if (OVERLAP) sc = (j == ii[s]) ? 0.0f : sc;
+                if (KAHAN) {
+                    float y, t;
+                    y = __builtin_fmaf(sc, dx, -kx[s]); t = sx[s] + y; kx[s] = (t - sx[s]) - y; sx[s] = t;
+                    y = __builtin_fmaf(sc, dy, -ky[s]); t = sy[s] + y; ky[s] = (t - sy[s]) - y; sy[s] = t;
+                    y = __builtin_fmaf(sc, dz, -kz[s]); t = sz[s] + y; kz[s] = (t - sz[s]) - y; sz[s] = t;
+                } else {
+                    sx[s] = __builtin_fmaf(sc, dx, sx[s]);
+                    sy[s] = __builtin_fmaf(sc, dy, sy[s]);
+                    sz[s] = __builtin_fmaf(sc, dz, sz[s]);
+                }
+                if (EV) mn[s] = fminf(mn[s], (OVERLAP && j == ii[s]) ? 3.0e38f : r2);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < SPT; s++) {
+            kahan_add(ax[s], cx[s], (double)sx[s]);
+            kahan_add(ay[s], cy[s], (double)sy[s]);
+            kahan_add(az[s], cz[s], (double)sz[s]);
+            if (EV && live[s] && mn[s] < cut2[s]) {
+                // exact FP64 re-evaluation of the reference predicate for this chunk
+                for (int j = j0; j < j1; ++j) {
+                    if (id[j] < 0 || j == ii[s]) continue;
+                    const d4 q = r[j];
+                    const double dx = q.x - xi[s];
+                    const double dy = q.y - yi[s];
+                    const double dz = q.z - zi[s];
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    pair_test(P, E, ctl, ii[s], j, sqrt(r2), Ri[s], radius[j], r, v, mass, id);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < SPT; s++) {
+        if (ii[s] < sw.sink1) {
+            d4 o;
+            o.x = live[s] ? ax[s] : 0.0;
+            o.y = live[s] ? ay[s] : 0.0;
+            o.z = live[s] ? az[s] : 0.0;
+            o.w = 0.0;
+            a[ii[s]] = o;
+        }
+    }
+}
+
+__global__ void k_zero_acc(d4* __restrict__ a, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+__global__ void k_pack_mixed(const d4* __restrict__ r, const int* __restrict__ id, SrcMixed* __restrict__ out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const d4 q = r[i];
+    SrcMixed s;
+    s.x = q.x; s.y = q.y; s.z = q.z;
+    s.active = id[i] >= 0;
+    s.m = s.active ? (float)q.w : 0.0f;
+    out[i] = s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Split sweep, phase 1: one work-item per source j, SG sinks per block row.
+constexpr int SPLIT_SG = 16;
+
+template <bool EV>
+__global__ __launch_bounds__(256) void k_split_terms(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                     const double* __restrict__ mass,
+                                                     const double* __restrict__ radius,
+                                                     const int* __restrict__ id, Sweep sw, PairTest P,
+                                                     EventSink E, Ctl* __restrict__ ctl, double* __restrict__ T)
+{
+    const int nsrc = sw.src1 - sw.src0;
+    const int jj = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = sw.sink0 + blockIdx.y * SPLIT_SG;
+    const int i1 = min(sw.sink1, i0 + SPLIT_SG);
+    if (jj >= nsrc) return;
+    const int j = sw.src0 + jj;
+    const bool act = id[j] >= 0;
+    const d4 q = r[j];
+    const double Rj = EV ? radius[j] : 0.0;
+    for (int i = i0; i < i1; ++i) {
+        const d4 p = r[i];
+        const double dx = q.x - p.x;
+        const double dy = q.y - p.y;
+        const double dz = q.z - p.z;
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        const double d = sqrt(r2);
+        double sc = q.w / (d * r2);
+        const bool use = act && j != i;
+        sc = use ? sc : 0.0;
+        double* row = T + (size_t)(i - sw.sink0) * 3 * nsrc;
+        row[jj] = sc * dx;
+        row[nsrc + jj] = sc * dy;
+        row[2 * nsrc + jj] = sc * dz;
+        if (EV && use && id[i] > 0) pair_test(P, E, ctl, i, j, d, radius[i], Rj, r, v, mass, id);
+    }
+}
+
+// Split sweep, phase 2: one block per (sink, component).  SEQ: a single work-item adds
+// the terms in ascending j (the reference order); otherwise 256 contiguous chunks are
+// summed in parallel and the chunk sums added in order.
+template <bool SEQ>
+__global__ __launch_bounds__(256) void k_split_sum(const double* __restrict__ T, d4* __restrict__ a,
+                                                   const int* __restrict__ id, Sweep sw)
+{
+    __shared__ double part[256];
+    const int nsrc = sw.src1 - sw.src0;
+    const int row = blockIdx.x;            // (sink - sink0) * 3 + comp
+    const int i = sw.sink0 + row / 3;
+    const int c = row % 3;
+    const double* t = T + (size_t)row * nsrc;
+    double sum = 0.0;
+    if (SEQ) {
+        if (threadIdx.x == 0) {
+            int j = 0;
+            for (; j + 4 <= nsrc; j += 4) {
+                const double t0 = t[j], t1 = t[j + 1], t2 = t[j + 2], t3 = t[j + 3];
+                sum += t0; sum += t1; sum += t2; sum += t3;
+            }
+            for (; j < nsrc; ++j) sum += t[j];
+        }
+    } else {
+        const int per = (nsrc + blockDim.x - 1) / blockDim.x;
+        const int b = threadIdx.x * per;
+        const int e = min(nsrc, b + per);
+        double s = 0.0;
+        for (int j = b; j < e; ++j) s += t[j];
+        part[threadIdx.x] = s;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < (int)blockDim.x; ++k) sum += part[k];
+    }
+    if (threadIdx.x == 0) {
+        const bool live = id[i] > 0;
+        double* o = reinterpret_cast<double*>(a + i);
+        o[c] = live ? sum : 0.0;
+        if (c == 0) o[3] = 0.0;
+    }
+}
+
+template <int SPT, bool OV, bool EV>
+hipError_t go_fp64(int strategy, dim3 g, dim3 b, const Sweep& sw, const GravBuffers& B, const PairTest& P,
+                   const EventSink& E, hipStream_t s)
+{
+    if (strategy == 2)
+        hipLaunchKernelGGL((k_grav_fp64_lds<SPT, OV, EV, 256>), g, b, 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
+                           sw, P, E, B.ctl);
+    else
+        hipLaunchKernelGGL((k_grav_fp64_bcast<SPT, OV, EV>), g, b, 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id, sw,
+                           P, E, B.ctl);
+    return hipGetLastError();
+}
+
+template <int SPT, bool OV, bool EV, bool KH>
+hipError_t go_mixed(dim3 g, dim3 b, const Sweep& sw, const GravBuffers& B, const GravConfig& C,
+                    const PairTest& P, const EventSink& E, double rmax, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_grav_mixed_bcast<SPT, OV, EV, KH>), g, b, 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
+                       B.src_mixed, sw, C.n_fp64_src, rmax, P, E, B.ctl);
+    return hipGetLastError();
+}
+
+template <int SPT>
+hipError_t dispatch_tile(const Sweep& sw, const GravBuffers& B, const GravConfig& C, const PairTest& P,
+                         const EventSink& E, hipStream_t s)
+{
+    const double rmax = P.rmax;
+    const int nsink = sw.sink1 - sw.sink0;
+    const int bs = C.block > 0 ? C.block : 256;
+    dim3 b(bs), g((nsink + bs * SPT - 1) / (bs * SPT));
+    const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
+    const bool ev = P.enabled != 0;
+    if (C.precision == RED_PREC_MIXED) {
+        if (C.kahan_pairs) {
+            if (ov) return ev ? go_mixed<SPT, true, true, true>(g, b, sw, B, C, P, E, rmax, s)
+                              : go_mixed<SPT, true, false, true>(g, b, sw, B, C, P, E, rmax, s);
+            return ev ? go_mixed<SPT, false, true, true>(g, b, sw, B, C, P, E, rmax, s)
+                      : go_mixed<SPT, false, false, true>(g, b, sw, B, C, P, E, rmax, s);
+        }
+        if (ov) return ev ? go_mixed<SPT, true, true, false>(g, b, sw, B, C, P, E, rmax, s)
+                          : go_mixed<SPT, true, false, false>(g, b, sw, B, C, P, E, rmax, s);
+        return ev ? go_mixed<SPT, false, true, false>(g, b, sw, B, C, P, E, rmax, s)
+                  : go_mixed<SPT, false, false, false>(g, b, sw, B, C, P, E, rmax, s);
+    }
+    if (ov) return ev ? go_fp64<SPT, true, true>(C.strategy, g, b, sw, B, P, E, s)
+                      : go_fp64<SPT, true, false>(C.strategy, g, b, sw, B, P, E, s);
+    return ev ? go_fp64<SPT, false, true>(C.strategy, g, b, sw, B, P, E, s)
+              : go_fp64<SPT, false, false>(C.strategy, g, b, sw, B, P, E, s);
+}
+
+}  // namespace
+
+size_t split_scratch_bytes(const Sweep& sw)
+{
+    return (size_t)(sw.sink1 - sw.sink0) * 3 * (size_t)(sw.src1 - sw.src0) * sizeof(double);
+}
+
+hipError_t launch_pack_mixed(const d4* r, const int* id, SrcMixed* out, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_mixed, dim3((n + 255) / 256), dim3(256), 0, s, r, id, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig& C, const PairTest& P,
+                        const EventSink& E, hipStream_t s)
+{
+    const int nsink = sw.sink1 - sw.sink0;
+    const int nsrc = sw.src1 - sw.src0;
+    if (nsink <= 0) return hipSuccess;
+    const bool use_split = nsrc > 0 && nsink <= C.split_max_sinks && nsrc >= 1024 &&
+                           split_scratch_bytes(sw) <= B.scratch_bytes;
+    if (use_split) {
+        double* T = (double*)B.scratch;
+        dim3 g1((nsrc + 255) / 256, (nsink + SPLIT_SG - 1) / SPLIT_SG);
+        if (P.enabled)
+            hipLaunchKernelGGL(k_split_terms<true>, g1, dim3(256), 0, s, B.r, B.v, B.mass, B.radius, B.id, sw, P, E,
+                               B.ctl, T);
+        else
+            hipLaunchKernelGGL(k_split_terms<false>, g1, dim3(256), 0, s, B.r, B.v, B.mass, B.radius, B.id, sw, P,
+                               E, B.ctl, T);
+        RED_HIP_TRY(hipGetLastError());
+        dim3 g2(nsink * 3);
+        if (C.precision == RED_PREC_FP64)
+            hipLaunchKernelGGL(k_split_sum<true>, g2, dim3(64), 0, s, T, B.a, B.id, sw);
+        else
+            hipLaunchKernelGGL(k_split_sum<false>, g2, dim3(256), 0, s, T, B.a, B.id, sw);
+        return hipGetLastError();
+    }
+    if (nsrc <= 0) {
+        // no sources: accelerations are zero (the reference zeroes a before the j loop)
+        hipLaunchKernelGGL(k_zero_acc, dim3((nsink + 255) / 256), dim3(256), 0, s, B.a + sw.sink0, nsink);
+        return hipGetLastError();
+    }
+    switch (C.spt) {
+    case 2: return dispatch_tile<2>(sw, B, C, P, E, s);
+    case 4: return dispatch_tile<4>(sw, B, C, P, E, s);
+    default: return dispatch_tile<1>(sw, B, C, P, E, s);
+    }
+}
+
+}  // namespace red

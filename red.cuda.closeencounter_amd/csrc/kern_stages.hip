@@ -1,0 +1,221 @@
+// kern_stages.hip -- Runge-Kutta stage combinations and the embedded-error norm (gfx950).
+//
+// One work-item per body handles both halves of the state: the position record (whose
+// derivative at stage k is the velocity record of stage state k) and the velocity record
+// (whose derivative is the acceleration of stage k).  Records are 32 bytes, so a wavefront
+// streams 2 KiB per operand with fully coalesced 16-byte loads.  The trial step and stage
+// time come from the control block, so a captured graph replays with current values.
+//
+// Expression shapes follow the reference kernels term for term (built with
+// -ffp-contract=off, so each product and sum is rounded separately as there):
+//   SF_AXPY     out = y + (a*dt) f                 Euler/RK2/RK4 sum_vector
+//               (int_euler.cu:21-33, int_rungekutta2.cu:19-31, int_rungekutta4.cu:36-48)
+//   SF_DTSUM    out = y + dt (a0 f0 + a1 f1 + ...)   RKF78 calc_ytemp_for_fN
+//               (int_rungekutta8.cu:137-305)
+//   SF_ACCUM    out = y; out += (dt*a_k) f_k         Cash-Karp calc_ytemp / calc_y_np1
+//               (int_rungekutta5.cu:41-77)
+//   SF_RK4NP1   out = y + (b0 dt)(f0+f3) + (b1 dt)(f1+f2)   (int_rungekutta4.cu:50-61)
+//   SF_RKF78NP1 out = y + dt (b0 f0 + b5 f5 + b6 (f6+f7) + b8 (f8+f9) + b10 f10)
+//               (int_rungekutta8.cu:321-333)
+// Error forms (fused with the max-reduction that integrator::get_max_error performs
+// with thrust::max_element, integrator.cu:211-256):
+//   EF_RK4      e = f3 - f4                         (int_rungekutta4.cu:63-74)
+//   EF_RKF78    e = f0 + f10 - f11 - f12            (int_rungekutta8.cu:307-319)
+//   EF_CK       e = sum_k (bh_k - b_k) f_k          (int_rungekutta5.cu:79-92)
+// The maximum starts from 0 and ignores NaN like the reference CPU loop; a NaN sets a
+// flag the host turns into an error.
+#include "red_internal.hpp"
+
+namespace red {
+namespace {
+
+__device__ __forceinline__ void axpy_rec(d4& o, const d4& y, double c, const d4& f)
+{
+    o.x = y.x + c * f.x;
+    o.y = y.y + c * f.y;
+    o.z = y.z + c * f.z;
+}
+
+template <int FORM>
+__device__ __forceinline__ d4 combine(const StageArgs& A, const d4& y, int i, bool vel, double dt)
+{
+    d4 o;
+    o.w = y.w;
+    if (FORM == SF_AXPY) {
+        const double c = A.t[0].coef * dt;
+        const d4 f = vel ? A.t[0].fa[i] : A.t[0].fv[i];
+        axpy_rec(o, y, c, f);
+    } else if (FORM == SF_DTSUM) {
+        d4 f = vel ? A.t[0].fa[i] : A.t[0].fv[i];
+        double sx = A.t[0].coef * f.x, sy = A.t[0].coef * f.y, sz = A.t[0].coef * f.z;
+        for (int k = 1; k < A.nterms; k++) {
+            f = vel ? A.t[k].fa[i] : A.t[k].fv[i];
+            sx = sx + A.t[k].coef * f.x;
+            sy = sy + A.t[k].coef * f.y;
+            sz = sz + A.t[k].coef * f.z;
+        }
+        o.x = y.x + dt * sx;
+        o.y = y.y + dt * sy;
+        o.z = y.z + dt * sz;
+    } else if (FORM == SF_ACCUM) {
+        o.x = y.x; o.y = y.y; o.z = y.z;
+        for (int k = 0; k < A.nterms; k++) {
+            const double c = dt * A.t[k].coef;
+            const d4 f = vel ? A.t[k].fa[i] : A.t[k].fv[i];
+            o.x += c * f.x;
+            o.y += c * f.y;
+            o.z += c * f.z;
+        }
+    } else if (FORM == SF_RK4NP1) {
+        const double b0 = A.c1 * dt, b1 = A.c2 * dt;
+        const d4 f0 = vel ? A.t[0].fa[i] : A.t[0].fv[i];
+        const d4 f1 = vel ? A.t[1].fa[i] : A.t[1].fv[i];
+        const d4 f2 = vel ? A.t[2].fa[i] : A.t[2].fv[i];
+        const d4 f3 = vel ? A.t[3].fa[i] : A.t[3].fv[i];
+        o.x = y.x + b0 * (f0.x + f3.x) + b1 * (f1.x + f2.x);
+        o.y = y.y + b0 * (f0.y + f3.y) + b1 * (f1.y + f2.y);
+        o.z = y.z + b0 * (f0.z + f3.z) + b1 * (f1.z + f2.z);
+    } else {  // SF_RKF78NP1: terms f0 f5 f6 f7 f8 f9 f10, coefficients b0 b5 b6 - b8 - b10
+        const d4 f0 = vel ? A.t[0].fa[i] : A.t[0].fv[i];
+        const d4 f5 = vel ? A.t[1].fa[i] : A.t[1].fv[i];
+        const d4 f6 = vel ? A.t[2].fa[i] : A.t[2].fv[i];
+        const d4 f7 = vel ? A.t[3].fa[i] : A.t[3].fv[i];
+        const d4 f8 = vel ? A.t[4].fa[i] : A.t[4].fv[i];
+        const d4 f9 = vel ? A.t[5].fa[i] : A.t[5].fv[i];
+        const d4 f10 = vel ? A.t[6].fa[i] : A.t[6].fv[i];
+        const double b0 = A.t[0].coef, b5 = A.t[1].coef, b6 = A.t[2].coef, b8 = A.t[4].coef, b10 = A.t[6].coef;
+        o.x = y.x + dt * (b0 * f0.x + b5 * f5.x + b6 * (f6.x + f7.x) + b8 * (f8.x + f9.x) + b10 * f10.x);
+        o.y = y.y + dt * (b0 * f0.y + b5 * f5.y + b6 * (f6.y + f7.y) + b8 * (f8.y + f9.y) + b10 * f10.y);
+        o.z = y.z + dt * (b0 * f0.z + b5 * f5.z + b6 * (f6.z + f7.z) + b8 * (f8.z + f9.z) + b10 * f10.z);
+    }
+    return o;
+}
+
+template <int FORM>
+__global__ __launch_bounds__(256) void k_stage(StageArgs A)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const double dt = A.ctl->dt;
+    const d4 yr = A.yr[i];
+    const d4 yv = A.yv[i];
+    d4 orr = combine<FORM>(A, yr, i, false, dt);
+    d4 ovv = combine<FORM>(A, yv, i, true, dt);
+    orr.w = yr.w;   // mass rides along unchanged
+    ovv.w = 0.0;
+    A.outr[i] = orr;
+    A.outv[i] = ovv;
+}
+
+template <int EF>
+__device__ __forceinline__ double3 err_rec(const StageArgs& A, int i, bool vel)
+{
+    double3 e;
+    if (EF == EF_RK4) {
+        const d4 f3 = vel ? A.t[0].fa[i] : A.t[0].fv[i];
+        const d4 f4 = vel ? A.t[1].fa[i] : A.t[1].fv[i];
+        e.x = f3.x - f4.x; e.y = f3.y - f4.y; e.z = f3.z - f4.z;
+    } else if (EF == EF_RKF78) {
+        const d4 f0 = vel ? A.t[0].fa[i] : A.t[0].fv[i];
+        const d4 f10 = vel ? A.t[1].fa[i] : A.t[1].fv[i];
+        const d4 f11 = vel ? A.t[2].fa[i] : A.t[2].fv[i];
+        const d4 f12 = vel ? A.t[3].fa[i] : A.t[3].fv[i];
+        e.x = (f0.x + f10.x - f11.x - f12.x);
+        e.y = (f0.y + f10.y - f11.y - f12.y);
+        e.z = (f0.z + f10.z - f11.z - f12.z);
+    } else {
+        d4 f = vel ? A.t[0].fa[i] : A.t[0].fv[i];
+        e.x = A.t[0].coef * f.x; e.y = A.t[0].coef * f.y; e.z = A.t[0].coef * f.z;
+        for (int k = 1; k < A.nterms; k++) {
+            f = vel ? A.t[k].fa[i] : A.t[k].fv[i];
+            e.x += A.t[k].coef * f.x;
+            e.y += A.t[k].coef * f.y;
+            e.z += A.t[k].coef * f.z;
+        }
+    }
+    return e;
+}
+
+__device__ __forceinline__ void upd_max(double& m, int& nan, double e)
+{
+    if (e != e) nan = 1;
+    else if (m < e) m = e;
+}
+
+template <int EF>
+__global__ __launch_bounds__(256) void k_err(StageArgs A, Ctl* ctl)
+{
+    __shared__ double wmax[4];
+    __shared__ int wnan[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    int nan = 0;
+    if (i < A.n) {
+        const double3 er = err_rec<EF>(A, i, false);
+        const double3 ev = err_rec<EF>(A, i, true);
+        upd_max(m, nan, er.x); upd_max(m, nan, er.y); upd_max(m, nan, er.z);
+        upd_max(m, nan, ev.x); upd_max(m, nan, ev.y); upd_max(m, nan, ev.z);
+    }
+    // wavefront max via cross-lane shuffles, then across the block's waves
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = m < o ? o : m;
+        nan |= __shfl_xor(nan, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { wmax[wave] = m; wnan[wave] = nan; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 1; w < nw; w++) { m = m < wmax[w] ? wmax[w] : m; nan |= wnan[w]; }
+        // m >= +0.0, whose IEEE bit patterns are ordered like the values
+        if (m > 0.0) atomicMax(&ctl->err_bits, (unsigned long long)__double_as_longlong(m));
+        if (nan) atomicOr(&ctl->nan_flag, 1);
+    }
+}
+
+__global__ void k_reset_ctl(Ctl* ctl, int clear_events)
+{
+    ctl->err_bits = 0ull;
+    ctl->nan_flag = 0;
+    if (clear_events) ctl->n_events = 0;
+}
+
+}  // namespace
+
+hipError_t launch_stage(int form, const StageArgs& A, hipStream_t s)
+{
+    if (A.n <= 0) return hipSuccess;
+    dim3 g((A.n + 255) / 256), b(256);
+    switch (form) {
+    case SF_AXPY: hipLaunchKernelGGL(k_stage<SF_AXPY>, g, b, 0, s, A); break;
+    case SF_DTSUM: hipLaunchKernelGGL(k_stage<SF_DTSUM>, g, b, 0, s, A); break;
+    case SF_ACCUM: hipLaunchKernelGGL(k_stage<SF_ACCUM>, g, b, 0, s, A); break;
+    case SF_RK4NP1: hipLaunchKernelGGL(k_stage<SF_RK4NP1>, g, b, 0, s, A); break;
+    case SF_RKF78NP1: hipLaunchKernelGGL(k_stage<SF_RKF78NP1>, g, b, 0, s, A); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_err(int form, const StageArgs& A, hipStream_t s)
+{
+    if (A.n <= 0) return hipSuccess;
+    dim3 g((A.n + 255) / 256), b(256);
+    Ctl* ctl = const_cast<Ctl*>(A.ctl);
+    switch (form) {
+    case EF_RK4: hipLaunchKernelGGL(k_err<EF_RK4>, g, b, 0, s, A, ctl); break;
+    case EF_RKF78: hipLaunchKernelGGL(k_err<EF_RKF78>, g, b, 0, s, A, ctl); break;
+    case EF_CK: hipLaunchKernelGGL(k_err<EF_CK>, g, b, 0, s, A, ctl); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reset_ctl(Ctl* ctl, int clear_events, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reset_ctl, dim3(1), dim3(1), 0, s, ctl, clear_events);
+    return hipGetLastError();
+}
+
+}  // namespace red

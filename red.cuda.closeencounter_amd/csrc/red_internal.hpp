@@ -1,0 +1,160 @@
+// red_internal.hpp -- shared device/host definitions of the MI355X integration path.
+//
+// Data layout in HBM (one context):
+//   state  Y[2]   : r[cap] then v[cap], each a 32-byte record {x, y, z, w}; the w lane of
+//                   r carries the body mass (so one 32-byte scalar load hands a source body
+//                   to a whole wavefront), the w lane of v is 0.  This is the reference's
+//                   vec_t AoS (red_type.h:142-148) with the unused lane put to work.
+//   stages S_k    : the same pair of records for every intermediate stage state.
+//   accels A_k    : one record {ax, ay, az, 0} per body per stage.
+//   mass, radius  : FP64 per body (canonical copies; r.w mirrors mass), id, type: int32.
+//   Ctl           : per-context control block (step-start time, trial step, attempt,
+//                   error maximum, NaN flag, event counter), read by every kernel so that a
+//                   captured HIP graph replays with the current step values.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/rednative.h"
+
+namespace red {
+
+struct alignas(32) d4 {
+    double x, y, z, w;
+};
+
+// Device control block.  The host writes the first three fields before an attempt;
+// kernels write the rest.
+struct Ctl {
+    double t;                      // step-start time (internal units)
+    double dt;                     // trial step of the current attempt
+    int attempt;                   // attempt index within the step
+    int pad0;
+    unsigned long long err_bits;   // max(0, max error component), IEEE bits (monotone for >= 0)
+    int nan_flag;                  // a non-finite error component was seen
+    int n_events;                  // pair events appended since the last clear
+    int n_events_hc;               // ejection / hit-centrum events of the last scan
+    int pad1;
+};
+
+// One force sweep: sinks [sink0, sink1) against sources [src0, src1).
+struct Sweep {
+    int sink0, sink1, src0, src1;
+};
+
+// Pair-test parameters shared by every force kernel.
+struct PairTest {
+    double radii_f;                // collision when d < radii_f * (R_i + R_j)
+    double hill_f;                 // close encounter when d < hill_f * R_Hill
+    double three_mstar;            // 3 M_star, for R_Hill
+    double rmax;                   // largest body radius (mixed-precision screen)
+    int close_encounter;           // 1: report CLOSE_ENCOUNTER, keep (i, j) order
+    int enabled;                   // any pair test active
+};
+
+struct EventSink {
+    red_event* buf;
+    int cap;
+    int stage;
+    double c_stage;                // event time = ctl->t + c_stage * ctl->dt
+};
+
+// Packed source record for the mixed-precision sweep: FP64 position, FP32 mass.
+struct alignas(32) SrcMixed {
+    double x, y, z;
+    float m;
+    int active;
+};
+
+}  // namespace red
+
+// ---- launchers implemented in the .hip files (host side) ----------------------------
+namespace red {
+
+struct GravBuffers {
+    const d4* r;                   // stage positions (w = mass)
+    const d4* v;                   // stage velocities (only read for event records)
+    d4* a;                         // output accelerations
+    const double* mass;
+    const double* radius;
+    const int* id;
+    Ctl* ctl;
+    void* scratch;                 // split-sweep scratch
+    size_t scratch_bytes;
+    SrcMixed* src_mixed;           // packed sources for the mixed sweep (cap entries)
+};
+
+struct GravConfig {
+    int precision;                 // RED_PREC_*
+    int n_fp64_src;                // mixed: sources [0, n_fp64_src) stay FP64
+    int block;                     // threads per block
+    int spt;                       // sinks per thread
+    int strategy;                  // 1 scalar broadcast, 2 LDS tiles
+    int kahan_pairs;
+    int split_max_sinks;           // sweeps with fewer sinks use the split path
+};
+
+// Launches one force sweep (all three classes are separate sweeps).
+hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig& C, const PairTest& P,
+                        const EventSink& E, hipStream_t s);
+// Scratch bytes the split path needs for a sweep.
+size_t split_scratch_bytes(const Sweep& sw);
+// Packs SrcMixed records for bodies [0, n).
+hipError_t launch_pack_mixed(const d4* r, const int* id, SrcMixed* out, int n, hipStream_t s);
+
+// Stage-sum forms (see kern_stages.hip).
+enum StageForm { SF_AXPY = 0, SF_DTSUM = 1, SF_ACCUM = 2, SF_RK4NP1 = 3, SF_RKF78NP1 = 4 };
+enum ErrForm { EF_RK4 = 0, EF_RKF78 = 1, EF_CK = 2 };
+
+struct StageTerm {
+    double coef;
+    const d4* fv;                  // position derivative: velocity record of stage state k
+    const d4* fa;                  // velocity derivative: acceleration of stage k
+};
+
+constexpr int MAX_TERMS = 13;
+
+struct StageArgs {
+    const d4* yr;
+    const d4* yv;
+    d4* outr;
+    d4* outv;
+    int n;
+    int nterms;
+    double c1, c2;
+    const Ctl* ctl;
+    StageTerm t[MAX_TERMS];
+};
+
+hipError_t launch_stage(int form, const StageArgs& A, hipStream_t s);
+hipError_t launch_err(int form, const StageArgs& A, hipStream_t s);
+hipError_t launch_reset_ctl(Ctl* ctl, int clear_events, hipStream_t s);
+
+// Ejection / hit-centrum scan (pp_disk.cu:72-142): appends events and negates ids.
+hipError_t launch_ejection_scan(const d4* r, const d4* v, int* id, const int* type, int n, double ej,
+                                double hc, Ctl* ctl, red_event* buf, int cap, hipStream_t s);
+
+// Scatter of merged-body updates.
+struct BodyUpdate {
+    int idx;
+    int id;
+    double r[4];
+    double v[4];
+    double mass, radius;
+};
+hipError_t launch_apply_updates(const BodyUpdate* u, int nu, d4* r, d4* v, double* mass, double* radius,
+                                int* id, hipStream_t s);
+
+// Energy / angular momentum partials per massive body.
+hipError_t launch_energy(const d4* r, const d4* v, const double* mass, const int* id, int n_massive, int n_si,
+                         double* out6, hipStream_t s);
+
+}  // namespace red
+
+#define RED_HIP_TRY(expr)                                   \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) return _e;                    \
+    } while (0)

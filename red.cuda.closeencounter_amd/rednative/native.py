@@ -1,0 +1,379 @@
+"""ctypes binding of librednative.so (the C ABI in include/rednative.h).
+
+``Disk`` mirrors the surface the reference's driver uses on pp_disk and the
+integrator (red.cuda/src/red.cuda/pp_disk.h:50-108, integrator.h:42): ``step``,
+``check_for_ejection_hit_centrum``, ``check_for_collision``,
+``check_for_close_encounter``, ``check_for_rebuild_vectors``,
+``clear_event_counter``, ``t``, plus state and event accessors.  Failures raise
+``RedError`` with the library's message, the way the reference throws
+``nbody_exception`` / ``std::string`` (red.cuda/src/red.cuda/nbody_exception.h).
+
+There is no fallback: if the HIP library is not built this module raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "librednative.so")
+
+RED_OK = 0
+ERR_NAMES = {1: "RED_ERR_ARG", 2: "RED_ERR_HIP", 3: "RED_ERR_EVENT_OVERFLOW", 4: "RED_ERR_NAN",
+             5: "RED_ERR_UNSUPPORTED", 6: "RED_ERR_NO_STAR", 7: "RED_ERR_IO", 8: "RED_ERR_STEP_LIMIT"}
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class RedError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class RedEvent(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_int), ("id1", ctypes.c_int), ("idx1", ctypes.c_int),
+                ("id2", ctypes.c_int), ("idx2", ctypes.c_int), ("attempt", ctypes.c_int),
+                ("stage", ctypes.c_int), ("sink", ctypes.c_int), ("source", ctypes.c_int),
+                ("pad", ctypes.c_int), ("t", ctypes.c_double), ("d", ctypes.c_double),
+                ("p1", ctypes.c_double * 4), ("p2", ctypes.c_double * 4), ("ps", ctypes.c_double * 4),
+                ("r1", ctypes.c_double * 3), ("v1", ctypes.c_double * 3),
+                ("r2", ctypes.c_double * 3), ("v2", ctypes.c_double * 3),
+                ("rs", ctypes.c_double * 3), ("vs", ctypes.c_double * 3),
+                ("oe1", ctypes.c_double * 6), ("oe2", ctypes.c_double * 6), ("oes", ctypes.c_double * 6)]
+
+
+class RedParams(ctypes.Structure):
+    _fields_ = [("integrator", ctypes.c_int), ("adaptive", ctypes.c_int), ("tolerance", ctypes.c_double),
+                ("error_check_for_tp", ctypes.c_int), ("close_encounter", ctypes.c_int),
+                ("inner_steps", ctypes.c_int), ("frame_center", ctypes.c_int),
+                ("hit_centrum", ctypes.c_double), ("ejection", ctypes.c_double),
+                ("radii_enhance_factor", ctypes.c_double), ("encounter_hill_factor", ctypes.c_double),
+                ("dt_day", ctypes.c_double), ("precision", ctypes.c_int), ("event_capacity", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("kahan_pairs", ctypes.c_int), ("device", ctypes.c_int),
+                ("block", ctypes.c_int), ("sinks_per_thread", ctypes.c_int), ("strategy", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
+
+
+class RedBodies(ctypes.Structure):
+    _fields_ = [("counts", ctypes.c_int * 7), ("id", ctypes.c_void_p), ("type", ctypes.c_void_p),
+                ("mig_type", ctypes.c_void_p), ("mig_stop_at", ctypes.c_void_p), ("epoch", ctypes.c_void_p),
+                ("mass", ctypes.c_void_p), ("radius", ctypes.c_void_p), ("density", ctypes.c_void_p),
+                ("cd", ctypes.c_void_p), ("r", ctypes.c_void_p), ("v", ctypes.c_void_p)]
+
+
+class RedStepInfo(ctypes.Structure):
+    _fields_ = [("dt_did", ctypes.c_double), ("dt_next", ctypes.c_double), ("err", ctypes.c_double),
+                ("tries", ctypes.c_int), ("n_events", ctypes.c_int)]
+
+
+# Every entry point the header declares, with its ctypes signature.
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_IP = ctypes.POINTER(ctypes.c_int)
+_DP = ctypes.POINTER(ctypes.c_double)
+SIGNATURES = {
+    "red_ctx_create": (_I, [ctypes.POINTER(RedBodies), ctypes.POINTER(RedParams), ctypes.POINTER(_P)]),
+    "red_ctx_destroy": (_I, [_P]),
+    "red_last_error": (ctypes.c_char_p, [_P]),
+    "red_get_counts": (_I, [_P, _IP]),
+    "red_get_time": (_I, [_P, _DP]),
+    "red_set_time": (_I, [_P, _D]),
+    "red_get_dt": (_I, [_P, _DP]),
+    "red_set_dt": (_I, [_P, _D]),
+    "red_get_state": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "red_set_state": (_I, [_P, _P, _P]),
+    "red_accel": (_I, [_P, _D, _I, _P, _P, _P]),
+    "red_state_device": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "red_sync": (_I, [_P]),
+    "red_step": (_I, [_P, ctypes.POINTER(RedStepInfo)]),
+    "red_check_ejection_hit_centrum": (_I, [_P, _IP]),
+    "red_check_collision": (_I, [_P, _IP]),
+    "red_check_close_encounter": (_I, [_P, _IP]),
+    "red_check_rebuild": (_I, [_P, _I, _IP]),
+    "red_clear_events": (_I, [_P]),
+    "red_raw_events": (_I, [_P, ctypes.POINTER(RedEvent), _I, _IP, _IP]),
+    "red_sp_events": (_I, [_P, ctypes.POINTER(RedEvent), _I, _IP]),
+    "red_last_rebuild_map": (_I, [_P, _IP, _I, _IP]),
+    "red_counters": (_I, [_P, _IP]),
+    "red_energy": (_I, [_P, _DP, _DP]),
+    "red_host_transform_to_bc": (_I, [_I, _P, _P, _P]),
+    "red_host_transform_velocity": (_I, [_I, _P]),
+    "red_set_profiling": (_I, [_P, _I]),
+    "red_get_profile": (_I, [_P, _DP, _IP, _I]),
+    "red_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def load_library(path: Optional[str] = None):
+    """Loads librednative.so; raises if it is not built (no fallback path exists)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is not built; run __graft_entry__.build() or make -C {PKG}")
+    L = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = L
+    return L
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def event_dict(e: RedEvent) -> dict:
+    d = {f: getattr(e, f) for f, _ in RedEvent._fields_ if f != "pad"}
+    for k, v in list(d.items()):
+        if not isinstance(v, (int, float)):
+            d[k] = tuple(v)
+    return d
+
+
+def host_transform_to_bc(mass: np.ndarray, r: np.ndarray, v: np.ndarray) -> None:
+    L = load_library()
+    rc = L.red_host_transform_to_bc(len(mass), _ptr(mass), _ptr(r), _ptr(v))
+    if rc:
+        raise RedError(rc, "red_host_transform_to_bc")
+
+
+def host_transform_velocity(v: np.ndarray) -> None:
+    L = load_library()
+    rc = L.red_host_transform_velocity(len(v), _ptr(v))
+    if rc:
+        raise RedError(rc, "red_host_transform_velocity")
+
+
+class Disk:
+    """One disk integrated on the GPU through the C ABI (internal units)."""
+
+    def __init__(self, bodies, integrator=0, adaptive=False, tolerance=1e-10, dt_day=0.001,
+                 error_check_for_tp=False, close_encounter=False, inner_steps=True, frame_center=1,
+                 hit_centrum=0.0, ejection=0.0, radii_enhance_factor=0.0, encounter_hill_factor=0.0,
+                 precision=PREC_FP64, event_capacity=0, use_graph=False, kahan_pairs=False, device=0,
+                 block=0, sinks_per_thread=0, strategy=0, r=None, v=None):
+        self.L = load_library()
+        self.ctx = ctypes.c_void_p()
+        prm = RedParams()
+        prm.integrator = integrator
+        prm.adaptive = int(adaptive)
+        prm.tolerance = tolerance
+        prm.error_check_for_tp = int(error_check_for_tp)
+        prm.close_encounter = int(close_encounter)
+        prm.inner_steps = int(inner_steps)
+        prm.frame_center = frame_center
+        prm.hit_centrum = hit_centrum
+        prm.ejection = ejection
+        prm.radii_enhance_factor = radii_enhance_factor
+        prm.encounter_hill_factor = encounter_hill_factor
+        prm.dt_day = dt_day
+        prm.precision = precision
+        prm.event_capacity = event_capacity
+        prm.use_graph = int(use_graph)
+        prm.kahan_pairs = int(kahan_pairs)
+        prm.device = device
+        prm.block = block
+        prm.sinks_per_thread = sinks_per_thread
+        prm.strategy = strategy
+        self._keep = dict(
+            id=np.ascontiguousarray(bodies.ids, dtype=np.int32),
+            type=np.ascontiguousarray(bodies.types, dtype=np.int32),
+            mig_type=np.ascontiguousarray(bodies.mig_type, dtype=np.int32),
+            mig_stop_at=np.ascontiguousarray(bodies.mig_stop_at, dtype=np.float64),
+            epoch=np.ascontiguousarray(bodies.epoch, dtype=np.float64),
+            mass=np.ascontiguousarray(bodies.mass, dtype=np.float64),
+            radius=np.ascontiguousarray(bodies.radius, dtype=np.float64),
+            density=np.ascontiguousarray(bodies.density, dtype=np.float64),
+            cd=np.ascontiguousarray(bodies.cd, dtype=np.float64),
+            r=np.ascontiguousarray(bodies.r if r is None else r, dtype=np.float64),
+            v=np.ascontiguousarray(bodies.v if v is None else v, dtype=np.float64),
+        )
+        b = RedBodies()
+        for k in range(7):
+            b.counts[k] = int(bodies.counts[k])
+        for k, a in self._keep.items():
+            setattr(b, k, _ptr(a))
+        rc = self.L.red_ctx_create(ctypes.byref(b), ctypes.byref(prm), ctypes.byref(self.ctx))
+        if rc:
+            msg = self.L.red_last_error(self.ctx).decode() if self.ctx else ""
+            self.close()
+            raise RedError(rc, msg or "red_ctx_create failed")
+        self._keep = None
+        self.params = prm
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx:
+            self.L.red_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc:
+            raise RedError(rc, self.L.red_last_error(self.ctx).decode())
+
+    # ---- counts / time ---------------------------------------------------------
+    def counts(self):
+        c = np.zeros(7, dtype=np.int32)
+        self._chk(self.L.red_get_counts(self.ctx, c.ctypes.data_as(_IP)))
+        return [int(x) for x in c]
+
+    @property
+    def n(self):
+        return int(sum(self.counts()))
+
+    @property
+    def t(self):
+        x = ctypes.c_double()
+        self._chk(self.L.red_get_time(self.ctx, ctypes.byref(x)))
+        return x.value
+
+    @t.setter
+    def t(self, value):
+        self._chk(self.L.red_set_time(self.ctx, float(value)))
+
+    @property
+    def dt_try(self):
+        x = ctypes.c_double()
+        self._chk(self.L.red_get_dt(self.ctx, ctypes.byref(x)))
+        return x.value
+
+    @dt_try.setter
+    def dt_try(self, value):
+        self._chk(self.L.red_set_dt(self.ctx, float(value)))
+
+    # ---- state --------------------------------------------------------------------
+    def state(self):
+        n = self.n
+        r = np.zeros((n, 3)); v = np.zeros((n, 3))
+        m = np.zeros(n); R = np.zeros(n); rho = np.zeros(n); cd = np.zeros(n)
+        ids = np.zeros(n, dtype=np.int32); types = np.zeros(n, dtype=np.int32)
+        self._chk(self.L.red_get_state(self.ctx, _ptr(r), _ptr(v), _ptr(m), _ptr(R), _ptr(rho), _ptr(cd),
+                                       _ptr(ids), _ptr(types)))
+        return dict(r=r, v=v, p=np.stack([m, R, rho, cd], axis=1), id=ids, type=types)
+
+    def set_state(self, r, v):
+        r = np.ascontiguousarray(r, dtype=np.float64); v = np.ascontiguousarray(v, dtype=np.float64)
+        self._chk(self.L.red_set_state(self.ctx, _ptr(r), _ptr(v)))
+
+    def state_device(self):
+        rp, vp = ctypes.c_void_p(), ctypes.c_void_p()
+        self._chk(self.L.red_state_device(self.ctx, ctypes.byref(rp), ctypes.byref(vp)))
+        return rp.value, vp.value
+
+    def sync(self):
+        self._chk(self.L.red_sync(self.ctx))
+
+    def accel_device(self, t, stage, r_dev, v_dev, a_dev):
+        """red_accel on caller-owned device buffers (raw addresses)."""
+        self._chk(self.L.red_accel(self.ctx, float(t), int(stage), ctypes.c_void_p(r_dev),
+                                   ctypes.c_void_p(v_dev), ctypes.c_void_p(a_dev)))
+
+    def accel(self, t, r, v, stage=0):
+        """Force evaluation through red_accel with device buffers from torch."""
+        import torch
+        n = len(r)
+        rr = np.zeros((n, 4)); vv = np.zeros((n, 4))
+        rr[:, :3] = r; vv[:, :3] = v
+        st = self.state()
+        rr[:, 3] = st["p"][:, 0]
+        dev = torch.device("cuda", int(self.params.device))
+        tr = torch.from_numpy(rr).to(dev)
+        tv = torch.from_numpy(vv).to(dev)
+        ta = torch.full((n, 4), float("nan"), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize(dev)
+        self.accel_device(t, stage, tr.data_ptr(), tv.data_ptr(), ta.data_ptr())
+        self.sync()
+        return ta.cpu().numpy()[:, :3].copy()
+
+    # ---- integration and events -----------------------------------------------------
+    def step(self) -> float:
+        info = RedStepInfo()
+        self._chk(self.L.red_step(self.ctx, ctypes.byref(info)))
+        self.last_step = info
+        return info.dt_did
+
+    def _flag(self, fn, *args):
+        h = ctypes.c_int(0)
+        self._chk(fn(self.ctx, *args, ctypes.byref(h)))
+        return bool(h.value)
+
+    def check_for_ejection_hit_centrum(self) -> bool:
+        return self._flag(self.L.red_check_ejection_hit_centrum)
+
+    def check_for_collision(self) -> bool:
+        return self._flag(self.L.red_check_collision)
+
+    def check_for_close_encounter(self) -> bool:
+        return self._flag(self.L.red_check_close_encounter)
+
+    def check_for_rebuild_vectors(self, n: int) -> bool:
+        return self._flag(self.L.red_check_rebuild, int(n))
+
+    def clear_event_counter(self):
+        self._chk(self.L.red_clear_events(self.ctx))
+
+    def raw_events(self):
+        n = ctypes.c_int(0); ov = ctypes.c_int(0)
+        self._chk(self.L.red_raw_events(self.ctx, None, 0, ctypes.byref(n), ctypes.byref(ov)))
+        if ov.value:
+            raise RedError(3, f"{n.value} pair events exceed the capacity")
+        buf = (RedEvent * max(n.value, 1))()
+        self._chk(self.L.red_raw_events(self.ctx, buf, n.value, ctypes.byref(n), ctypes.byref(ov)))
+        return [event_dict(buf[k]) for k in range(n.value)]
+
+    def n_event(self):
+        n = ctypes.c_int(0); ov = ctypes.c_int(0)
+        self._chk(self.L.red_raw_events(self.ctx, None, 0, ctypes.byref(n), ctypes.byref(ov)))
+        return n.value
+
+    def sp_events(self):
+        n = ctypes.c_int(0)
+        self._chk(self.L.red_sp_events(self.ctx, None, 0, ctypes.byref(n)))
+        buf = (RedEvent * max(n.value, 1))()
+        self._chk(self.L.red_sp_events(self.ctx, buf, n.value, ctypes.byref(n)))
+        return [event_dict(buf[k]) for k in range(n.value)]
+
+    def rebuild_map(self):
+        n = ctypes.c_int(0)
+        self._chk(self.L.red_last_rebuild_map(self.ctx, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.int32)
+        self._chk(self.L.red_last_rebuild_map(self.ctx, out.ctypes.data_as(_IP), n.value, ctypes.byref(n)))
+        return out[:n.value]
+
+    def counters(self):
+        c = np.zeros(8, dtype=np.int32)
+        self._chk(self.L.red_counters(self.ctx, c.ctypes.data_as(_IP)))
+        keys = ("tried", "failed", "passed", "n_collision", "n_ejection", "n_hit_centrum",
+                "n_close_encounter", "n_event_last_clear")
+        return dict(zip(keys, (int(x) for x in c)))
+
+    PROF_CLASSES = ("si", "nsi", "ni", "stage", "err", "pack", "r6", "r7")
+
+    def set_profiling(self, on: bool):
+        self._chk(self.L.red_set_profiling(self.ctx, int(on)))
+
+    def profile(self, reset=True):
+        ms = (ctypes.c_double * 8)(); cnt = (ctypes.c_int * 8)()
+        self._chk(self.L.red_get_profile(self.ctx, ms, cnt, int(reset)))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.PROF_CLASSES)}
+
+    def energy(self):
+        E = ctypes.c_double(); L = (ctypes.c_double * 3)()
+        self._chk(self.L.red_energy(self.ctx, ctypes.byref(E), L))
+        return E.value, np.array(list(L))
