@@ -97,7 +97,11 @@ typedef struct red_params {
     int    block;                 /* gravity launch shape, 0 = autotuned/default    */
     int    sinks_per_thread;      /* 0 = default                                    */
     int    strategy;              /* 0 = auto, 1 = scalar-broadcast, 2 = LDS tiles  */
-    int    reserved[4];
+    int    fp64_split;            /* FP64: let sums split across source ranges for
+                                     occupancy (faster, not bit-identical to the
+                                     reference's sequential order); 0 = sequential  */
+    int    source_parts;          /* source-range split factor, 0 = automatic       */
+    int    reserved[2];
 } red_params;
 
 /* Bodies in body-file order (host arrays, read during the call only). */
@@ -209,6 +213,14 @@ int red_host_transform_velocity(int n, double* v);
 #define RED_PROF_CLASSES 8
 int red_set_profiling(red_ctx* ctx, int on);
 int red_get_profile(red_ctx* ctx, double ms[RED_PROF_CLASSES], int launches[RED_PROF_CLASSES], int reset);
+
+/* Device buffers for callers without an allocator of their own, on the context's device
+ * and stream (the reference's ALLOCATE_DEVICE_VECTOR / copy_vector_to_device /
+ * copy_vector_to_host, red.cuda/src/red.cuda.utility/util.cu:147-330). */
+int red_device_alloc(red_ctx* ctx, unsigned long long bytes, void** dev);
+int red_device_free(red_ctx* ctx, void* dev);
+int red_copy_to_device(red_ctx* ctx, void* dev, const void* host, unsigned long long bytes);
+int red_copy_to_host(red_ctx* ctx, void* host, const void* dev, unsigned long long bytes);
 
 /* Library version string. */
 const char* red_version(void);

@@ -199,6 +199,9 @@ struct red_ctx {
     void* d_scratch = nullptr;
     size_t scratch_bytes = 0;
     SrcMixed* d_src_mixed = nullptr;
+    d4* d_partial = nullptr;
+    size_t partial_elems = 0;
+    std::vector<void*> user_buffers;      // red_device_alloc
     BodyUpdate* d_updates = nullptr;
     int updates_cap = 0;
     double* d_energy = nullptr;
@@ -271,8 +274,9 @@ void refresh_pair_test(red_ctx* c)
     P.hill_f = c->prm.encounter_hill_factor;
     const double ms = mass_of_star(c);
     P.three_mstar = 3.0 * (ms > 0 ? ms : 1.0);
+    // the mixed sweep screens only sources past the FP64 head (star, giants)
     double rmax = 0.0;
-    for (int i = 0; i < c->n; i++) rmax = std::max(rmax, c->h_radius[i]);
+    for (int i = std::max(0, c->cnt[0] + c->cnt[1]); i < c->n; i++) rmax = std::max(rmax, c->h_radius[i]);
     P.rmax = rmax;
     P.close_encounter = c->prm.close_encounter;
     P.enabled = c->prm.inner_steps && (P.radii_f > 0.0 || P.hill_f > 0.0);
@@ -316,6 +320,8 @@ hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const
     B.mass = c->d_mass; B.radius = c->d_radius; B.id = c->d_id; B.ctl = c->d_ctl;
     B.scratch = c->d_scratch; B.scratch_bytes = c->scratch_bytes;
     B.src_mixed = c->d_src_mixed;
+    B.partial = c->d_partial;
+    B.partial_elems = c->partial_elems;
     EventSink E;
     E.buf = c->d_events; E.cap = c->ev_cap; E.stage = stage; E.c_stage = c_stage;
     if (c->gcfg.precision == RED_PREC_MIXED)
@@ -490,6 +496,7 @@ int apply_updates(red_ctx* c, const std::vector<BodyUpdate>& ups)
                                     c->d_radius, c->d_id, c->stream));
     CTX_HIP(c, hipStreamSynchronize(c->stream));
     refresh_pair_test(c);
+    destroy_graphs(c);   // captured launches hold the pair-test parameters by value
     return RED_OK;
 }
 
@@ -562,6 +569,8 @@ int red_ctx_destroy(red_ctx* c)
     (void)hipFree(c->d_mass); (void)hipFree(c->d_radius); (void)hipFree(c->d_id); (void)hipFree(c->d_type);
     (void)hipFree(c->d_ctl); (void)hipFree(c->d_events); (void)hipFree(c->d_events_hc); (void)hipFree(c->d_scratch);
     (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy);
+    (void)hipFree(c->d_partial);
+    for (void* p : c->user_buffers) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -612,11 +621,13 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     // launch configuration
     c->gcfg.precision = p->precision;
     c->gcfg.n_fp64_src = c->cnt[0] + c->cnt[1];
-    c->gcfg.block = p->block > 0 ? p->block : 256;
+    c->gcfg.block = p->block > 0 ? std::min(256, std::max(64, p->block / 64 * 64)) : 256;
     c->gcfg.spt = p->sinks_per_thread > 0 ? p->sinks_per_thread : (p->precision == RED_PREC_MIXED ? 2 : 1);
     c->gcfg.strategy = p->strategy > 0 ? p->strategy : 1;
     c->gcfg.kahan_pairs = p->kahan_pairs;
     c->gcfg.split_max_sinks = 2048;
+    c->gcfg.fp64_split = p->fp64_split;
+    c->gcfg.source_parts = p->source_parts;
     refresh_pair_test(c);
 
     hipError_t e = hipSetDevice(p->device);
@@ -652,6 +663,11 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     CTX_HIP(c, dmalloc(&c->d_events, c->ev_cap));
     CTX_HIP(c, dmalloc(&c->d_events_hc, std::max(1, n)));
     if (p->precision == RED_PREC_MIXED) CTX_HIP(c, dmalloc(&c->d_src_mixed, cap));
+    if (p->precision == RED_PREC_MIXED || p->fp64_split) {
+        // per-part sums: up to 64 parts of the largest sweep, capped at 1 GiB
+        c->partial_elems = std::min((size_t)64 * cap, ((size_t)1 << 30) / sizeof(d4));
+        CTX_HIP(c, dmalloc(&c->d_partial, c->partial_elems));
+    }
     CTX_HIP(c, dmalloc(&c->d_energy, 6 * (size_t)cap));
     // split-sweep scratch: the largest few-sink sweep
     {
@@ -1070,6 +1086,42 @@ int red_get_profile(red_ctx* c, double ms[RED_PROF_CLASSES], int launches[RED_PR
         if (launches) launches[k] = c->prof_n[k];
         if (reset) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
     }
+    return RED_OK;
+}
+
+int red_device_alloc(red_ctx* c, unsigned long long bytes, void** dev)
+{
+    if (!c || !dev) return RED_ERR_ARG;
+    *dev = nullptr;
+    CTX_HIP(c, hipMalloc(dev, bytes ? bytes : 1));
+    c->user_buffers.push_back(*dev);
+    return RED_OK;
+}
+
+int red_device_free(red_ctx* c, void* dev)
+{
+    if (!c) return RED_ERR_ARG;
+    auto it = std::find(c->user_buffers.begin(), c->user_buffers.end(), dev);
+    if (it == c->user_buffers.end()) return c->fail(RED_ERR_ARG, "pointer was not allocated by red_device_alloc");
+    c->user_buffers.erase(it);
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    CTX_HIP(c, hipFree(dev));
+    return RED_OK;
+}
+
+int red_copy_to_device(red_ctx* c, void* dev, const void* host, unsigned long long bytes)
+{
+    if (!c || (bytes && (!dev || !host))) return RED_ERR_ARG;
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    CTX_HIP(c, hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+    return RED_OK;
+}
+
+int red_copy_to_host(red_ctx* c, void* host, const void* dev, unsigned long long bytes)
+{
+    if (!c || (bytes && (!dev || !host))) return RED_ERR_ARG;
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    CTX_HIP(c, hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
     return RED_OK;
 }
 

@@ -4,7 +4,10 @@
 // (red.cuda/src/red.cuda/pp_disk.cu:144-296) and the launcher
 // pp_disk::call_kernel_calc_grav_accel (pp_disk.cu:609-681).
 //
-// Per sweep (SI x massive, NSI x SI, NI x massive) one of three paths runs:
+// Per sweep (SI x massive, NSI x SI, NI x massive) one of three paths runs.  Sweeps whose
+// sums may be reordered (mixed mode, or FP64 with fp64_split) are also cut into source
+// ranges ("parts", blockIdx.y) so that even 65536 sinks fill 256 CUs with about eight
+// wavefronts per SIMD; the parts are added in order by k_reduce_parts.
 //
 //  * broadcast sweep: a work-item owns SPT sinks, the j loop is wave-uniform so every
 //    source record {x, y, z, m} arrives through one 32-byte scalar load into SGPRs and
@@ -27,12 +30,15 @@
 // planets) stay in the FP64 path.  Pair tests stay exact: a conservative FP32 screen on
 // r^2 selects candidates and the reference predicate d < f (R_i + R_j) is re-evaluated
 // in FP64 for them, so event lists equal the FP64 path's for a given state.
+#include <algorithm>
+
 #include "red_internal.hpp"
 
 namespace red {
 namespace {
 
 constexpr int CHUNK = 64;   // mixed sweep: sources per FP32 partial sum
+
 
 __device__ __forceinline__ void put3(double* d, const d4& q) { d[0] = q.x; d[1] = q.y; d[2] = q.z; }
 
@@ -106,78 +112,71 @@ __device__ __forceinline__ void pair_test(const PairTest& P, const EventSink& E,
     if (collide || hill) append_pair_event(P, E, ctl, i, j, d, collide, r, v, mass, id);
 }
 
+
 // ---------------------------------------------------------------------------------------
-// FP64 broadcast sweep.
-template <int SPT, bool OVERLAP, bool EV>
-__global__ __launch_bounds__(256) void k_grav_fp64_bcast(const d4* __restrict__ r, const d4* __restrict__ v,
-                                                         d4* __restrict__ a, const double* __restrict__ mass,
-                                                         const double* __restrict__ radius,
-                                                         const int* __restrict__ id, Sweep sw, PairTest P,
-                                                         EventSink E, Ctl* __restrict__ ctl)
+// Source-range partition: part p of `parts` covers [lo, hi) of [j0, j1), slices rounded
+// to `align` sources.
+__device__ __forceinline__ void part_range(int j0, int j1, int parts, int p, int align, int& lo, int& hi)
 {
-    const int nt = blockDim.x;
-    const int base = sw.sink0 + blockIdx.x * nt * SPT + threadIdx.x;
-    int ii[SPT];
-    bool live[SPT];
-    double xi[SPT], yi[SPT], zi[SPT], Ri[SPT], ax[SPT], ay[SPT], az[SPT];
-#pragma unroll
-    for (int s = 0; s < SPT; s++) {
-        ii[s] = base + s * nt;
-        const bool in = ii[s] < sw.sink1;
-        const int k = in ? ii[s] : sw.sink0;
-        const d4 q = r[k];
-        xi[s] = q.x; yi[s] = q.y; zi[s] = q.z;
-        live[s] = in && id[k] > 0;
-        Ri[s] = EV ? radius[k] : 0.0;
-        ax[s] = ay[s] = az[s] = 0.0;
-    }
-    // Inactive sources (id < 0) and i == j are skipped in the reference (pp_disk.cu:242);
-    // here their term is replaced by +0, which leaves the sum bit-identical: the
-    // accumulator starts at +0 and x + (+-0) == x for every other value it can hold.
-#pragma unroll 2
-    for (int j = sw.src0; j < sw.src1; ++j) {
-        const bool act = id[j] >= 0;
-        const d4 q = r[j];
-#pragma unroll
-        for (int s = 0; s < SPT; s++) {
-            const double dx = q.x - xi[s];
-            const double dy = q.y - yi[s];
-            const double dz = q.z - zi[s];
-            const double r2 = (dx * dx + dy * dy) + dz * dz;
-            const double d = sqrt(r2);
-            const bool use = OVERLAP ? (act && j != ii[s]) : act;
-            const double sc = use ? q.w / (d * r2) : 0.0;
-            ax[s] += sc * dx;
-            ay[s] += sc * dy;
-            az[s] += sc * dz;
-            if (EV && use && live[s]) pair_test(P, E, ctl, ii[s], j, d, Ri[s], radius[j], r, v, mass, id);
-        }
-    }
-#pragma unroll
-    for (int s = 0; s < SPT; s++) {
-        if (ii[s] < sw.sink1) {
-            d4 o;
-            o.x = live[s] ? ax[s] : 0.0;
-            o.y = live[s] ? ay[s] : 0.0;
-            o.z = live[s] ? az[s] : 0.0;
-            o.w = 0.0;
-            a[ii[s]] = o;
-        }
+    const int n = j1 - j0;
+    int len = (n + parts - 1) / parts;
+    len = (len + align - 1) / align * align;
+    lo = min(j1, j0 + p * len);
+    hi = min(j1, lo + len);
+}
+
+// Writes a sink's sum: straight to a (masked by activity) or to its part slot.
+__device__ __forceinline__ void store_acc(d4* a, d4* partial, int parts, int p, const Sweep& sw, int i, bool live,
+                                          double x, double y, double z)
+{
+    if (parts == 1) {
+        a[i] = d4{live ? x : 0.0, live ? y : 0.0, live ? z : 0.0, 0.0};
+    } else {
+        partial[(size_t)p * (sw.sink1 - sw.sink0) + (i - sw.sink0)] = d4{x, y, z, 0.0};
     }
 }
 
-// FP64 LDS-tile sweep (same arithmetic, sources staged in LDS).
-template <int SPT, bool OVERLAP, bool EV, int TILE>
-__global__ __launch_bounds__(256) void k_grav_fp64_lds(const d4* __restrict__ r, const d4* __restrict__ v,
-                                                       d4* __restrict__ a, const double* __restrict__ mass,
-                                                       const double* __restrict__ radius,
-                                                       const int* __restrict__ id, Sweep sw, PairTest P,
-                                                       EventSink E, Ctl* __restrict__ ctl)
+// One FP64 pair in the reference's operation order.  Inactive sources (id < 0) and
+// i == j are skipped in the reference (pp_disk.cu:242); here their term is replaced by
+// +0, which leaves the sum bit-identical: the accumulator starts at +0 and
+// x + (+-0) == x for every other value it can hold.
+template <bool OVERLAP, bool EV>
+__device__ __forceinline__ void fp64_pair(const d4& q, int j, bool act, int i, double xi, double yi, double zi,
+                                          double Ri, bool live, double& ax, double& ay, double& az,
+                                          const PairTest& P, const EventSink& E, Ctl* ctl, const d4* r, const d4* v,
+                                          const double* mass, const double* radius, const int* id)
 {
-    __shared__ d4 tile[TILE];
-    __shared__ int tact[TILE];
+    const double dx = q.x - xi;
+    const double dy = q.y - yi;
+    const double dz = q.z - zi;
+    const double r2 = (dx * dx + dy * dy) + dz * dz;
+    const double d = sqrt(r2);
+    const bool use = OVERLAP ? (act && j != i) : act;
+    const double sc = use ? q.w / (d * r2) : 0.0;
+    ax += sc * dx;
+    ay += sc * dy;
+    az += sc * dz;
+    if (EV && use && live) pair_test(P, E, ctl, i, j, d, Ri, radius[j], r, v, mass, id);
+}
+
+// FP64 sweep.  LDS = false: every source record reaches the wavefront through a scalar
+// load (wave-uniform j); LDS = true: sources are staged in LDS tiles of TILE bodies.
+template <int SPT, bool OVERLAP, bool EV, bool LDS>
+__global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                   d4* __restrict__ a, d4* __restrict__ partial,
+                                                   const double* __restrict__ mass,
+                                                   const double* __restrict__ radius, const int* __restrict__ id,
+                                                   Sweep sw, int parts, PairTest P, EventSink E,
+                                                   Ctl* __restrict__ ctl)
+{
+    constexpr int TILE = 256;
+    __shared__ d4 tile[LDS ? TILE : 1];
+    __shared__ int tact[LDS ? TILE : 1];
     const int nt = blockDim.x;
+    const int p = blockIdx.y;
     const int base = sw.sink0 + blockIdx.x * nt * SPT + threadIdx.x;
+    int jlo, jhi;
+    part_range(sw.src0, sw.src1, parts, p, 1, jlo, jhi);
     int ii[SPT];
     bool live[SPT];
     double xi[SPT], yi[SPT], zi[SPT], Ri[SPT], ax[SPT], ay[SPT], az[SPT];
@@ -192,46 +191,39 @@ __global__ __launch_bounds__(256) void k_grav_fp64_lds(const d4* __restrict__ r,
         Ri[s] = EV ? radius[k] : 0.0;
         ax[s] = ay[s] = az[s] = 0.0;
     }
-    for (int j0 = sw.src0; j0 < sw.src1; j0 += TILE) {
-        const int nj = min(TILE, sw.src1 - j0);
-        __syncthreads();
-        for (int k = threadIdx.x; k < nj; k += nt) {
-            tile[k] = r[j0 + k];
-            tact[k] = id[j0 + k] >= 0;
-        }
-        __syncthreads();
+    if (!LDS) {
 #pragma unroll 2
-        for (int k = 0; k < nj; ++k) {
-            const bool act = tact[k] != 0;
-            const d4 q = tile[k];
-            const int j = j0 + k;
+        for (int j = jlo; j < jhi; ++j) {
+            const bool act = id[j] >= 0;
+            const d4 q = r[j];
 #pragma unroll
-            for (int s = 0; s < SPT; s++) {
-                const double dx = q.x - xi[s];
-                const double dy = q.y - yi[s];
-                const double dz = q.z - zi[s];
-                const double r2 = (dx * dx + dy * dy) + dz * dz;
-                const double d = sqrt(r2);
-                const bool use = OVERLAP ? (act && j != ii[s]) : act;
-                const double sc = use ? q.w / (d * r2) : 0.0;
-                ax[s] += sc * dx;
-                ay[s] += sc * dy;
-                az[s] += sc * dz;
-                if (EV && use && live[s]) pair_test(P, E, ctl, ii[s], j, d, Ri[s], radius[j], r, v, mass, id);
+            for (int s = 0; s < SPT; s++)
+                fp64_pair<OVERLAP, EV>(q, j, act, ii[s], xi[s], yi[s], zi[s], Ri[s], live[s], ax[s], ay[s], az[s], P,
+                                       E, ctl, r, v, mass, radius, id);
+        }
+    } else {
+        for (int j0 = jlo; j0 < jhi; j0 += TILE) {
+            const int nj = min(TILE, jhi - j0);
+            __syncthreads();
+            for (int k = threadIdx.x; k < nj; k += nt) {
+                tile[k] = r[j0 + k];
+                tact[k] = id[j0 + k] >= 0;
+            }
+            __syncthreads();
+#pragma unroll 2
+            for (int k = 0; k < nj; ++k) {
+                const d4 q = tile[k];
+                const bool act = tact[k] != 0;
+#pragma unroll
+                for (int s = 0; s < SPT; s++)
+                    fp64_pair<OVERLAP, EV>(q, j0 + k, act, ii[s], xi[s], yi[s], zi[s], Ri[s], live[s], ax[s], ay[s],
+                                           az[s], P, E, ctl, r, v, mass, radius, id);
             }
         }
     }
 #pragma unroll
-    for (int s = 0; s < SPT; s++) {
-        if (ii[s] < sw.sink1) {
-            d4 o;
-            o.x = live[s] ? ax[s] : 0.0;
-            o.y = live[s] ? ay[s] : 0.0;
-            o.z = live[s] ? az[s] : 0.0;
-            o.w = 0.0;
-            a[ii[s]] = o;
-        }
-    }
+    for (int s = 0; s < SPT; s++)
+        if (ii[s] < sw.sink1) store_acc(a, partial, parts, p, sw, ii[s], live[s], ax[s], ay[s], az[s]);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -246,14 +238,16 @@ __device__ __forceinline__ void kahan_add(double& sum, double& comp, double x)
 
 template <int SPT, bool OVERLAP, bool EV, bool KAHAN>
 __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__ r, const d4* __restrict__ v,
-                                                          d4* __restrict__ a, const double* __restrict__ mass,
+                                                          d4* __restrict__ a, d4* __restrict__ partial,
+                                                          const double* __restrict__ mass,
                                                           const double* __restrict__ radius,
                                                           const int* __restrict__ id,
-                                                          const SrcMixed* __restrict__ src, Sweep sw,
-                                                          int n_fp64_src, double rmax, PairTest P, EventSink E,
+                                                          const SrcMixed* __restrict__ src, Sweep sw, int parts,
+                                                          int n_fp64_src, PairTest P, EventSink E,
                                                           Ctl* __restrict__ ctl)
 {
     const int nt = blockDim.x;
+    const int p = blockIdx.y;
     const int base = sw.sink0 + blockIdx.x * nt * SPT + threadIdx.x;
     int ii[SPT];
     bool live[SPT];
@@ -272,33 +266,26 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
         cx[s] = cy[s] = cz[s] = 0.0;
         if (EV) {
             // conservative screen: FP32 r^2 is within ~5e-7 relative of the FP64 value
-            const double cut = P.radii_f * (Ri[s] + rmax) * 1.001;
+            const double cut = P.radii_f * (Ri[s] + P.rmax) * 1.001;
             cut2[s] = (float)(cut * cut);
         }
     }
-    // FP64 head: the star and giant planets (same arithmetic as the FP64 sweep)
+    // FP64 head (the star and giant planets), processed by part 0
     const int jsplit = min(sw.src1, max(sw.src0, n_fp64_src));
-    for (int j = sw.src0; j < jsplit; ++j) {
-        const bool act = id[j] >= 0;
-        const d4 q = r[j];
+    if (p == 0) {
+        for (int j = sw.src0; j < jsplit; ++j) {
+            const bool act = id[j] >= 0;
+            const d4 q = r[j];
 #pragma unroll
-        for (int s = 0; s < SPT; s++) {
-            const double dx = q.x - xi[s];
-            const double dy = q.y - yi[s];
-            const double dz = q.z - zi[s];
-            const double r2 = (dx * dx + dy * dy) + dz * dz;
-            const double d = sqrt(r2);
-            const bool use = OVERLAP ? (act && j != ii[s]) : act;
-            const double sc = use ? q.w / (d * r2) : 0.0;
-            ax[s] += sc * dx;
-            ay[s] += sc * dy;
-            az[s] += sc * dz;
-            if (EV && use && live[s]) pair_test(P, E, ctl, ii[s], j, d, Ri[s], radius[j], r, v, mass, id);
+            for (int s = 0; s < SPT; s++)
+                fp64_pair<OVERLAP, EV>(q, j, act, ii[s], xi[s], yi[s], zi[s], Ri[s], live[s], ax[s], ay[s], az[s], P,
+                                       E, ctl, r, v, mass, radius, id);
         }
     }
-    // FP32 body in chunks
-    for (int j0 = jsplit; j0 < sw.src1; j0 += CHUNK) {
-        const int j1 = min(sw.src1, j0 + CHUNK);
+    int jlo, jhi;
+    part_range(jsplit, sw.src1, parts, p, CHUNK, jlo, jhi);
+    for (int j0 = jlo; j0 < jhi; j0 += CHUNK) {
+        const int j1 = min(jhi, j0 + CHUNK);
         float sx[SPT], sy[SPT], sz[SPT], kx[SPT], ky[SPT], kz[SPT], mn[SPT];
 #pragma unroll
         for (int s = 0; s < SPT; s++) {
@@ -353,16 +340,25 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
         }
     }
 #pragma unroll
-    for (int s = 0; s < SPT; s++) {
-        if (ii[s] < sw.sink1) {
-            d4 o;
-            o.x = live[s] ? ax[s] : 0.0;
-            o.y = live[s] ? ay[s] : 0.0;
-            o.z = live[s] ? az[s] : 0.0;
-            o.w = 0.0;
-            a[ii[s]] = o;
-        }
+    for (int s = 0; s < SPT; s++)
+        if (ii[s] < sw.sink1) store_acc(a, partial, parts, p, sw, ii[s], live[s], ax[s], ay[s], az[s]);
+}
+
+// Sums the parts of a split sweep in part order (deterministic).
+__global__ __launch_bounds__(256) void k_reduce_parts(const d4* __restrict__ partial, d4* __restrict__ a,
+                                                      const int* __restrict__ id, Sweep sw, int parts)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nsink = sw.sink1 - sw.sink0;
+    if (k >= nsink) return;
+    double x = 0.0, y = 0.0, z = 0.0;
+    for (int p = 0; p < parts; p++) {
+        const d4 q = partial[(size_t)p * nsink + k];
+        x += q.x; y += q.y; z += q.z;
     }
+    const int i = sw.sink0 + k;
+    const bool live = id[i] > 0;
+    a[i] = d4{live ? x : 0.0, live ? y : 0.0, live ? z : 0.0, 0.0};
 }
 
 __global__ void k_zero_acc(d4* __restrict__ a, int n)
@@ -384,7 +380,7 @@ __global__ void k_pack_mixed(const d4* __restrict__ r, const int* __restrict__ i
 }
 
 // ---------------------------------------------------------------------------------------
-// Split sweep, phase 1: one work-item per source j, SG sinks per block row.
+// Split sweep, phase 1: one work-item per source j, SPLIT_SG sinks per block row.
 constexpr int SPLIT_SG = 16;
 
 template <bool EV>
@@ -410,9 +406,8 @@ __global__ __launch_bounds__(256) void k_split_terms(const d4* __restrict__ r, c
         const double dz = q.z - p.z;
         const double r2 = (dx * dx + dy * dy) + dz * dz;
         const double d = sqrt(r2);
-        double sc = q.w / (d * r2);
         const bool use = act && j != i;
-        sc = use ? sc : 0.0;
+        const double sc = use ? q.w / (d * r2) : 0.0;
         double* row = T + (size_t)(i - sw.sink0) * 3 * nsrc;
         row[jj] = sc * dx;
         row[nsrc + jj] = sc * dy;
@@ -422,8 +417,8 @@ __global__ __launch_bounds__(256) void k_split_terms(const d4* __restrict__ r, c
 }
 
 // Split sweep, phase 2: one block per (sink, component).  SEQ: a single work-item adds
-// the terms in ascending j (the reference order); otherwise 256 contiguous chunks are
-// summed in parallel and the chunk sums added in order.
+// the terms in ascending j (the reference order); otherwise contiguous chunks are summed
+// in parallel and the chunk sums added in order.
 template <bool SEQ>
 __global__ __launch_bounds__(256) void k_split_sum(const double* __restrict__ T, d4* __restrict__ a,
                                                    const int* __restrict__ id, Sweep sw)
@@ -464,53 +459,52 @@ __global__ __launch_bounds__(256) void k_split_sum(const double* __restrict__ T,
 }
 
 template <int SPT, bool OV, bool EV>
-hipError_t go_fp64(int strategy, dim3 g, dim3 b, const Sweep& sw, const GravBuffers& B, const PairTest& P,
-                   const EventSink& E, hipStream_t s)
+hipError_t go_fp64(int strategy, dim3 g, dim3 b, const Sweep& sw, const GravBuffers& B, int parts,
+                   const PairTest& P, const EventSink& E, hipStream_t s)
 {
     if (strategy == 2)
-        hipLaunchKernelGGL((k_grav_fp64_lds<SPT, OV, EV, 256>), g, b, 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
-                           sw, P, E, B.ctl);
+        hipLaunchKernelGGL((k_grav_fp64<SPT, OV, EV, true>), g, b, 0, s, B.r, B.v, B.a, B.partial, B.mass, B.radius,
+                           B.id, sw, parts, P, E, B.ctl);
     else
-        hipLaunchKernelGGL((k_grav_fp64_bcast<SPT, OV, EV>), g, b, 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id, sw,
-                           P, E, B.ctl);
+        hipLaunchKernelGGL((k_grav_fp64<SPT, OV, EV, false>), g, b, 0, s, B.r, B.v, B.a, B.partial, B.mass, B.radius,
+                           B.id, sw, parts, P, E, B.ctl);
     return hipGetLastError();
 }
 
 template <int SPT, bool OV, bool EV, bool KH>
-hipError_t go_mixed(dim3 g, dim3 b, const Sweep& sw, const GravBuffers& B, const GravConfig& C,
-                    const PairTest& P, const EventSink& E, double rmax, hipStream_t s)
+hipError_t go_mixed(dim3 g, dim3 b, const Sweep& sw, const GravBuffers& B, const GravConfig& C, int parts,
+                    const PairTest& P, const EventSink& E, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_grav_mixed_bcast<SPT, OV, EV, KH>), g, b, 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
-                       B.src_mixed, sw, C.n_fp64_src, rmax, P, E, B.ctl);
+    hipLaunchKernelGGL((k_grav_mixed_bcast<SPT, OV, EV, KH>), g, b, 0, s, B.r, B.v, B.a, B.partial, B.mass,
+                       B.radius, B.id, B.src_mixed, sw, parts, C.n_fp64_src, P, E, B.ctl);
     return hipGetLastError();
 }
 
 template <int SPT>
-hipError_t dispatch_tile(const Sweep& sw, const GravBuffers& B, const GravConfig& C, const PairTest& P,
+hipError_t dispatch_tile(const Sweep& sw, const GravBuffers& B, const GravConfig& C, int parts, const PairTest& P,
                          const EventSink& E, hipStream_t s)
 {
-    const double rmax = P.rmax;
     const int nsink = sw.sink1 - sw.sink0;
     const int bs = C.block > 0 ? C.block : 256;
-    dim3 b(bs), g((nsink + bs * SPT - 1) / (bs * SPT));
+    dim3 b(bs), g((nsink + bs * SPT - 1) / (bs * SPT), parts);
     const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
     const bool ev = P.enabled != 0;
     if (C.precision == RED_PREC_MIXED) {
         if (C.kahan_pairs) {
-            if (ov) return ev ? go_mixed<SPT, true, true, true>(g, b, sw, B, C, P, E, rmax, s)
-                              : go_mixed<SPT, true, false, true>(g, b, sw, B, C, P, E, rmax, s);
-            return ev ? go_mixed<SPT, false, true, true>(g, b, sw, B, C, P, E, rmax, s)
-                      : go_mixed<SPT, false, false, true>(g, b, sw, B, C, P, E, rmax, s);
+            if (ov) return ev ? go_mixed<SPT, true, true, true>(g, b, sw, B, C, parts, P, E, s)
+                              : go_mixed<SPT, true, false, true>(g, b, sw, B, C, parts, P, E, s);
+            return ev ? go_mixed<SPT, false, true, true>(g, b, sw, B, C, parts, P, E, s)
+                      : go_mixed<SPT, false, false, true>(g, b, sw, B, C, parts, P, E, s);
         }
-        if (ov) return ev ? go_mixed<SPT, true, true, false>(g, b, sw, B, C, P, E, rmax, s)
-                          : go_mixed<SPT, true, false, false>(g, b, sw, B, C, P, E, rmax, s);
-        return ev ? go_mixed<SPT, false, true, false>(g, b, sw, B, C, P, E, rmax, s)
-                  : go_mixed<SPT, false, false, false>(g, b, sw, B, C, P, E, rmax, s);
+        if (ov) return ev ? go_mixed<SPT, true, true, false>(g, b, sw, B, C, parts, P, E, s)
+                          : go_mixed<SPT, true, false, false>(g, b, sw, B, C, parts, P, E, s);
+        return ev ? go_mixed<SPT, false, true, false>(g, b, sw, B, C, parts, P, E, s)
+                  : go_mixed<SPT, false, false, false>(g, b, sw, B, C, parts, P, E, s);
     }
-    if (ov) return ev ? go_fp64<SPT, true, true>(C.strategy, g, b, sw, B, P, E, s)
-                      : go_fp64<SPT, true, false>(C.strategy, g, b, sw, B, P, E, s);
-    return ev ? go_fp64<SPT, false, true>(C.strategy, g, b, sw, B, P, E, s)
-              : go_fp64<SPT, false, false>(C.strategy, g, b, sw, B, P, E, s);
+    if (ov) return ev ? go_fp64<SPT, true, true>(C.strategy, g, b, sw, B, parts, P, E, s)
+                      : go_fp64<SPT, true, false>(C.strategy, g, b, sw, B, parts, P, E, s);
+    return ev ? go_fp64<SPT, false, true>(C.strategy, g, b, sw, B, parts, P, E, s)
+              : go_fp64<SPT, false, false>(C.strategy, g, b, sw, B, parts, P, E, s);
 }
 
 }  // namespace
@@ -525,6 +519,26 @@ hipError_t launch_pack_mixed(const d4* r, const int* id, SrcMixed* out, int n, h
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_pack_mixed, dim3((n + 255) / 256), dim3(256), 0, s, r, id, out, n);
     return hipGetLastError();
+}
+
+int sweep_parts(const Sweep& sw, const GravConfig& C, size_t partial_elems)
+{
+    const int nsink = sw.sink1 - sw.sink0;
+    const int nsrc = sw.src1 - sw.src0;
+    if (nsink <= 0 || nsrc <= 0) return 1;
+    const bool sequential = C.precision == RED_PREC_FP64 && !C.fp64_split;
+    if (sequential) return 1;
+    int parts = C.source_parts;
+    if (parts <= 0) {
+        // aim for about eight wavefronts per SIMD (256 CUs x 4 SIMDs)
+        const int spt = C.spt > 0 ? C.spt : 1;
+        const int waves = (nsink + 64 * spt - 1) / (64 * spt);
+        parts = (8 * 1024 + waves - 1) / waves;
+        parts = std::min(parts, std::max(1, nsrc / 512));
+    }
+    parts = std::max(1, std::min(parts, 64));
+    while (parts > 1 && (size_t)parts * nsink > partial_elems) parts--;
+    return parts;
 }
 
 hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig& C, const PairTest& P,
@@ -546,7 +560,7 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
                                E, B.ctl, T);
         RED_HIP_TRY(hipGetLastError());
         dim3 g2(nsink * 3);
-        if (C.precision == RED_PREC_FP64)
+        if (C.precision == RED_PREC_FP64 && !C.fp64_split)
             hipLaunchKernelGGL(k_split_sum<true>, g2, dim3(64), 0, s, T, B.a, B.id, sw);
         else
             hipLaunchKernelGGL(k_split_sum<false>, g2, dim3(256), 0, s, T, B.a, B.id, sw);
@@ -557,11 +571,19 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
         hipLaunchKernelGGL(k_zero_acc, dim3((nsink + 255) / 256), dim3(256), 0, s, B.a + sw.sink0, nsink);
         return hipGetLastError();
     }
+    const int parts = sweep_parts(sw, C, B.partial_elems);
+    hipError_t e;
     switch (C.spt) {
-    case 2: return dispatch_tile<2>(sw, B, C, P, E, s);
-    case 4: return dispatch_tile<4>(sw, B, C, P, E, s);
-    default: return dispatch_tile<1>(sw, B, C, P, E, s);
+    case 2: e = dispatch_tile<2>(sw, B, C, parts, P, E, s); break;
+    case 4: e = dispatch_tile<4>(sw, B, C, parts, P, E, s); break;
+    default: e = dispatch_tile<1>(sw, B, C, parts, P, E, s); break;
     }
+    RED_HIP_TRY(e);
+    if (parts > 1) {
+        hipLaunchKernelGGL(k_reduce_parts, dim3((nsink + 255) / 256), dim3(256), 0, s, B.partial, B.a, B.id, sw, parts);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
 }  // namespace red

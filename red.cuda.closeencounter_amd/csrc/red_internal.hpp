@@ -84,6 +84,8 @@ struct GravBuffers {
     void* scratch;                 // split-sweep scratch
     size_t scratch_bytes;
     SrcMixed* src_mixed;           // packed sources for the mixed sweep (cap entries)
+    d4* partial;                   // per-part sums of source-partitioned sweeps
+    size_t partial_elems;
 };
 
 struct GravConfig {
@@ -94,7 +96,12 @@ struct GravConfig {
     int strategy;                  // 1 scalar broadcast, 2 LDS tiles
     int kahan_pairs;
     int split_max_sinks;           // sweeps with fewer sinks use the split path
+    int fp64_split;                // FP64 sums may be partitioned over sources
+    int source_parts;              // forced part count, 0 = automatic
 };
+
+// Number of source parts a sweep is cut into.
+int sweep_parts(const Sweep& sw, const GravConfig& C, size_t partial_elems);
 
 // Launches one force sweep (all three classes are separate sweeps).
 hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig& C, const PairTest& P,

@@ -55,7 +55,7 @@ class RedParams(ctypes.Structure):
                 ("dt_day", ctypes.c_double), ("precision", ctypes.c_int), ("event_capacity", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("kahan_pairs", ctypes.c_int), ("device", ctypes.c_int),
                 ("block", ctypes.c_int), ("sinks_per_thread", ctypes.c_int), ("strategy", ctypes.c_int),
-                ("reserved", ctypes.c_int * 4)]
+                ("fp64_split", ctypes.c_int), ("source_parts", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
 
 
 class RedBodies(ctypes.Structure):
@@ -105,6 +105,10 @@ SIGNATURES = {
     "red_host_transform_velocity": (_I, [_I, _P]),
     "red_set_profiling": (_I, [_P, _I]),
     "red_get_profile": (_I, [_P, _DP, _IP, _I]),
+    "red_device_alloc": (_I, [_P, ctypes.c_ulonglong, ctypes.POINTER(_P)]),
+    "red_device_free": (_I, [_P, _P]),
+    "red_copy_to_device": (_I, [_P, _P, _P, ctypes.c_ulonglong]),
+    "red_copy_to_host": (_I, [_P, _P, _P, ctypes.c_ulonglong]),
     "red_version": (ctypes.c_char_p, []),
 }
 
@@ -162,7 +166,7 @@ class Disk:
                  error_check_for_tp=False, close_encounter=False, inner_steps=True, frame_center=1,
                  hit_centrum=0.0, ejection=0.0, radii_enhance_factor=0.0, encounter_hill_factor=0.0,
                  precision=PREC_FP64, event_capacity=0, use_graph=False, kahan_pairs=False, device=0,
-                 block=0, sinks_per_thread=0, strategy=0, r=None, v=None):
+                 block=0, sinks_per_thread=0, strategy=0, fp64_split=False, source_parts=0, r=None, v=None):
         self.L = load_library()
         self.ctx = ctypes.c_void_p()
         prm = RedParams()
@@ -186,6 +190,8 @@ class Disk:
         prm.block = block
         prm.sinks_per_thread = sinks_per_thread
         prm.strategy = strategy
+        prm.fp64_split = int(fp64_split)
+        prm.source_parts = source_parts
         self._keep = dict(
             id=np.ascontiguousarray(bodies.ids, dtype=np.int32),
             type=np.ascontiguousarray(bodies.types, dtype=np.int32),
@@ -284,22 +290,41 @@ class Disk:
         self._chk(self.L.red_accel(self.ctx, float(t), int(stage), ctypes.c_void_p(r_dev),
                                    ctypes.c_void_p(v_dev), ctypes.c_void_p(a_dev)))
 
+    def device_alloc(self, nbytes: int) -> int:
+        p = ctypes.c_void_p()
+        self._chk(self.L.red_device_alloc(self.ctx, nbytes, ctypes.byref(p)))
+        return p.value
+
+    def device_free(self, dev: int):
+        self._chk(self.L.red_device_free(self.ctx, ctypes.c_void_p(dev)))
+
+    def to_device(self, dev: int, host: np.ndarray):
+        host = np.ascontiguousarray(host)
+        self._chk(self.L.red_copy_to_device(self.ctx, ctypes.c_void_p(dev), _ptr(host), host.nbytes))
+
+    def to_host(self, host: np.ndarray, dev: int):
+        self._chk(self.L.red_copy_to_host(self.ctx, _ptr(host), ctypes.c_void_p(dev), host.nbytes))
+
     def accel(self, t, r, v, stage=0):
-        """Force evaluation through red_accel with device buffers from torch."""
-        import torch
+        """Force evaluation through red_accel on device buffers of the library's allocator."""
         n = len(r)
         rr = np.zeros((n, 4)); vv = np.zeros((n, 4))
         rr[:, :3] = r; vv[:, :3] = v
-        st = self.state()
-        rr[:, 3] = st["p"][:, 0]
-        dev = torch.device("cuda", int(self.params.device))
-        tr = torch.from_numpy(rr).to(dev)
-        tv = torch.from_numpy(vv).to(dev)
-        ta = torch.full((n, 4), float("nan"), dtype=torch.float64, device=dev)
-        torch.cuda.synchronize(dev)
-        self.accel_device(t, stage, tr.data_ptr(), tv.data_ptr(), ta.data_ptr())
-        self.sync()
-        return ta.cpu().numpy()[:, :3].copy()
+        rr[:, 3] = self.state()["p"][:, 0]
+        nb = n * 32
+        bufs = [self.device_alloc(nb) for _ in range(3)]
+        try:
+            self.to_device(bufs[0], rr)
+            self.to_device(bufs[1], vv)
+            self.to_device(bufs[2], np.full((n, 4), np.nan))
+            self.accel_device(t, stage, bufs[0], bufs[1], bufs[2])
+            self.sync()
+            out = np.empty((n, 4))
+            self.to_host(out, bufs[2])
+        finally:
+            for b in bufs:
+                self.device_free(b)
+        return out[:, :3].copy()
 
     # ---- integration and events -----------------------------------------------------
     def step(self) -> float:

@@ -83,14 +83,28 @@ def test_fp64_events_bit_exact(oracle_mod, native_mod):
 
 
 def test_fp64_hill_events_bit_exact(oracle_mod, native_mod):
-    b = generate.to_internal(generate.disk_small(400, seed=8, a_range=(1.0, 1.2)))
-    print("case hill n=403", flush=True)
-    o, d = _pair(oracle_mod, native_mod, b, radii_enhance_factor=1.0, encounter_hill_factor=3.0,
+    # protoplanets are self-interacting, so their pairs meet the Hill test inside the sweep
+    b = generate.to_internal(generate.disk_allsi(600, seed=8, a_range=(1.0, 1.2), m_range=(1e-11, 1e-9)))
+    print("case hill n=600", flush=True)
+    o, d = _pair(oracle_mod, native_mod, b, radii_enhance_factor=1.0, encounter_hill_factor=30.0,
                  event_capacity=100000)
     assert_bitwise(d.accel(0.0, b.r, b.v), o.accel(0.0, b.r, b.v), "accel")
     ev_o, ev_g = o.raw_events(), d.raw_events()
     assert len(ev_o) > 0
     _events_equal(ev_o, ev_g)
+
+
+@pytest.mark.parametrize("parts", [0, 3, 16])
+def test_fp64_split_sums_close(oracle_mod, native_mod, parts):
+    """FP64 with source-partitioned sums: same terms, different summation order."""
+    b = generate.to_internal(generate.disk_mixed(n_pl=3000, n_tp=5000, seed=6))
+    print(f"case fp64_split parts={parts}", flush=True)
+    o = oracle_mod.Oracle(b)
+    d = native_mod.Disk(b, fp64_split=True, source_parts=parts)
+    a_o = o.accel(0.0, b.r, b.v)
+    a_g = d.accel(0.0, b.r, b.v)
+    rel = np.linalg.norm(a_g - a_o, axis=1) / np.maximum(np.linalg.norm(a_o, axis=1), 1e-300)
+    assert rel.max() < 1e-13, rel.max()
 
 
 @pytest.mark.parametrize("kahan", [False, True])
