@@ -553,15 +553,21 @@ struct Disk {
         return false;
     }
 
-    // pp_disk.cu:717-738 + 1078-1089
+    // pp_disk.cu:717-738 + 1078-1089.  Close-encounter records (NEW Hill criterion) are
+    // reported and counted, not merged.
     bool check_for_collision() {
         if (0 < (int)events.size()) {
             create_sp_events();
             for (size_t i = 0; i < sp_events.size(); i++) {
-                handle_collision_pair(&sp_events[i]);
-                increment_event_counter(n_collision);
+                if (sp_events[i].name == EV_COLLISION) {
+                    handle_collision_pair(&sp_events[i]);
+                    increment_event_counter(n_collision);
+                } else {
+                    increment_event_counter(n_close_encounter);
+                }
             }
             n_collision[CNT_LAST_STEP] = 0;
+            n_close_encounter[CNT_LAST_STEP] = 0;
             return true;
         }
         return false;

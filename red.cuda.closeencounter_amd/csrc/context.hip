@@ -939,6 +939,10 @@ int red_check_collision(red_ctx* c, int* handled)
         }
         c->sp.push_back(cur);
     }
+    // close-encounter records take the parameters the bodies had at detection, i.e.
+    // before any merge of this pass
+    for (auto& e : c->sp)
+        if (e.name != RED_EV_COLLISION) close_encounter_elements(c, &e);
     std::vector<BodyUpdate> ups;
     for (auto& e : c->sp) {
         if (e.name == RED_EV_COLLISION) {
@@ -946,7 +950,6 @@ int red_check_collision(red_ctx* c, int* handled)
             if (rc) return rc;
             increment_event_counter(c, c->n_collision);
         } else {
-            close_encounter_elements(c, &e);
             increment_event_counter(c, c->n_close_encounter);
         }
     }
