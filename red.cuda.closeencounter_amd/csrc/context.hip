@@ -808,9 +808,9 @@ int red_step(red_ctx* c, red_step_info* info)
         if (c->h_ctl->n_events > c->ev_cap)
             return c->fail(RED_ERR_EVENT_OVERFLOW, "pair events exceed the event capacity");
         // integrator.cu:255 and the controllers (int_rungekutta8.cu:776-777 etc.)
-        max_err = std::fabs(c->dt_try * S.lambda * bits_to_double(c->h_ctl->err_bits));
+        max_err = tools::controller_update(&c->dt_try, S.lambda, bits_to_double(c->h_ctl->err_bits),
+                                           c->prm.tolerance, S.exponent);
         c->last_err = max_err;
-        c->dt_try *= 0.9 * std::pow(c->prm.tolerance / max_err, S.exponent);
         if (S.accept_on_event && c->h_ctl->n_events > 0) {
             if (c->dt_try < c->dt_did) c->dt_try = c->dt_did;
             break;   // leaves before iter++ as int_rungekutta4.cu:255-264 does
@@ -891,13 +891,18 @@ int red_check_ejection_hit_centrum(red_ctx* c, int* handled)
     for (auto& e : ev) c->h_id[e.idx2] = -c->h_id[e.idx2];   // mirror the kernel's negation
     std::vector<BodyUpdate> ups;
     c->sp = ev;
+    // ejection records carry the parameters at detection (pp_disk.cu:58-66), before any
+    // hit-centrum merge of this pass changes the star
+    for (auto& e : c->sp) {
+        if (e.name != RED_EV_EJECTION) continue;
+        const int i = e.idx2;
+        const double p0[4] = {c->h_mass[0], c->h_radius[0], c->h_density[0], c->h_cd[0]};
+        const double pi[4] = {c->h_mass[i], c->h_radius[i], c->h_density[i], c->h_cd[i]};
+        memcpy(e.p1, p0, sizeof(p0)); memcpy(e.p2, pi, sizeof(pi));
+        memcpy(e.rs, e.r1, sizeof(e.rs)); memcpy(e.vs, e.v1, sizeof(e.vs)); memcpy(e.ps, e.p1, sizeof(e.ps));
+    }
     for (auto& e : c->sp) {
         if (e.name == RED_EV_EJECTION) {
-            const int i = e.idx2;
-            const double p0[4] = {c->h_mass[0], c->h_radius[0], c->h_density[0], c->h_cd[0]};
-            const double pi[4] = {c->h_mass[i], c->h_radius[i], c->h_density[i], c->h_cd[i]};
-            memcpy(e.p1, p0, sizeof(p0)); memcpy(e.p2, pi, sizeof(pi));
-            memcpy(e.rs, e.r1, sizeof(e.rs)); memcpy(e.vs, e.v1, sizeof(e.vs)); memcpy(e.ps, e.p1, sizeof(e.ps));
             increment_event_counter(c, c->n_ejection);
         } else {
             rc = handle_collision_pair(c, &e, true, ups);
