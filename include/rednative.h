@@ -222,6 +222,32 @@ int red_device_free(red_ctx* ctx, void* dev);
 int red_copy_to_device(red_ctx* ctx, void* dev, const void* host, unsigned long long bytes);
 int red_copy_to_host(red_ctx* ctx, void* host, const void* dev, unsigned long long bytes);
 
+/* ---- ensembles of independent disks ---------------------------------------------
+ * Not in the reference (which integrates one disk per process): many disks of uneven size
+ * share one set of device buffers; every stage of an attempt is one launch over all disks,
+ * the attempt is captured once in a HIP graph (params->use_graph) and replayed, and each
+ * disk runs the reference's step control (integrator.cu:211-263, int_rungekutta8.cu:704-797)
+ * on its own time and trial step, so it takes exactly the steps it would take alone.
+ * FP64 only; pair tests are not applied inside an ensemble. */
+typedef struct red_ensemble red_ensemble;
+typedef struct red_ensemble_info {
+    int    supersteps;            /* attempts launched (one graph replay each)      */
+    int    active_disks;          /* disks still integrating after the call         */
+    double seconds;               /* wall time of the call                          */
+} red_ensemble_info;
+int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* params, red_ensemble** out);
+int red_ensemble_destroy(red_ensemble* ens);
+const char* red_ensemble_last_error(const red_ensemble* ens);
+int red_ensemble_set_time(red_ensemble* ens, double t);
+/* Steps every disk while its t <= t_stop (main.cu:176), at most max_attempts attempts. */
+int red_ensemble_advance(red_ensemble* ens, double t_stop, int max_attempts, red_ensemble_info* info);
+int red_ensemble_get_state(red_ensemble* ens, int disk, double* r, double* v);
+/* out[3] = tried failed passed of one disk; its time and trial step. */
+int red_ensemble_disk_info(red_ensemble* ens, int disk, int out[3], double* t, double* dt_try);
+/* Picks the force-launch shape (threads per block x source tile, each in {64,128,256}) by
+ * timing `reps` force evaluations per shape; ms[9] receives the timings (block-major). */
+int red_ensemble_autotune(red_ensemble* ens, int reps, int* best_block, int* best_jtile, double ms[9]);
+
 /* Library version string. */
 const char* red_version(void);
 

@@ -96,7 +96,14 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= A.n) return;
-    const double dt = A.ctl->dt;
+    double dt;
+    if (A.disk_of) {
+        const int d = A.disk_of[i];
+        if (!(A.disk_flags[d] & 1)) return;   // disk done or waiting: its buffers stay as they are
+        dt = A.disk_dt[d];
+    } else {
+        dt = A.ctl->dt;
+    }
     const d4 yr = A.yr[i];
     const d4 yv = A.yv[i];
     d4 orr = combine<FORM>(A, yr, i, false, dt);
@@ -140,6 +147,42 @@ __device__ __forceinline__ void upd_max(double& m, int& nan, double e)
 {
     if (e != e) nan = 1;
     else if (m < e) m = e;
+}
+
+// Ensemble form: every body's maximum goes to its disk's slot.  A wavefront whose lanes
+// all belong to one disk reduces first and issues one atomic.
+template <int EF>
+__global__ __launch_bounds__(256) void k_err_disks(StageArgs A)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    int nan = 0;
+    int d = -1;
+    if (i < A.n) {
+        d = A.disk_of[i];
+        if ((A.disk_flags[d] & 1) && A.err_on[i]) {
+            const double3 er = err_rec<EF>(A, i, false);
+            const double3 ev = err_rec<EF>(A, i, true);
+            upd_max(m, nan, er.x); upd_max(m, nan, er.y); upd_max(m, nan, er.z);
+            upd_max(m, nan, ev.x); upd_max(m, nan, ev.y); upd_max(m, nan, ev.z);
+        }
+    }
+    const int d0 = __shfl(d, 0, 64);
+    const bool uniform = __all(d == d0 || d < 0);
+    if (uniform) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(m, off, 64);
+            m = m < o ? o : m;
+            nan |= __shfl_xor(nan, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0 && d0 >= 0) {
+            if (m > 0.0) atomicMax(&A.disk_err[d0], (unsigned long long)__double_as_longlong(m));
+            if (nan) atomicOr(&A.disk_nan[d0], 1);
+        }
+    } else if (d >= 0) {
+        if (m > 0.0) atomicMax(&A.disk_err[d], (unsigned long long)__double_as_longlong(m));
+        if (nan) atomicOr(&A.disk_nan[d], 1);
+    }
 }
 
 template <int EF>
@@ -202,6 +245,15 @@ hipError_t launch_err(int form, const StageArgs& A, hipStream_t s)
 {
     if (A.n <= 0) return hipSuccess;
     dim3 g((A.n + 255) / 256), b(256);
+    if (A.disk_of) {
+        switch (form) {
+        case EF_RK4: hipLaunchKernelGGL(k_err_disks<EF_RK4>, g, b, 0, s, A); break;
+        case EF_RKF78: hipLaunchKernelGGL(k_err_disks<EF_RKF78>, g, b, 0, s, A); break;
+        case EF_CK: hipLaunchKernelGGL(k_err_disks<EF_CK>, g, b, 0, s, A); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     Ctl* ctl = const_cast<Ctl*>(A.ctl);
     switch (form) {
     case EF_RK4: hipLaunchKernelGGL(k_err<EF_RK4>, g, b, 0, s, A, ctl); break;

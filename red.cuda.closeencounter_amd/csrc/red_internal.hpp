@@ -132,6 +132,15 @@ struct StageArgs {
     int nterms;
     double c1, c2;
     const Ctl* ctl;
+    // ensembles of independent disks (null for one disk): per-body disk index, per-disk
+    // trial step and flags (bit 0: the disk takes part in this attempt), per-body error-norm
+    // membership, per-disk error maximum and NaN flag
+    const int* disk_of;
+    const double* disk_dt;
+    const int* disk_flags;
+    const unsigned char* err_on;
+    unsigned long long* disk_err;
+    int* disk_nan;
     StageTerm t[MAX_TERMS];
 };
 

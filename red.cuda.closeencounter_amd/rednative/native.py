@@ -109,8 +109,20 @@ SIGNATURES = {
     "red_device_free": (_I, [_P, _P]),
     "red_copy_to_device": (_I, [_P, _P, _P, ctypes.c_ulonglong]),
     "red_copy_to_host": (_I, [_P, _P, _P, ctypes.c_ulonglong]),
+    "red_ensemble_create": (_I, [_I, _P, ctypes.POINTER(RedParams), ctypes.POINTER(_P)]),
+    "red_ensemble_destroy": (_I, [_P]),
+    "red_ensemble_last_error": (ctypes.c_char_p, [_P]),
+    "red_ensemble_set_time": (_I, [_P, _D]),
+    "red_ensemble_advance": (_I, [_P, _D, _I, _P]),
+    "red_ensemble_get_state": (_I, [_P, _I, _P, _P]),
+    "red_ensemble_disk_info": (_I, [_P, _I, _IP, _DP, _DP]),
+    "red_ensemble_autotune": (_I, [_P, _I, _IP, _IP, _DP]),
     "red_version": (ctypes.c_char_p, []),
 }
+
+
+class RedEnsembleInfo(ctypes.Structure):
+    _fields_ = [("supersteps", ctypes.c_int), ("active_disks", ctypes.c_int), ("seconds", ctypes.c_double)]
 
 _lib = None
 
@@ -402,3 +414,90 @@ class Disk:
         E = ctypes.c_double(); L = (ctypes.c_double * 3)()
         self._chk(self.L.red_energy(self.ctx, ctypes.byref(E), L))
         return E.value, np.array(list(L))
+
+
+def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_check_for_tp=False,
+            use_graph=True, device=0, block=0, **_):
+    prm = RedParams()
+    prm.integrator = integrator
+    prm.adaptive = int(adaptive)
+    prm.tolerance = tolerance
+    prm.error_check_for_tp = int(error_check_for_tp)
+    prm.inner_steps = 1
+    prm.frame_center = 1
+    prm.dt_day = dt_day
+    prm.use_graph = int(use_graph)
+    prm.device = device
+    prm.block = block
+    return prm
+
+
+class Ensemble:
+    """Independent disks integrated together through red_ensemble_* (internal units)."""
+
+    def __init__(self, disks, **kw):
+        self.L = load_library()
+        self.h = ctypes.c_void_p()
+        self.sizes = [d.n for d in disks]
+        keep = []
+        arr = (RedBodies * len(disks))()
+        for k, bodies in enumerate(disks):
+            a = dict(id=np.ascontiguousarray(bodies.ids, dtype=np.int32),
+                     type=np.ascontiguousarray(bodies.types, dtype=np.int32),
+                     mass=np.ascontiguousarray(bodies.mass, dtype=np.float64),
+                     radius=np.ascontiguousarray(bodies.radius, dtype=np.float64),
+                     density=np.ascontiguousarray(bodies.density, dtype=np.float64),
+                     cd=np.ascontiguousarray(bodies.cd, dtype=np.float64),
+                     r=np.ascontiguousarray(bodies.r, dtype=np.float64),
+                     v=np.ascontiguousarray(bodies.v, dtype=np.float64))
+            keep.append(a)
+            for c in range(7):
+                arr[k].counts[c] = int(bodies.counts[c])
+            for f, x in a.items():
+                setattr(arr[k], f, _ptr(x))
+        prm = _params(**kw)
+        rc = self.L.red_ensemble_create(len(disks), ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(prm),
+                                        ctypes.byref(self.h))
+        if rc:
+            msg = self.L.red_ensemble_last_error(self.h).decode() if self.h else ""
+            self.close()
+            raise RedError(rc, msg or "red_ensemble_create failed")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h:
+            self.L.red_ensemble_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc:
+            raise RedError(rc, self.L.red_ensemble_last_error(self.h).decode())
+
+    def set_time(self, t):
+        self._chk(self.L.red_ensemble_set_time(self.h, float(t)))
+
+    def advance(self, t_stop, max_attempts=1 << 30):
+        info = RedEnsembleInfo()
+        self._chk(self.L.red_ensemble_advance(self.h, float(t_stop), int(max_attempts), ctypes.byref(info)))
+        return dict(supersteps=info.supersteps, active_disks=info.active_disks, seconds=info.seconds)
+
+    def state(self, disk):
+        n = self.sizes[disk]
+        r = np.zeros((n, 3)); v = np.zeros((n, 3))
+        self._chk(self.L.red_ensemble_get_state(self.h, disk, _ptr(r), _ptr(v)))
+        return r, v
+
+    def disk_info(self, disk):
+        c = np.zeros(3, dtype=np.int32); t = ctypes.c_double(); dt = ctypes.c_double()
+        self._chk(self.L.red_ensemble_disk_info(self.h, disk, c.ctypes.data_as(_IP), ctypes.byref(t), ctypes.byref(dt)))
+        return dict(tried=int(c[0]), failed=int(c[1]), passed=int(c[2]), t=t.value, dt_try=dt.value)
+
+    def autotune(self, reps=5):
+        b = ctypes.c_int(); j = ctypes.c_int(); ms = (ctypes.c_double * 9)()
+        self._chk(self.L.red_ensemble_autotune(self.h, reps, ctypes.byref(b), ctypes.byref(j), ms))
+        return b.value, j.value, np.array(list(ms)).reshape(3, 3)
