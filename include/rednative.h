@@ -210,6 +210,7 @@ int red_host_transform_velocity(int n, double* v);
 #define RED_PROF_STAGE  3   /* stage combinations and y_(n+1)                        */
 #define RED_PROF_ERR    4   /* error estimate + max-reduction                        */
 #define RED_PROF_PACK   5   /* mixed-precision source packing                        */
+#define RED_PROF_LEARN  6   /* step features and learned step proposal               */
 #define RED_PROF_CLASSES 8
 int red_set_profiling(red_ctx* ctx, int on);
 int red_get_profile(red_ctx* ctx, double ms[RED_PROF_CLASSES], int launches[RED_PROF_CLASSES], int reset);
@@ -247,6 +248,60 @@ int red_ensemble_disk_info(red_ensemble* ens, int disk, int out[3], double* t, d
 /* Picks the force-launch shape (threads per block x source tile, each in {64,128,256}) by
  * timing `reps` force evaluations per shape; ms[9] receives the timings (block-major). */
 int red_ensemble_autotune(red_ensemble* ens, int reps, int* best_block, int* best_jtile, double ms[9]);
+
+/* ---- learned step-size proposer -----------------------------------------------------
+ * Not in the reference, whose only step control is the elementary controller
+ * dt <- 0.9 dt (tol/err)^(1/order) (int_rungekutta8.cu:776-777 and its siblings).
+ *
+ * Step records.  With recording on, every step of an adaptive method writes one record of
+ * RED_HIST_FIELDS floats once its first attempt has finished.  eta denotes the
+ * tolerance-matching step of an attempt, eta = dt (tol/err)^(1/order); step n is the last
+ * accepted step, dt_n its step and err_n its error.
+ *   [0] log10 dt_n              [1] log10(err_n / tol)       [2] ln(eta_n / eta_(n-1))
+ *   [3] ln(eta_(n-1)/eta_(n-2)) [4] rejected attempts of n   [5] ln(1 + pair events of n)
+ *   [6] ln(eta_(n-2)/eta_(n-3)) [7] log10 min d/R_H           [8] change of [7] since step n
+ *   [9] log10 max |a|           [10] log10 max |a - a_kepler|/|a_kepler|  [11] change of [10]
+ *   [12] label ln(eta_(n+1) / eta_n) from the first attempt of step n+1
+ *   [13] network output used for the step (0 without one)   [14] ln(dt_first / eta_n)
+ *   [15] 1 if the first attempt was accepted, else 0
+ * d/R_H: the smallest ratio of separation to mutual Hill radius
+ * ((m_i+m_j)/(3 m_star))^(1/3) (r_i+r_j)/2 (heliocentric r) over the interacting pairs of
+ * massive bodies other than the star; a from the step's first force evaluation.
+ * Features [0..11] are the network input. */
+#define RED_HIST_FIELDS   16
+#define RED_MLP_FEATURES  12
+typedef struct red_mlp red_mlp;
+/* 12 -> 64 -> 64 -> 1 GELU network, BF16 weights, FP32 accumulation, on `device`.
+ * params: red_mlp_param_count() FP32 values W1[64][12] b1[64] W2[64][64] b2[64] W3[64] b3
+ * (NULL = zeros).  The input is standardised as (x - mean) * inv_std (default 0 and 1). */
+int red_mlp_param_count(void);
+int red_mlp_create(int device, const float* params, red_mlp** out);
+int red_mlp_destroy(red_mlp* net);
+const char* red_mlp_last_error(const red_mlp* net);
+int red_mlp_set_normalizer(red_mlp* net, const float* mean, const float* inv_std);
+/* FP32 master parameters, their BF16 image and the normaliser (mean[12] then inv_std[12]);
+ * any pointer may be NULL. */
+int red_mlp_get_params(red_mlp* net, float* params, unsigned short* bf16, float* norm);
+int red_mlp_predict(red_mlp* net, const float* x, int n, float* y);
+/* Mean squared error over (x[n][12], target[n]) and its gradient (BF16 weights, FP32 sums). */
+int red_mlp_gradient(red_mlp* net, const float* x, const float* target, int n, float* grad, float* loss);
+/* Adam (beta 0.9, 0.999, eps 1e-8) over minibatches of `batch` records in the given order,
+ * `epochs` passes; losses[epoch] receives the mean minibatch loss (may be NULL). */
+int red_mlp_train(red_mlp* net, const float* x, const float* target, int n, int epochs, int batch, float lr,
+                  float* losses);
+int red_mlp_adam_state(red_mlp* net, float* m, float* v, long* step);
+
+/* Records steps into a buffer of `capacity` records (0 switches recording off). */
+int red_history_enable(red_ctx* ctx, int capacity);
+/* Copies up to `cap` records, oldest first, and clears the buffer; *n = records copied. */
+int red_history_get(red_ctx* ctx, float* records, int cap, int* n);
+/* Lets `net` propose the first trial step of every step once four accepted steps are known:
+ * dt = safety * eta_n * exp(clamp(y, -max_log_change, max_log_change)).  Rejected attempts
+ * are retried with the reference controller's step.  net = NULL restores the reference
+ * controller.  The network must live on the context's device and outlive its use. */
+int red_set_proposer(red_ctx* ctx, red_mlp* net, double safety, double max_log_change);
+/* out[0] proposals made, out[1] first attempts accepted after a proposal. */
+int red_proposer_stats(red_ctx* ctx, int out[2]);
 
 /* Library version string. */
 const char* red_version(void);

@@ -111,6 +111,22 @@ struct red_ctx {
     // captured graphs, per state-buffer parity
     hipGraphExec_t g_attempt[2] = {nullptr, nullptr};
     hipGraphExec_t g_f0[2] = {nullptr, nullptr};
+    // learned step-size proposer and step records
+    red_mlp* prop_net = nullptr;
+    double prop_safety = 0.9, prop_max_log = 3.0;
+    int prop_count = 0, prop_first_ok = 0;
+    int hist_cap = 0;
+    std::vector<float> hist;              // RED_HIST_FIELDS floats per record
+    StepFeat* d_feat = nullptr;
+    float* d_prop_out = nullptr;
+    float* h_prop_out = nullptr;          // pinned
+    // accepted-step history: eta[0] = latest tolerance-matching step
+    double eta_hist[4] = {0.0, 0.0, 0.0, 0.0};
+    int n_eta = 0;
+    double last_dt = 0.0, last_err_ratio = 0.0;
+    int last_attempts = 0, last_events = 0;
+    float prev_log_min_sep = 0.0f, prev_log_max_pert = 0.0f;
+    bool have_prev_feat = false;
 
     int nSI() const { return cnt[0] + cnt[1] + cnt[2] + cnt[3]; }
     int nNSI() const { return cnt[4] + cnt[5]; }
@@ -131,10 +147,19 @@ struct red_ctx {
         if (_e != hipSuccess) return (ctx)->hipfail(_e, #expr);  \
     } while (0)
 
+namespace red {
+std::recursive_mutex& setup_mutex()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+}  // namespace red
+
 namespace {
 
 void destroy_graphs(red_ctx* c)
 {
+    SetupLock lk;
     for (int k = 0; k < 2; k++) {
         if (c->g_attempt[k]) (void)hipGraphExecDestroy(c->g_attempt[k]);
         if (c->g_f0[k]) (void)hipGraphExecDestroy(c->g_f0[k]);
@@ -275,6 +300,7 @@ hipError_t run_maybe_graph(red_ctx* c, hipGraphExec_t* slot, F fn)
 {
     if (!c->prm.use_graph) return fn(c);
     if (!*slot) {
+        SetupLock lk;
         hipGraph_t g = nullptr;
         RED_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = fn(c);
@@ -314,7 +340,7 @@ int fetch_raw_events(red_ctx* c)
                                                    std::to_string(c->ev_cap) + ")");
     c->raw.resize(n);
     if (n > 0) {
-        CTX_HIP(c, hipMemcpy(c->raw.data(), c->d_events, sizeof(red_event) * n, hipMemcpyDeviceToHost));
+        CTX_HIP(c, stream_copy(c->stream, c->raw.data(), c->d_events, sizeof(red_event) * n, hipMemcpyDeviceToHost));
         std::stable_sort(c->raw.begin(), c->raw.end(), event_key_less);
     }
     c->raw_valid = true;
@@ -369,6 +395,7 @@ int apply_updates(red_ctx* c, const std::vector<BodyUpdate>& ups)
 {
     if (ups.empty()) return RED_OK;
     if ((int)ups.size() > c->updates_cap) {
+        SetupLock lk;
         if (c->d_updates) (void)hipFree(c->d_updates);
         c->updates_cap = (int)ups.size() * 2;
         CTX_HIP(c, dmalloc(&c->d_updates, c->updates_cap));
@@ -410,10 +437,10 @@ int upload_all(red_ctx* c)
     std::vector<d4> r(c->n), v(c->n);
     // keep the caller's values: only the parameters and ids are re-uploaded here
     (void)r; (void)v;
-    CTX_HIP(c, hipMemcpy(c->d_mass, c->h_mass.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
-    CTX_HIP(c, hipMemcpy(c->d_radius, c->h_radius.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
-    CTX_HIP(c, hipMemcpy(c->d_id, c->h_id.data(), sizeof(int) * c->n, hipMemcpyHostToDevice));
-    CTX_HIP(c, hipMemcpy(c->d_type, c->h_type.data(), sizeof(int) * c->n, hipMemcpyHostToDevice));
+    CTX_HIP(c, stream_copy(c->stream, c->d_mass, c->h_mass.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
+    CTX_HIP(c, stream_copy(c->stream, c->d_radius, c->h_radius.data(), sizeof(double) * c->n, hipMemcpyHostToDevice));
+    CTX_HIP(c, stream_copy(c->stream, c->d_id, c->h_id.data(), sizeof(int) * c->n, hipMemcpyHostToDevice));
+    CTX_HIP(c, stream_copy(c->stream, c->d_type, c->h_type.data(), sizeof(int) * c->n, hipMemcpyHostToDevice));
     return RED_OK;
 }
 
@@ -422,8 +449,8 @@ int download_state(red_ctx* c, std::vector<d4>& r, std::vector<d4>& v)
     r.resize(c->n); v.resize(c->n);
     if (c->n == 0) return RED_OK;
     CTX_HIP(c, hipStreamSynchronize(c->stream));
-    CTX_HIP(c, hipMemcpy(r.data(), c->d_r[c->cur], sizeof(d4) * c->n, hipMemcpyDeviceToHost));
-    CTX_HIP(c, hipMemcpy(v.data(), c->d_v[c->cur], sizeof(d4) * c->n, hipMemcpyDeviceToHost));
+    CTX_HIP(c, stream_copy(c->stream, r.data(), c->d_r[c->cur], sizeof(d4) * c->n, hipMemcpyDeviceToHost));
+    CTX_HIP(c, stream_copy(c->stream, v.data(), c->d_v[c->cur], sizeof(d4) * c->n, hipMemcpyDeviceToHost));
     return RED_OK;
 }
 
@@ -442,6 +469,7 @@ const char* red_last_error(const red_ctx* ctx)
 int red_ctx_destroy(red_ctx* c)
 {
     if (!c) return RED_OK;
+    SetupLock lk;
     destroy_graphs(c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int k = 0; k < 2; k++) { (void)hipFree(c->d_r[k]); (void)hipFree(c->d_v[k]); }
@@ -452,8 +480,10 @@ int red_ctx_destroy(red_ctx* c)
     (void)hipFree(c->d_ctl); (void)hipFree(c->d_events); (void)hipFree(c->d_events_hc); (void)hipFree(c->d_scratch);
     (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy);
     (void)hipFree(c->d_partial);
+    (void)hipFree(c->d_feat); (void)hipFree(c->d_prop_out);
     for (void* p : c->user_buffers) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->h_prop_out) (void)hipHostFree(c->h_prop_out);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return RED_OK;
@@ -462,6 +492,7 @@ int red_ctx_destroy(red_ctx* c)
 int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
 {
     if (!b || !p || !out) return RED_ERR_ARG;
+    SetupLock lk;
     *out = nullptr;
     red_ctx* c = new red_ctx();
     c->prm = *p;
@@ -574,8 +605,8 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
             v[i] = d4{b->v[3 * i], b->v[3 * i + 1], b->v[3 * i + 2], 0.0};
         }
         if (n > 0) {
-            CTX_HIP(c, hipMemcpy(c->d_r[0], r.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
-            CTX_HIP(c, hipMemcpy(c->d_v[0], v.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+            CTX_HIP(c, stream_copy(c->stream, c->d_r[0], r.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
+            CTX_HIP(c, stream_copy(c->stream, c->d_v[0], v.data(), sizeof(d4) * n, hipMemcpyHostToDevice));
         }
     }
     int rc = upload_all(c);
@@ -628,9 +659,11 @@ int red_set_state(red_ctx* c, const double* r, const double* v)
     }
     CTX_HIP(c, hipStreamSynchronize(c->stream));
     if (c->n > 0) {
-        CTX_HIP(c, hipMemcpy(c->d_r[c->cur], R.data(), sizeof(d4) * c->n, hipMemcpyHostToDevice));
-        CTX_HIP(c, hipMemcpy(c->d_v[c->cur], V.data(), sizeof(d4) * c->n, hipMemcpyHostToDevice));
+        CTX_HIP(c, stream_copy(c->stream, c->d_r[c->cur], R.data(), sizeof(d4) * c->n, hipMemcpyHostToDevice));
+        CTX_HIP(c, stream_copy(c->stream, c->d_v[c->cur], V.data(), sizeof(d4) * c->n, hipMemcpyHostToDevice));
     }
+    c->n_eta = 0;                 // a new state starts a new step history
+    c->have_prev_feat = false;
     return RED_OK;
 }
 
@@ -659,6 +692,122 @@ int red_accel(red_ctx* c, double t, int stage, const double* r_dev, const double
     return RED_OK;
 }
 
+namespace {
+
+float clampf(double x, double lo, double hi) { return (float)std::min(hi, std::max(lo, x)); }
+
+int learn_buffers(red_ctx* c)
+{
+    if (c->d_feat && c->d_prop_out && c->h_prop_out) return RED_OK;
+    SetupLock lk;
+    if (!c->d_feat) CTX_HIP(c, hipMalloc(&c->d_feat, sizeof(StepFeat)));
+    if (!c->d_prop_out) CTX_HIP(c, hipMalloc(&c->d_prop_out, sizeof(float) * RED_HIST_FIELDS));
+    if (!c->h_prop_out)
+        CTX_HIP(c, hipHostMalloc((void**)&c->h_prop_out, sizeof(float) * RED_HIST_FIELDS, hipHostMallocDefault));
+    return RED_OK;
+}
+
+// Fills rec[0..11] with the features of the step about to start (history on the host, state
+// features of (y_n, f0) on the device) and rec[13] with the network output, if any.
+int learn_features(red_ctx* c, float* rec)
+{
+    int rc = learn_buffers(c);
+    if (rc) return rc;
+    ProposeArgs A;
+    memset(&A, 0, sizeof(A));
+    const bool hist = c->n_eta >= 4;
+    if (hist) {
+        const double* e = c->eta_hist;
+        A.x[0] = clampf(std::log10(c->last_dt), -30.0, 30.0);
+        A.x[1] = clampf(std::log10(c->last_err_ratio), -30.0, 30.0);
+        A.x[2] = clampf(std::log(e[0] / e[1]), -30.0, 30.0);
+        A.x[3] = clampf(std::log(e[1] / e[2]), -30.0, 30.0);
+        A.x[4] = (float)(c->last_attempts - 1);
+        A.x[5] = (float)std::log1p((double)c->last_events);
+        A.x[6] = clampf(std::log(e[2] / e[3]), -30.0, 30.0);
+    }
+    A.prev_log_min_sep = c->prev_log_min_sep;
+    A.prev_log_max_pert = c->prev_log_max_pert;
+    A.have_prev = c->have_prev_feat ? 1 : 0;
+    CTX_HIP(c, timed(c, RED_PROF_LEARN, [&] {
+        RED_HIP_TRY(launch_step_features(c->d_r[c->cur], c->d_a[0], c->d_id, c->nSI(), c->nMassive(), c->n,
+                                         c->d_feat, c->stream));
+        return launch_mlp_propose(hist ? c->prop_net : nullptr, A, c->d_feat, c->d_prop_out, c->stream);
+    }));
+    CTX_HIP(c, hipMemcpyAsync(c->h_prop_out, c->d_prop_out, sizeof(float) * (MLP_NF + 1), hipMemcpyDeviceToHost,
+                              c->stream));
+    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k <= MLP_NF; k++) rec[k < MLP_NF ? k : 13] = c->h_prop_out[k];
+    c->prev_log_min_sep = c->h_prop_out[FEAT_LOG_MIN_SEP];
+    c->prev_log_max_pert = c->h_prop_out[FEAT_LOG_MAX_PERT];
+    c->have_prev_feat = true;
+    return RED_OK;
+}
+
+// Accepted step: shift the history (an unusable error estimate restarts it).
+void learn_accept(red_ctx* c, double dt, double err, int attempts, int events)
+{
+    const double eta = tools::tolerance_step(dt, err, c->prm.tolerance, c->S.exponent);
+    if (!(eta > 0.0) || !std::isfinite(eta) || !(dt > 0.0)) {
+        c->n_eta = 0;
+        return;
+    }
+    for (int k = 3; k > 0; k--) c->eta_hist[k] = c->eta_hist[k - 1];
+    c->eta_hist[0] = eta;
+    c->n_eta = std::min(4, c->n_eta + 1);
+    c->last_dt = dt;
+    c->last_err_ratio = err / c->prm.tolerance;
+    c->last_attempts = attempts;
+    c->last_events = events;
+}
+
+}  // namespace
+
+int red_history_enable(red_ctx* c, int capacity)
+{
+    if (!c || capacity < 0) return RED_ERR_ARG;
+    c->hist_cap = capacity;
+    c->hist.clear();
+    c->hist.reserve((size_t)std::min(capacity, 1 << 20) * RED_HIST_FIELDS);
+    return RED_OK;
+}
+
+int red_history_get(red_ctx* c, float* records, int cap, int* n)
+{
+    if (!c || cap < 0 || (cap > 0 && !records)) return RED_ERR_ARG;
+    const int have = (int)(c->hist.size() / RED_HIST_FIELDS);
+    const int k = std::min(have, cap);
+    if (k > 0) memcpy(records, c->hist.data(), sizeof(float) * (size_t)k * RED_HIST_FIELDS);
+    c->hist.erase(c->hist.begin(), c->hist.begin() + (size_t)k * RED_HIST_FIELDS);
+    if (n) *n = k;
+    return RED_OK;
+}
+
+int red_set_proposer(red_ctx* c, red_mlp* net, double safety, double max_log_change)
+{
+    if (!c) return RED_ERR_ARG;
+    if (net) {
+        if (!(safety > 0.0 && safety <= 2.0) || !(max_log_change > 0.0 && max_log_change <= 20.0))
+            return c->fail(RED_ERR_ARG, "proposer safety must lie in (0, 2] and max_log_change in (0, 20]");
+        if (mlp_device(net) != c->prm.device)
+            return c->fail(RED_ERR_ARG, "the network lives on another device");
+        if (!(c->prm.adaptive && c->S.err_form >= 0))
+            return c->fail(RED_ERR_UNSUPPORTED, "a step-size proposer needs an adaptive method");
+    }
+    c->prop_net = net;
+    c->prop_safety = safety;
+    c->prop_max_log = max_log_change;
+    return RED_OK;
+}
+
+int red_proposer_stats(red_ctx* c, int out[2])
+{
+    if (!c || !out) return RED_ERR_ARG;
+    out[0] = c->prop_count;
+    out[1] = c->prop_first_ok;
+    return RED_OK;
+}
+
 int red_step(red_ctx* c, red_step_info* info)
 {
     if (!c) return RED_ERR_ARG;
@@ -669,8 +818,26 @@ int red_step(red_ctx* c, red_step_info* info)
     CTX_HIP(c, hipGetLastError());
     CTX_HIP(c, run_maybe_graph(c, &c->g_f0[c->cur], enqueue_f0));
     c->raw_valid = false;
+    // learned proposal of the first trial step (the reference controller's step otherwise)
+    float rec[RED_HIST_FIELDS] = {0.0f};
+    bool rec_open = false, proposed = false;
+    double eta_n = 0.0;
+    if (adaptive && (c->prop_net || c->hist_cap > 0)) {
+        int rc = learn_features(c, rec);
+        if (rc) return rc;
+        if (c->n_eta >= 4) {
+            rec_open = true;
+            eta_n = c->eta_hist[0];
+            if (c->prop_net) {
+                const double y = std::min(c->prop_max_log, std::max(-c->prop_max_log, (double)rec[13]));
+                c->dt_try = c->prop_safety * eta_n * std::exp(y);
+                proposed = true;
+                c->prop_count++;
+            }
+        }
+    }
     double max_err = 0.0;
-    int iter = 0;
+    int iter = 0, attempts = 0;
     bool done = false;
     while (!done) {
         if (iter >= c->attempts_cap)
@@ -693,6 +860,15 @@ int red_step(red_ctx* c, red_step_info* info)
         max_err = tools::controller_update(&c->dt_try, S.lambda, bits_to_double(c->h_ctl->err_bits),
                                            c->prm.tolerance, S.exponent);
         c->last_err = max_err;
+        if (attempts++ == 0 && rec_open) {
+            const double eta1 = tools::tolerance_step(c->dt_did, max_err, c->prm.tolerance, S.exponent);
+            rec[12] = clampf(std::log(eta1 / eta_n), -30.0, 30.0);
+            rec[14] = clampf(std::log(c->dt_did / eta_n), -30.0, 30.0);
+            rec[15] = max_err > c->prm.tolerance ? 0.0f : 1.0f;
+            if (c->hist_cap > 0 && c->hist.size() < (size_t)c->hist_cap * RED_HIST_FIELDS)
+                c->hist.insert(c->hist.end(), rec, rec + RED_HIST_FIELDS);
+            if (proposed && !(max_err > c->prm.tolerance)) c->prop_first_ok++;
+        }
         if (S.accept_on_event && c->h_ctl->n_events > 0) {
             if (c->dt_try < c->dt_did) c->dt_try = c->dt_did;
             break;   // leaves before iter++ as int_rungekutta4.cu:255-264 does
@@ -700,6 +876,8 @@ int red_step(red_ctx* c, red_step_info* info)
         iter++;
         done = !(max_err > c->prm.tolerance);
     }
+    if (adaptive && (c->prop_net || c->hist_cap > 0))
+        learn_accept(c, c->dt_did, max_err, attempts, c->h_ctl->n_events);
     // integrator.cu:258-263
     c->n_tried += iter;
     c->n_failed += (iter - 1);
@@ -768,7 +946,7 @@ int red_check_ejection_hit_centrum(red_ctx* c, int* handled)
     const int ne = c->h_ctl->n_events_hc;
     if (ne <= 0) return RED_OK;
     std::vector<red_event> ev(ne);
-    CTX_HIP(c, hipMemcpy(ev.data(), c->d_events_hc, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
+    CTX_HIP(c, stream_copy(c->stream, ev.data(), c->d_events_hc, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
     std::stable_sort(ev.begin(), ev.end(), [](const red_event& a, const red_event& b) { return a.sink < b.sink; });
     for (auto& e : ev) c->h_id[e.idx2] = -c->h_id[e.idx2];   // mirror the kernel's negation
     std::vector<BodyUpdate> ups;
@@ -902,8 +1080,8 @@ int red_check_rebuild(red_ctx* c, int threshold, int* rebuilt)
     for (auto* vec : {&c->h_mass, &c->h_radius, &c->h_density, &c->h_cd, &c->h_epoch, &c->h_mig_stop}) vec->resize(m);
     for (auto* vec : {&c->h_id, &c->h_type, &c->h_mig_type}) vec->resize(m);
     if (m > 0) {
-        CTX_HIP(c, hipMemcpy(c->d_r[c->cur], R2.data(), sizeof(d4) * m, hipMemcpyHostToDevice));
-        CTX_HIP(c, hipMemcpy(c->d_v[c->cur], V2.data(), sizeof(d4) * m, hipMemcpyHostToDevice));
+        CTX_HIP(c, stream_copy(c->stream, c->d_r[c->cur], R2.data(), sizeof(d4) * m, hipMemcpyHostToDevice));
+        CTX_HIP(c, stream_copy(c->stream, c->d_v[c->cur], V2.data(), sizeof(d4) * m, hipMemcpyHostToDevice));
     }
     rc = upload_all(c);
     if (rc) return rc;
@@ -939,7 +1117,7 @@ int red_energy(red_ctx* c, double* E, double L[3])
     std::vector<double> o(6 * (size_t)std::max(nm, 1));
     CTX_HIP(c, launch_energy(c->d_r[c->cur], c->d_v[c->cur], c->d_mass, c->d_id, nm, c->nSI(), c->d_energy, c->stream));
     CTX_HIP(c, hipStreamSynchronize(c->stream));
-    if (nm > 0) CTX_HIP(c, hipMemcpy(o.data(), c->d_energy, sizeof(double) * 6 * nm, hipMemcpyDeviceToHost));
+    if (nm > 0) CTX_HIP(c, stream_copy(c->stream, o.data(), c->d_energy, sizeof(double) * 6 * nm, hipMemcpyDeviceToHost));
     double T = 0.0, U = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
     for (int i = 0; i < nm; i++) {
         T += o[6 * i]; U += o[6 * i + 1];
@@ -983,6 +1161,7 @@ int red_device_alloc(red_ctx* c, unsigned long long bytes, void** dev)
 {
     if (!c || !dev) return RED_ERR_ARG;
     *dev = nullptr;
+    SetupLock lk;
     CTX_HIP(c, hipMalloc(dev, bytes ? bytes : 1));
     c->user_buffers.push_back(*dev);
     return RED_OK;
@@ -995,6 +1174,7 @@ int red_device_free(red_ctx* c, void* dev)
     if (it == c->user_buffers.end()) return c->fail(RED_ERR_ARG, "pointer was not allocated by red_device_alloc");
     c->user_buffers.erase(it);
     CTX_HIP(c, hipStreamSynchronize(c->stream));
+    SetupLock lk;
     CTX_HIP(c, hipFree(dev));
     return RED_OK;
 }
@@ -1003,7 +1183,7 @@ int red_copy_to_device(red_ctx* c, void* dev, const void* host, unsigned long lo
 {
     if (!c || (bytes && (!dev || !host))) return RED_ERR_ARG;
     CTX_HIP(c, hipStreamSynchronize(c->stream));
-    CTX_HIP(c, hipMemcpy(dev, host, bytes, hipMemcpyHostToDevice));
+    CTX_HIP(c, stream_copy(c->stream, dev, host, bytes, hipMemcpyHostToDevice));
     return RED_OK;
 }
 
@@ -1011,7 +1191,7 @@ int red_copy_to_host(red_ctx* c, void* host, const void* dev, unsigned long long
 {
     if (!c || (bytes && (!dev || !host))) return RED_ERR_ARG;
     CTX_HIP(c, hipStreamSynchronize(c->stream));
-    CTX_HIP(c, hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost));
+    CTX_HIP(c, stream_copy(c->stream, host, dev, bytes, hipMemcpyDeviceToHost));
     return RED_OK;
 }
 

@@ -264,6 +264,7 @@ hipError_t ens_attempt(red_ensemble* E)
 
 void destroy_graph(red_ensemble* E)
 {
+    SetupLock lk;
     if (E->graph) (void)hipGraphExecDestroy(E->graph);
     E->graph = nullptr;
 }
@@ -272,6 +273,7 @@ hipError_t run_attempt(red_ensemble* E)
 {
     if (!E->use_graph) return ens_attempt(E);
     if (!E->graph) {
+        SetupLock lk;
         hipGraph_t g = nullptr;
         RED_HIP_TRY(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = ens_attempt(E);
@@ -287,13 +289,14 @@ hipError_t run_attempt(red_ensemble* E)
 
 int upload_tiles(red_ensemble* E)
 {
+    SetupLock lk;
     std::vector<EnsTile> tiles = build_tiles(E, E->block);
     if (E->d_tiles) (void)hipFree(E->d_tiles);
     E->d_tiles = nullptr;
     E->n_tiles = (int)tiles.size();
     ENS_HIP(E, dmalloc(&E->d_tiles, tiles.size()));
     if (!tiles.empty())
-        ENS_HIP(E, hipMemcpy(E->d_tiles, tiles.data(), sizeof(EnsTile) * tiles.size(), hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->d_tiles, tiles.data(), sizeof(EnsTile) * tiles.size(), hipMemcpyHostToDevice));
     destroy_graph(E);
     return RED_OK;
 }
@@ -305,6 +308,7 @@ extern "C" {
 int red_ensemble_destroy(red_ensemble* E)
 {
     if (!E) return RED_OK;
+    SetupLock lk;
     destroy_graph(E);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     for (void* p : {(void*)E->yr, (void*)E->yv, (void*)E->outr, (void*)E->outv, (void*)E->disk_of, (void*)E->err_on,
@@ -325,6 +329,7 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
 {
     if (!out || !disks || !p || n_disks <= 0) return RED_ERR_ARG;
     *out = nullptr;
+    SetupLock lk;
     if (p->integrator < RED_EULER || p->integrator > RED_RKF78 || p->precision != RED_PREC_FP64)
         return RED_ERR_UNSUPPORTED;
     red_ensemble* E = new red_ensemble();
@@ -374,12 +379,12 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     ENS_HIP(E, hipHostMalloc((void**)&E->h_err, sizeof(unsigned long long) * E->nd, hipHostMallocDefault));
     ENS_HIP(E, hipHostMalloc((void**)&E->h_nan, sizeof(int) * E->nd, hipHostMallocDefault));
     if (E->n > 0) {
-        ENS_HIP(E, hipMemcpy(E->yr, r.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
-        ENS_HIP(E, hipMemcpy(E->yv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
-        ENS_HIP(E, hipMemcpy(E->outr, r.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
-        ENS_HIP(E, hipMemcpy(E->outv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
-        ENS_HIP(E, hipMemcpy(E->disk_of, E->disk_of_h.data(), sizeof(int) * E->n, hipMemcpyHostToDevice));
-        ENS_HIP(E, hipMemcpy(E->err_on, eon.data(), E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->yr, r.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->yv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->outr, r.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->outv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->disk_of, E->disk_of_h.data(), sizeof(int) * E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->err_on, eon.data(), E->n, hipMemcpyHostToDevice));
     }
     const double dt0 = p->dt_day * tools::GAUSS;   // integrator.cu:29
     E->t.assign(E->nd, 0.0); E->dt_try.assign(E->nd, dt0); E->dt_did.assign(E->nd, 0.0);
@@ -483,8 +488,8 @@ int red_ensemble_get_state(red_ensemble* E, int disk, double* r, double* v)
     std::vector<d4> R(std::max(nb, 1)), V(std::max(nb, 1));
     ENS_HIP(E, hipStreamSynchronize(E->stream));
     if (nb > 0) {
-        ENS_HIP(E, hipMemcpy(R.data(), E->yr + b0, sizeof(d4) * nb, hipMemcpyDeviceToHost));
-        ENS_HIP(E, hipMemcpy(V.data(), E->yv + b0, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+        ENS_HIP(E, stream_copy(E->stream, R.data(), E->yr + b0, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+        ENS_HIP(E, stream_copy(E->stream, V.data(), E->yv + b0, sizeof(d4) * nb, hipMemcpyDeviceToHost));
     }
     for (int i = 0; i < nb; i++) {
         if (r) { r[3 * i] = R[i].x; r[3 * i + 1] = R[i].y; r[3 * i + 2] = R[i].z; }
@@ -511,7 +516,7 @@ int red_ensemble_autotune(red_ensemble* E, int reps, int* best_block, int* best_
     if (!E || reps <= 0) return RED_ERR_ARG;
     const int choices[3] = {64, 128, 256};
     std::vector<int> flags(E->nd, 3);
-    ENS_HIP(E, hipMemcpy(E->d_flags, flags.data(), sizeof(int) * E->nd, hipMemcpyHostToDevice));
+    ENS_HIP(E, stream_copy(E->stream, E->d_flags, flags.data(), sizeof(int) * E->nd, hipMemcpyHostToDevice));
     hipEvent_t a, b;
     ENS_HIP(E, hipEventCreate(&a));
     ENS_HIP(E, hipEventCreate(&b));

@@ -151,5 +151,11 @@ double controller_update(double* dt, double lambda, double errmax, double tol, d
     return max_err;
 }
 
+double tolerance_step(double dt, double err, double tol, double exponent)
+{
+    if (!(err > 0.0)) return HUGE_VAL;
+    return dt * std::pow(tol / err, exponent);
+}
+
 }  // namespace tools
 }  // namespace red

@@ -27,6 +27,8 @@ int calculate_phase(double mu, const double* oe, double* r, double* v);         
 // Controller update of the adaptive schemes (integrator.cu:255, int_rungekutta8.cu:776-777):
 // returns |dt * lambda * errmax| and multiplies dt by 0.9 (tol / err)^exponent.
 double controller_update(double* dt, double lambda, double errmax, double tol, double exponent);
+// Tolerance-matching step of an attempt: dt (tol / err)^exponent (+inf when err <= 0).
+double tolerance_step(double dt, double err, double tol, double exponent);
 
 }  // namespace tools
 }  // namespace red

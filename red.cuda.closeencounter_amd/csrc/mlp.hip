@@ -1,0 +1,510 @@
+// mlp.hip -- the learned step-size proposer's network: 12 -> 64 -> 64 -> 1 with GELU
+// (erf form), BF16 weights and FP32 accumulation; training and inference kernels (gfx950).
+//
+// Not in the reference.  The network maps a 12-feature summary of the integrator's recent
+// steps and of the current state to the log-change of the tolerance-matching step size
+// (red_step proposes dt from it; the embedded error test still accepts or rejects every
+// attempt and the reference controller computes every retry).
+//
+// Parameters: one FP32 master vector and its BF16 image (round to nearest even),
+//   W1[64][12] b1[64] W2[64][64] b2[64] W3[64] b3          (5057 values)
+// plus the input standardisation z = clip((x - mean) * inv_std, -8, 8) kept with the model,
+// applied in-kernel so training and inference see identical inputs (the clip bounds what an
+// input far outside the training records can do to the output).
+// Forward (per sample, FP32, fused multiply-adds in index order):
+//   p1[u] = b1[u] + sum_f W1[u][f] z[f];     h1 = gelu(p1)
+//   p2[u] = b2[u] + sum_v W2[u][v] h1[v];    h2 = gelu(p2)
+//   y     = b3    + sum_u W3[u] h2[u]
+// Loss: mean squared error over the batch.  Gradients are taken at the BF16 weights
+// (the FP32 master receives them), per 32-sample block, summed in block order: results
+// are deterministic.  Adam updates the master copy and re-rounds the BF16 image.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "red_internal.hpp"
+
+namespace {
+
+using red::MLP_NF;
+using red::MLP_NH;
+using red::MLP_NPARAM;
+constexpr int NF = MLP_NF, NH = MLP_NH, NPARAM = MLP_NPARAM;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NF, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH;
+static_assert(OFF_B3 + 1 == NPARAM, "parameter layout");
+constexpr int SB = 32;         // samples per block
+constexpr int W2S = NH + 2;    // padded W2 row (BF16 elements): rows start in different LDS banks
+constexpr int HS = NH + 1;     // padded activation row (floats)
+constexpr float Z_CLIP = 8.0f; // standardised inputs are clipped to [-8, 8]
+
+__device__ __forceinline__ float bf(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+
+__device__ __forceinline__ unsigned short to_bf16(float f)
+{
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);   // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                                  // nearest, ties to even
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+struct Lds {
+    unsigned short w1[NH * NF];
+    unsigned short w2[NH * W2S];
+    float b1[NH], b2[NH], w3[NH], b3;
+    float z[SB][NF];
+    float p1[SB][HS], h1[SB][HS], p2[SB][HS], h2[SB][HS];
+    float out[SB], dy[SB];
+};
+
+// Loads the BF16 weights and the standardised inputs of samples [s0, s0 + ns).
+__device__ void load_block(Lds& L, const unsigned short* __restrict__ p, const float* __restrict__ norm,
+                           const float* X, int s0, int ns)
+{
+    for (int k = threadIdx.x; k < NH * NF; k += blockDim.x) L.w1[k] = p[OFF_W1 + k];
+    for (int k = threadIdx.x; k < NH * NH; k += blockDim.x) L.w2[(k / NH) * W2S + k % NH] = p[OFF_W2 + k];
+    for (int k = threadIdx.x; k < NH; k += blockDim.x) {
+        L.b1[k] = bf(p[OFF_B1 + k]);
+        L.b2[k] = bf(p[OFF_B2 + k]);
+        L.w3[k] = bf(p[OFF_W3 + k]);
+    }
+    if (threadIdx.x == 0) L.b3 = bf(p[OFF_B3]);
+    for (int k = threadIdx.x; k < ns * NF; k += blockDim.x) {
+        const int s = k / NF, f = k % NF;
+        const float z = (X[(size_t)(s0 + s) * NF + f] - norm[f]) * norm[NF + f];
+        L.z[s][f] = fminf(fmaxf(z, -Z_CLIP), Z_CLIP);
+    }
+    __syncthreads();
+}
+
+__device__ void forward(Lds& L, int ns)
+{
+    for (int k = threadIdx.x; k < ns * NH; k += blockDim.x) {
+        const int s = k / NH, u = k % NH;
+        float acc = L.b1[u];
+        for (int f = 0; f < NF; f++) acc = fmaf(bf(L.w1[u * NF + f]), L.z[s][f], acc);
+        L.p1[s][u] = acc;
+        L.h1[s][u] = gelu(acc);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < ns * NH; k += blockDim.x) {
+        const int s = k / NH, u = k % NH;
+        float acc = L.b2[u];
+        for (int v = 0; v < NH; v++) acc = fmaf(bf(L.w2[u * W2S + v]), L.h1[s][v], acc);
+        L.p2[s][u] = acc;
+        L.h2[s][u] = gelu(acc);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        float acc = L.b3;
+        for (int u = 0; u < NH; u++) acc = fmaf(L.w3[u], L.h2[s][u], acc);
+        L.out[s] = acc;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_mlp_predict(const unsigned short* __restrict__ p, const float* __restrict__ norm,
+                                                     const float* __restrict__ X, float* __restrict__ Y, int n)
+{
+    __shared__ Lds L;
+    const int s0 = blockIdx.x * SB;
+    const int ns = min(SB, n - s0);
+    load_block(L, p, norm, X, s0, ns);
+    forward(L, ns);
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) Y[s0 + s] = L.out[s];
+}
+
+// Per-block gradient sums of mean((y - t)^2) over n samples (the 1/n is applied here) into
+// G[block][NPARAM] and the block's sum of squared errors into Lsum[block].
+__global__ __launch_bounds__(256) void k_mlp_grad(const unsigned short* __restrict__ p, const float* __restrict__ norm,
+                                                  const float* __restrict__ X, const float* __restrict__ T, int n,
+                                                  float* __restrict__ G, float* __restrict__ Lsum)
+{
+    __shared__ Lds L;
+    const int s0 = blockIdx.x * SB;
+    const int ns = min(SB, n - s0);
+    load_block(L, p, norm, X, s0, ns);
+    forward(L, ns);
+    const float two_over_n = 2.0f / (float)n;
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) L.dy[s] = two_over_n * (L.out[s] - T[s0 + s]);
+    __syncthreads();
+    float* g = G + (size_t)blockIdx.x * NPARAM;
+    if (threadIdx.x == 0) {
+        float se = 0.0f, db3 = 0.0f;
+        for (int s = 0; s < ns; s++) {
+            const float e = L.out[s] - T[s0 + s];
+            se = fmaf(e, e, se);
+            db3 += L.dy[s];
+        }
+        Lsum[blockIdx.x] = se;
+        g[OFF_B3] = db3;
+    }
+    for (int u = threadIdx.x; u < NH; u += blockDim.x) {
+        float acc = 0.0f;
+        for (int s = 0; s < ns; s++) acc = fmaf(L.dy[s], L.h2[s][u], acc);
+        g[OFF_W3 + u] = acc;
+    }
+    // layer-2 delta d2 = dy * W3 * gelu'(p2), kept in p2
+    for (int k = threadIdx.x; k < ns * NH; k += blockDim.x) {
+        const int s = k / NH, u = k % NH;
+        L.p2[s][u] = L.dy[s] * L.w3[u] * gelu_grad(L.p2[s][u]);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < NH * NH; k += blockDim.x) {
+        const int u = k / NH, v = k % NH;
+        float acc = 0.0f;
+        for (int s = 0; s < ns; s++) acc = fmaf(L.p2[s][u], L.h1[s][v], acc);
+        g[OFF_W2 + k] = acc;
+    }
+    for (int u = threadIdx.x; u < NH; u += blockDim.x) {
+        float acc = 0.0f;
+        for (int s = 0; s < ns; s++) acc += L.p2[s][u];
+        g[OFF_B2 + u] = acc;
+    }
+    // layer-1 delta d1 = (W2^T d2) * gelu'(p1), kept in p1
+    for (int k = threadIdx.x; k < ns * NH; k += blockDim.x) {
+        const int s = k / NH, v = k % NH;
+        float acc = 0.0f;
+        for (int u = 0; u < NH; u++) acc = fmaf(L.p2[s][u], bf(L.w2[u * W2S + v]), acc);
+        L.p1[s][v] = acc * gelu_grad(L.p1[s][v]);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < NH * NF; k += blockDim.x) {
+        const int v = k / NF, f = k % NF;
+        float acc = 0.0f;
+        for (int s = 0; s < ns; s++) acc = fmaf(L.p1[s][v], L.z[s][f], acc);
+        g[OFF_W1 + k] = acc;
+    }
+    for (int v = threadIdx.x; v < NH; v += blockDim.x) {
+        float acc = 0.0f;
+        for (int s = 0; s < ns; s++) acc += L.p1[s][v];
+        g[OFF_B1 + v] = acc;
+    }
+}
+
+// g[k] = sum over blocks in block order; *loss = (sum of block SSE) / n.
+__global__ void k_mlp_reduce(const float* __restrict__ G, int nb, float* __restrict__ g, const float* __restrict__ Lsum,
+                             float* __restrict__ loss, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < NPARAM) {
+        float acc = 0.0f;
+        for (int b = 0; b < nb; b++) acc += G[(size_t)b * NPARAM + k];
+        g[k] = acc;
+    }
+    if (k == 0) {
+        float se = 0.0f;
+        for (int b = 0; b < nb; b++) se += Lsum[b];
+        *loss = se / (float)n;
+    }
+}
+
+__global__ void k_mlp_adam(float* __restrict__ w, unsigned short* __restrict__ wb, const float* __restrict__ g,
+                           float* __restrict__ m, float* __restrict__ v, float lr, float b1, float b2, float eps,
+                           float c1, float c2)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= NPARAM) return;
+    const float gk = g[k];
+    const float mk = b1 * m[k] + (1.0f - b1) * gk;
+    const float vk = b2 * v[k] + (1.0f - b2) * (gk * gk);
+    m[k] = mk;
+    v[k] = vk;
+    const float wk = w[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    w[k] = wk;
+    wb[k] = to_bf16(wk);
+}
+
+__global__ void k_mlp_to_bf16(const float* __restrict__ w, unsigned short* __restrict__ wb)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < NPARAM) wb[k] = to_bf16(w[k]);
+}
+
+// One proposal inside the step loop: completes the feature vector from the device-side
+// state features, writes it to out[0..11] and, with a network, its output to out[12].
+__global__ __launch_bounds__(64) void k_mlp_propose(const unsigned short* __restrict__ p, const float* __restrict__ norm,
+                                                    red::ProposeArgs A, const red::StepFeat* __restrict__ f,
+                                                    float* __restrict__ out)
+{
+    __shared__ Lds L;
+    __shared__ float x[NF];
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NF; k++) x[k] = A.x[k];
+        const float minq3 = __uint_as_float(f->min_q3_bits);
+        const float maxa = __uint_as_float(f->max_a_bits);
+        const float pert = __uint_as_float(f->max_pert_bits);
+        // log10(d / R_H) = log10(q^3) / 3; every value is clamped to keep the features finite
+        const float lq = fminf(fmaxf(log10f(fmaxf(minq3, 1e-30f)) / 3.0f, -10.0f), 10.0f);
+        const float la = fminf(fmaxf(log10f(fmaxf(maxa, 1e-30f)), -30.0f), 30.0f);
+        const float lp = fminf(fmaxf(log10f(fmaxf(pert, 1e-30f)), -30.0f), 30.0f);
+        x[red::FEAT_LOG_MIN_SEP] = lq;
+        x[red::FEAT_D_LOG_MIN_SEP] = A.have_prev ? lq - A.prev_log_min_sep : 0.0f;
+        x[red::FEAT_LOG_MAX_ACC] = la;
+        x[red::FEAT_LOG_MAX_PERT] = lp;
+        x[red::FEAT_D_LOG_MAX_PERT] = A.have_prev ? lp - A.prev_log_max_pert : 0.0f;
+        for (int k = 0; k < NF; k++) out[k] = x[k];
+        out[NF] = 0.0f;
+    }
+    __syncthreads();
+    if (!p) return;
+    load_block(L, p, norm, x, 0, 1);
+    forward(L, 1);
+    if (threadIdx.x == 0) out[NF] = L.out[0];
+}
+
+}  // namespace
+
+struct red_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    float* w = nullptr;            // FP32 master
+    unsigned short* wb = nullptr;  // BF16 image
+    float* norm = nullptr;         // mean[12], inv_std[12]
+    float *m = nullptr, *v = nullptr, *g = nullptr, *G = nullptr, *Ls = nullptr, *loss = nullptr;
+    float *X = nullptr, *T = nullptr, *P = nullptr;
+    int cap_n = 0, cap_blocks = 0, cap_losses = 0;
+    long step = 0;
+    int hipfail(hipError_t e, const char* where)
+    {
+        err = std::string(where) + ": " + hipGetErrorString(e);
+        return RED_ERR_HIP;
+    }
+};
+
+#define MLP_HIP(M, expr)                                        \
+    do {                                                        \
+        hipError_t _e = (expr);                                 \
+        if (_e != hipSuccess) return (M)->hipfail(_e, #expr);   \
+    } while (0)
+
+namespace red {
+
+hipError_t launch_mlp_propose(const red_mlp* M, const ProposeArgs& A, const StepFeat* f, float* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mlp_propose, dim3(1), dim3(64), 0, s, M ? M->wb : nullptr, M ? M->norm : nullptr, A, f, out);
+    return hipGetLastError();
+}
+
+int mlp_device(const red_mlp* M) { return M->device; }
+
+}  // namespace red
+
+namespace {
+
+int ensure(red_mlp* M, int n, int nblocks, int nlosses)
+{
+    if (n <= M->cap_n && nblocks <= M->cap_blocks && nlosses <= M->cap_losses) return RED_OK;
+    red::SetupLock lk;
+    if (n > M->cap_n) {
+        (void)hipFree(M->X); (void)hipFree(M->T); (void)hipFree(M->P);
+        M->X = M->T = M->P = nullptr;
+        M->cap_n = 0;
+        MLP_HIP(M, hipMalloc(&M->X, sizeof(float) * (size_t)n * NF));
+        MLP_HIP(M, hipMalloc(&M->T, sizeof(float) * (size_t)n));
+        MLP_HIP(M, hipMalloc(&M->P, sizeof(float) * (size_t)n));
+        M->cap_n = n;
+    }
+    if (nblocks > M->cap_blocks) {
+        (void)hipFree(M->G); (void)hipFree(M->Ls);
+        M->G = M->Ls = nullptr;
+        M->cap_blocks = 0;
+        MLP_HIP(M, hipMalloc(&M->G, sizeof(float) * (size_t)nblocks * NPARAM));
+        MLP_HIP(M, hipMalloc(&M->Ls, sizeof(float) * (size_t)nblocks));
+        M->cap_blocks = nblocks;
+    }
+    if (nlosses > M->cap_losses) {
+        (void)hipFree(M->loss);
+        M->loss = nullptr;
+        M->cap_losses = 0;
+        MLP_HIP(M, hipMalloc(&M->loss, sizeof(float) * (size_t)nlosses));
+        M->cap_losses = nlosses;
+    }
+    return RED_OK;
+}
+
+constexpr int PB = 256;
+inline dim3 pgrid() { return dim3((NPARAM + PB - 1) / PB); }
+
+}  // namespace
+
+extern "C" {
+
+int red_mlp_param_count(void) { return NPARAM; }
+
+int red_mlp_destroy(red_mlp* M)
+{
+    if (!M) return RED_OK;
+    red::SetupLock lk;
+    (void)hipSetDevice(M->device);
+    if (M->stream) (void)hipStreamSynchronize(M->stream);
+    for (void* p : {(void*)M->w, (void*)M->wb, (void*)M->norm, (void*)M->m, (void*)M->v, (void*)M->g, (void*)M->G,
+                    (void*)M->Ls, (void*)M->loss, (void*)M->X, (void*)M->T, (void*)M->P})
+        (void)hipFree(p);
+    if (M->stream) (void)hipStreamDestroy(M->stream);
+    delete M;
+    return RED_OK;
+}
+
+const char* red_mlp_last_error(const red_mlp* M) { return M ? M->err.c_str() : "no network"; }
+
+int red_mlp_create(int device, const float* params, red_mlp** out)
+{
+    if (!out) return RED_ERR_ARG;
+    *out = nullptr;
+    red::SetupLock lk;
+    red_mlp* M = new red_mlp();
+    M->device = device;
+    *out = M;
+    MLP_HIP(M, hipSetDevice(device));
+    MLP_HIP(M, hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+    MLP_HIP(M, hipMalloc(&M->w, sizeof(float) * NPARAM));
+    MLP_HIP(M, hipMalloc(&M->wb, sizeof(unsigned short) * NPARAM));
+    MLP_HIP(M, hipMalloc(&M->norm, sizeof(float) * 2 * NF));
+    MLP_HIP(M, hipMalloc(&M->m, sizeof(float) * NPARAM));
+    MLP_HIP(M, hipMalloc(&M->v, sizeof(float) * NPARAM));
+    MLP_HIP(M, hipMalloc(&M->g, sizeof(float) * NPARAM));
+    MLP_HIP(M, hipMemset(M->m, 0, sizeof(float) * NPARAM));
+    MLP_HIP(M, hipMemset(M->v, 0, sizeof(float) * NPARAM));
+    std::vector<float> w(NPARAM, 0.0f), nrm(2 * NF, 0.0f);
+    if (params) memcpy(w.data(), params, sizeof(float) * NPARAM);
+    for (int f = 0; f < NF; f++) nrm[NF + f] = 1.0f;
+    MLP_HIP(M, red::stream_copy(M->stream, M->w, w.data(), sizeof(float) * NPARAM, hipMemcpyHostToDevice));
+    MLP_HIP(M, red::stream_copy(M->stream, M->norm, nrm.data(), sizeof(float) * 2 * NF, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_mlp_to_bf16, pgrid(), dim3(PB), 0, M->stream, M->w, M->wb);
+    MLP_HIP(M, hipGetLastError());
+    MLP_HIP(M, hipStreamSynchronize(M->stream));
+    return RED_OK;
+}
+
+int red_mlp_set_normalizer(red_mlp* M, const float* mean, const float* inv_std)
+{
+    if (!M || !mean || !inv_std) return RED_ERR_ARG;
+    float nrm[2 * NF];
+    for (int f = 0; f < NF; f++) {
+        if (!std::isfinite(mean[f]) || !std::isfinite(inv_std[f])) {
+            M->err = "normaliser values must be finite";
+            return RED_ERR_ARG;
+        }
+        nrm[f] = mean[f];
+        nrm[NF + f] = inv_std[f];
+    }
+    MLP_HIP(M, hipSetDevice(M->device));
+    MLP_HIP(M, hipMemcpyAsync(M->norm, nrm, sizeof(nrm), hipMemcpyHostToDevice, M->stream));
+    MLP_HIP(M, hipStreamSynchronize(M->stream));
+    return RED_OK;
+}
+
+int red_mlp_get_params(red_mlp* M, float* params, unsigned short* bf16, float* norm)
+{
+    if (!M) return RED_ERR_ARG;
+    MLP_HIP(M, hipSetDevice(M->device));
+    MLP_HIP(M, hipStreamSynchronize(M->stream));
+    if (params) MLP_HIP(M, red::stream_copy(M->stream, params, M->w, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+    if (bf16) MLP_HIP(M, red::stream_copy(M->stream, bf16, M->wb, sizeof(unsigned short) * NPARAM, hipMemcpyDeviceToHost));
+    if (norm) MLP_HIP(M, red::stream_copy(M->stream, norm, M->norm, sizeof(float) * 2 * NF, hipMemcpyDeviceToHost));
+    return RED_OK;
+}
+
+int red_mlp_predict(red_mlp* M, const float* X, int n, float* out)
+{
+    if (!M || n < 0 || (n && (!X || !out))) return RED_ERR_ARG;
+    if (n == 0) return RED_OK;
+    MLP_HIP(M, hipSetDevice(M->device));
+    const int nb = (n + SB - 1) / SB;
+    int rc = ensure(M, n, 1, 1);
+    if (rc) return rc;
+    MLP_HIP(M, hipMemcpyAsync(M->X, X, sizeof(float) * (size_t)n * NF, hipMemcpyHostToDevice, M->stream));
+    hipLaunchKernelGGL(k_mlp_predict, dim3(nb), dim3(256), 0, M->stream, M->wb, M->norm, M->X, M->P, n);
+    MLP_HIP(M, hipGetLastError());
+    MLP_HIP(M, hipMemcpyAsync(out, M->P, sizeof(float) * n, hipMemcpyDeviceToHost, M->stream));
+    MLP_HIP(M, hipStreamSynchronize(M->stream));
+    return RED_OK;
+}
+
+int red_mlp_gradient(red_mlp* M, const float* X, const float* T, int n, float* grad, float* loss)
+{
+    if (!M || n <= 0 || !X || !T) return RED_ERR_ARG;
+    MLP_HIP(M, hipSetDevice(M->device));
+    const int nb = (n + SB - 1) / SB;
+    int rc = ensure(M, n, nb, 1);
+    if (rc) return rc;
+    MLP_HIP(M, hipMemcpyAsync(M->X, X, sizeof(float) * (size_t)n * NF, hipMemcpyHostToDevice, M->stream));
+    MLP_HIP(M, hipMemcpyAsync(M->T, T, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, M->stream));
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, M->stream, M->wb, M->norm, M->X, M->T, n, M->G, M->Ls);
+    MLP_HIP(M, hipGetLastError());
+    hipLaunchKernelGGL(k_mlp_reduce, pgrid(), dim3(PB), 0, M->stream, M->G, nb, M->g, M->Ls, M->loss, n);
+    MLP_HIP(M, hipGetLastError());
+    if (grad) MLP_HIP(M, hipMemcpyAsync(grad, M->g, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, M->stream));
+    if (loss) MLP_HIP(M, hipMemcpyAsync(loss, M->loss, sizeof(float), hipMemcpyDeviceToHost, M->stream));
+    MLP_HIP(M, hipStreamSynchronize(M->stream));
+    return RED_OK;
+}
+
+int red_mlp_train(red_mlp* M, const float* X, const float* T, int n, int epochs, int batch, float lr, float* losses)
+{
+    if (!M || n <= 0 || !X || !T || epochs <= 0 || batch <= 0 || !(lr > 0.0f)) return RED_ERR_ARG;
+    MLP_HIP(M, hipSetDevice(M->device));
+    const int nbatch = (n + batch - 1) / batch;
+    const long nl = (long)nbatch * epochs;
+    if (nl > (1L << 26)) {
+        M->err = "too many minibatches";
+        return RED_ERR_ARG;
+    }
+    int rc = ensure(M, n, (std::min(batch, n) + SB - 1) / SB, (int)nl);
+    if (rc) return rc;
+    MLP_HIP(M, hipMemcpyAsync(M->X, X, sizeof(float) * (size_t)n * NF, hipMemcpyHostToDevice, M->stream));
+    MLP_HIP(M, hipMemcpyAsync(M->T, T, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, M->stream));
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    long li = 0;
+    for (int e = 0; e < epochs; e++) {
+        for (int s = 0; s < n; s += batch, li++) {
+            const int nbs = std::min(batch, n - s);
+            const int nb = (nbs + SB - 1) / SB;
+            hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, M->stream, M->wb, M->norm, M->X + (size_t)s * NF,
+                               M->T + s, nbs, M->G, M->Ls);
+            MLP_HIP(M, hipGetLastError());
+            hipLaunchKernelGGL(k_mlp_reduce, pgrid(), dim3(PB), 0, M->stream, M->G, nb, M->g, M->Ls, M->loss + li, nbs);
+            MLP_HIP(M, hipGetLastError());
+            M->step++;
+            const float c1 = 1.0f - (float)std::pow((double)b1, (double)M->step);
+            const float c2 = 1.0f - (float)std::pow((double)b2, (double)M->step);
+            hipLaunchKernelGGL(k_mlp_adam, pgrid(), dim3(PB), 0, M->stream, M->w, M->wb, M->g, M->m, M->v, lr, b1, b2,
+                               eps, c1, c2);
+            MLP_HIP(M, hipGetLastError());
+        }
+    }
+    if (losses) {
+        std::vector<float> all((size_t)nl);
+        MLP_HIP(M, hipMemcpyAsync(all.data(), M->loss, sizeof(float) * (size_t)nl, hipMemcpyDeviceToHost, M->stream));
+        MLP_HIP(M, hipStreamSynchronize(M->stream));
+        for (int e = 0; e < epochs; e++) {
+            double sum = 0.0;
+            for (int b = 0; b < nbatch; b++) sum += all[(size_t)e * nbatch + b];
+            losses[e] = (float)(sum / nbatch);
+        }
+    }
+    MLP_HIP(M, hipStreamSynchronize(M->stream));
+    return RED_OK;
+}
+
+int red_mlp_adam_state(red_mlp* M, float* m, float* v, long* step)
+{
+    if (!M) return RED_ERR_ARG;
+    MLP_HIP(M, hipSetDevice(M->device));
+    MLP_HIP(M, hipStreamSynchronize(M->stream));
+    if (m) MLP_HIP(M, red::stream_copy(M->stream, m, M->m, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+    if (v) MLP_HIP(M, red::stream_copy(M->stream, v, M->v, sizeof(float) * NPARAM, hipMemcpyDeviceToHost));
+    if (step) *step = M->step;
+    return RED_OK;
+}
+
+}  // extern "C"

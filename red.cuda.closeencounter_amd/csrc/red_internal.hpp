@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/rednative.h"
 
@@ -167,6 +168,50 @@ hipError_t launch_apply_updates(const BodyUpdate* u, int nu, d4* r, d4* v, doubl
 hipError_t launch_energy(const d4* r, const d4* v, const double* mass, const int* id, int n_massive, int n_si,
                          double* out6, hipStream_t s);
 
+// ---- learned step-size proposer ----
+constexpr int MLP_NF = RED_MLP_FEATURES, MLP_NH = 64;
+constexpr int MLP_NPARAM = MLP_NH * MLP_NF + MLP_NH + MLP_NH * MLP_NH + MLP_NH + MLP_NH + 1;
+// feature slots filled on the device (the rest come from the step history on the host)
+enum {
+    FEAT_LOG_MIN_SEP = 7,
+    FEAT_D_LOG_MIN_SEP = 8,
+    FEAT_LOG_MAX_ACC = 9,
+    FEAT_LOG_MAX_PERT = 10,
+    FEAT_D_LOG_MAX_PERT = 11
+};
+// State features of one step start, reduced by bit-pattern atomics (non-negative floats
+// order like their bits): min over pairs of (d / R_H)^3, max |a|, max |a - a_kep| / |a_kep|.
+struct StepFeat {
+    unsigned min_q3_bits;
+    unsigned max_a_bits;
+    unsigned max_pert_bits;
+    unsigned pad;
+};
+struct ProposeArgs {
+    float x[MLP_NF];            // history features; device slots are overwritten
+    float prev_log_min_sep;
+    float prev_log_max_pert;
+    int have_prev;
+};
+// Resets f and reduces the state features of (r, a): pairs SI x massive and NSI x SI without
+// the star (index 0), bodies 1..n-1 with id > 0.
+hipError_t launch_step_features(const d4* r, const d4* a, const int* id, int n_si, int n_massive, int n,
+                                StepFeat* f, hipStream_t s);
+hipError_t launch_mlp_propose(const red_mlp* M, const ProposeArgs& A, const StepFeat* f, float* out, hipStream_t s);
+int mlp_device(const red_mlp* M);
+
+}  // namespace red
+
+namespace red {
+
+// Serialises, across host threads, what must not overlap a HIP graph capture running on
+// another thread's stream: device and pinned allocation and release, stream set-up and the
+// capture itself.  Every other call stays on its context's own stream and takes no lock.
+std::recursive_mutex& setup_mutex();
+struct SetupLock {
+    std::lock_guard<std::recursive_mutex> g{setup_mutex()};
+};
+
 }  // namespace red
 
 #define RED_HIP_TRY(expr)                                   \
@@ -174,3 +219,12 @@ hipError_t launch_energy(const d4* r, const d4* v, const double* mass, const int
         hipError_t _e = (expr);                             \
         if (_e != hipSuccess) return _e;                    \
     } while (0)
+
+namespace red {
+// Blocking copy on a context's own stream (no legacy-stream synchronisation).
+inline hipError_t stream_copy(hipStream_t s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+{
+    RED_HIP_TRY(hipMemcpyAsync(dst, src, bytes, kind, s));
+    return hipStreamSynchronize(s);
+}
+}  // namespace red

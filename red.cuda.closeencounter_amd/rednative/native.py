@@ -117,8 +117,24 @@ SIGNATURES = {
     "red_ensemble_get_state": (_I, [_P, _I, _P, _P]),
     "red_ensemble_disk_info": (_I, [_P, _I, _IP, _DP, _DP]),
     "red_ensemble_autotune": (_I, [_P, _I, _IP, _IP, _DP]),
+    "red_mlp_param_count": (_I, []),
+    "red_mlp_create": (_I, [_I, _P, ctypes.POINTER(_P)]),
+    "red_mlp_destroy": (_I, [_P]),
+    "red_mlp_last_error": (ctypes.c_char_p, [_P]),
+    "red_mlp_set_normalizer": (_I, [_P, _P, _P]),
+    "red_mlp_get_params": (_I, [_P, _P, _P, _P]),
+    "red_mlp_predict": (_I, [_P, _P, _I, _P]),
+    "red_mlp_gradient": (_I, [_P, _P, _P, _I, _P, _P]),
+    "red_mlp_train": (_I, [_P, _P, _P, _I, _I, _I, ctypes.c_float, _P]),
+    "red_mlp_adam_state": (_I, [_P, _P, _P, ctypes.POINTER(ctypes.c_long)]),
+    "red_history_enable": (_I, [_P, _I]),
+    "red_history_get": (_I, [_P, _P, _I, _IP]),
+    "red_set_proposer": (_I, [_P, _P, _D, _D]),
+    "red_proposer_stats": (_I, [_P, _IP]),
     "red_version": (ctypes.c_char_p, []),
 }
+HIST_FIELDS = 16
+MLP_FEATURES = 12
 
 
 class RedEnsembleInfo(ctypes.Structure):
@@ -400,7 +416,7 @@ class Disk:
                 "n_close_encounter", "n_event_last_clear")
         return dict(zip(keys, (int(x) for x in c)))
 
-    PROF_CLASSES = ("si", "nsi", "ni", "stage", "err", "pack", "r6", "r7")
+    PROF_CLASSES = ("si", "nsi", "ni", "stage", "err", "pack", "learn", "r7")
 
     def set_profiling(self, on: bool):
         self._chk(self.L.red_set_profiling(self.ctx, int(on)))
@@ -414,6 +430,33 @@ class Disk:
         E = ctypes.c_double(); L = (ctypes.c_double * 3)()
         self._chk(self.L.red_energy(self.ctx, ctypes.byref(E), L))
         return E.value, np.array(list(L))
+
+    # ---- learned step-size proposer -------------------------------------------------
+    def history_enable(self, capacity: int):
+        self._chk(self.L.red_history_enable(self.ctx, int(capacity)))
+
+    def history(self, cap: int = 1 << 20) -> np.ndarray:
+        """Step records (n, 16) float32, oldest first; the buffer is emptied."""
+        out = []
+        while True:
+            buf = np.zeros((min(cap, 65536), HIST_FIELDS), dtype=np.float32)
+            n = ctypes.c_int(0)
+            self._chk(self.L.red_history_get(self.ctx, _ptr(buf), buf.shape[0], ctypes.byref(n)))
+            out.append(buf[: n.value])
+            cap -= n.value
+            if n.value < buf.shape[0] or cap <= 0:
+                break
+        return np.concatenate(out) if out else np.zeros((0, HIST_FIELDS), np.float32)
+
+    def set_proposer(self, net: Optional["Mlp"], safety: float = 0.9, max_log_change: float = 3.0):
+        self._proposer = net          # the network must outlive its use
+        self._chk(self.L.red_set_proposer(self.ctx, net.h if net is not None else None, float(safety),
+                                          float(max_log_change)))
+
+    def proposer_stats(self):
+        o = np.zeros(2, dtype=np.int32)
+        self._chk(self.L.red_proposer_stats(self.ctx, o.ctypes.data_as(_IP)))
+        return dict(proposals=int(o[0]), first_accepted=int(o[1]))
 
 
 def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_check_for_tp=False,
@@ -501,3 +544,84 @@ class Ensemble:
         b = ctypes.c_int(); j = ctypes.c_int(); ms = (ctypes.c_double * 9)()
         self._chk(self.L.red_ensemble_autotune(self.h, reps, ctypes.byref(b), ctypes.byref(j), ms))
         return b.value, j.value, np.array(list(ms)).reshape(3, 3)
+
+
+class Mlp:
+    """The proposer network (red_mlp_*): 12 -> 64 -> 64 -> 1 GELU, BF16 weights, FP32 sums."""
+
+    NF = MLP_FEATURES
+
+    def __init__(self, params: Optional[np.ndarray] = None, device: int = 0, norm: Optional[np.ndarray] = None):
+        self.L = load_library()
+        self.h = ctypes.c_void_p()
+        self.n_param = int(self.L.red_mlp_param_count())
+        p = None
+        if params is not None:
+            p = np.ascontiguousarray(params, dtype=np.float32)
+            if p.shape != (self.n_param,):
+                raise ValueError(f"expected {self.n_param} parameters, got {p.shape}")
+        rc = self.L.red_mlp_create(int(device), _ptr(p), ctypes.byref(self.h))
+        if rc:
+            msg = self.L.red_mlp_last_error(self.h).decode() if self.h else ""
+            self.close()
+            raise RedError(rc, msg or "red_mlp_create failed")
+        if norm is not None:
+            self.set_normalizer(norm[: self.NF], norm[self.NF:])
+
+    def close(self):
+        if getattr(self, "h", None) and self.h:
+            self.L.red_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc:
+            raise RedError(rc, self.L.red_mlp_last_error(self.h).decode())
+
+    def set_normalizer(self, mean, inv_std):
+        m = np.ascontiguousarray(mean, dtype=np.float32)
+        s = np.ascontiguousarray(inv_std, dtype=np.float32)
+        self._chk(self.L.red_mlp_set_normalizer(self.h, _ptr(m), _ptr(s)))
+
+    def params(self):
+        """(FP32 master parameters, BF16 image as uint16, normaliser[24])."""
+        w = np.zeros(self.n_param, np.float32); b = np.zeros(self.n_param, np.uint16)
+        nrm = np.zeros(2 * self.NF, np.float32)
+        self._chk(self.L.red_mlp_get_params(self.h, _ptr(w), _ptr(b), _ptr(nrm)))
+        return w, b, nrm
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, self.NF)
+        y = np.zeros(X.shape[0], np.float32)
+        self._chk(self.L.red_mlp_predict(self.h, _ptr(X), X.shape[0], _ptr(y)))
+        return y
+
+    def gradient(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, self.NF)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.n_param, np.float32); loss = np.zeros(1, np.float32)
+        self._chk(self.L.red_mlp_gradient(self.h, _ptr(X), _ptr(T), X.shape[0], _ptr(g), _ptr(loss)))
+        return float(loss[0]), g
+
+    def train(self, X, T, epochs=1, batch=4096, lr=1e-3):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, self.NF)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        losses = np.zeros(epochs, np.float32)
+        self._chk(self.L.red_mlp_train(self.h, _ptr(X), _ptr(T), X.shape[0], int(epochs), int(batch),
+                                       ctypes.c_float(lr), _ptr(losses)))
+        return losses
+
+    def adam_state(self):
+        m = np.zeros(self.n_param, np.float32); v = np.zeros(self.n_param, np.float32)
+        st = ctypes.c_long(0)
+        self._chk(self.L.red_mlp_adam_state(self.h, _ptr(m), _ptr(v), ctypes.byref(st)))
+        return m, v, st.value
+
+
+def mlp_param_count() -> int:
+    return int(load_library().red_mlp_param_count())

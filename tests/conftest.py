@@ -42,7 +42,9 @@ def assert_bitwise(a, b, what=""):
     a = np.asarray(a); b = np.asarray(b)
     assert a.shape == b.shape, f"{what}: shape {a.shape} != {b.shape}"
     if a.dtype.kind == "f":
-        ok = (a.view(np.uint64) == b.view(np.uint64)) | ((a == 0) & (b == 0))
+        assert a.dtype == b.dtype, f"{what}: dtype {a.dtype} != {b.dtype}"
+        u = np.dtype(f"u{a.dtype.itemsize}")
+        ok = (a.view(u) == b.view(u)) | ((a == 0) & (b == 0))
     else:
         ok = a == b
     if not np.all(ok):
