@@ -103,6 +103,60 @@ def cpu_baseline(bodies_full, target_s=10.0):
                        f"{ntp} of its 65533 test particles: {p} pairs in {dt:.2f} s")
 
 
+def proposer_section(native, years):
+    """Learned step-size proposer on the 16384-body disk (RKF7(8), tol 1e-12, FP64):
+    reference controller against the shipped network, same span of simulated time."""
+    import numpy as np
+    from rednative import generate, proposer
+    b = generate.to_internal(generate.disk_powerlaw(n=16383, seed=202))
+    kw = dict(integrator=4, adaptive=True, tolerance=1e-12, dt_day=0.01, use_graph=True)
+    net = proposer.load(proposer.DEFAULT_MODEL)
+    out = {"workload": "1 star + 16383 protoplanets/planetesimals (m >= 1e-10 M_sun self-interacting), "
+                       "power-law masses, RKF7(8) tol 1e-12, FP64 sequential sums",
+           "sim_years": years}
+    for name, nn, s in (("reference_controller", None, 0.9), ("learned", net, proposer.SAFETY)):
+        st = proposer.run(b, years, net=nn, safety=s, **kw)
+        out[name] = {"attempts_per_sim_yr": st.attempts / st.sim_years, "rejected_fraction": st.rejected_fraction,
+                     "sim_yr_per_hr": st.sim_yr_per_hr, "dE_over_E": st.rel_energy, "dL_over_L": st.rel_angmom,
+                     "hours_for_1000_yr": 1000.0 / st.sim_yr_per_hr}
+    ref, lrn = out["reference_controller"], out["learned"]
+    out["attempts_ratio"] = lrn["attempts_per_sim_yr"] / ref["attempts_per_sim_yr"]
+    out["sim_yr_per_hr_ratio"] = lrn["sim_yr_per_hr"] / ref["sim_yr_per_hr"]
+    return out
+
+
+def ensemble_section(native, days, sample):
+    """512 disks (N log-uniform in [64, 4096], seeded) integrated together against the same
+    disks stepped one context at a time (a sample of them, scaled): disk-steps per second."""
+    from rednative import generate
+    disks = [generate.to_internal(d) for d in generate.ensemble(n_disks=512, seed=404)]
+    kw = dict(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001)
+    t_stop = days * GAUSS
+    E = native.Ensemble(disks, use_graph=True, **kw)
+    block, jtile, _ = E.autotune(reps=3)
+    t0 = time.perf_counter()
+    info = E.advance(t_stop)
+    t_ens = time.perf_counter() - t0
+    steps_ens = sum(E.disk_info(k)["passed"] for k in range(len(disks)))
+    E.close()
+    t1 = time.perf_counter()
+    steps_one = 0
+    for k in range(sample):
+        d = native.Disk(disks[k], use_graph=True, **kw)
+        while d.t <= t_stop:
+            d.step()
+        d.sync()
+        steps_one += d.counters()["passed"]
+        d.close()
+    t_one = time.perf_counter() - t1
+    rate_ens = steps_ens / t_ens
+    rate_one = steps_one / t_one
+    return {"workload": f"512 disks, N log-uniform in [64, 4096] (seed 404), RKF7(8) tol 1e-11, {days} days each",
+            "ensemble_disk_steps_per_s": rate_ens, "per_disk_disk_steps_per_s": rate_one,
+            "per_disk_sample": sample, "speedup": rate_ens / rate_one, "supersteps": info["supersteps"],
+            "autotuned_block": block, "autotuned_jtile": jtile}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +165,10 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kahan", action="store_true")
+    ap.add_argument("--no-proposer", action="store_true")
+    ap.add_argument("--proposer-years", type=float, default=5.0)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-days", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -145,6 +203,8 @@ def main():
     achieved64 = FLOP_PER_PAIR * ni_pairs / ((ni64_ms / max(ni64_n, 1)) * 1e-3) / 1e12
 
     cpu = None if args.no_cpu else cpu_baseline(bodies)
+    prop = None if args.no_proposer else proposer_section(native, args.proposer_years)
+    ens = None if args.no_ensemble else ensemble_section(native, args.ensemble_days, 16)
 
     line = {
         "metric": "pair-interactions/s & sim-yr/hr at N=131072, mixed vs FP64, ΔE/E≤1e-9",
@@ -179,6 +239,8 @@ def main():
                      "flop_per_pair": FLOP_PER_PAIR},
         "profile_ms": {k: v[0] for k, v in mixed["prof"].items() if v[1]},
         "cpu_baseline": cpu,
+        "proposer": prop,
+        "ensemble": ens,
     }
     print(json.dumps(line))
 

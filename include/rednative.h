@@ -96,7 +96,9 @@ typedef struct red_params {
     int    device;                /* HIP device ordinal                             */
     int    block;                 /* gravity launch shape, 0 = autotuned/default    */
     int    sinks_per_thread;      /* 0 = default                                    */
-    int    strategy;              /* 0 = auto, 1 = scalar-broadcast, 2 = LDS tiles  */
+    int    strategy;              /* 0 = auto, 1 = scalar-broadcast, 2 = LDS tiles,
+                                     3 = FP64 producer/consumer (sequential sums;
+                                     the FP64 default)                              */
     int    fp64_split;            /* FP64: let sums split across source ranges for
                                      occupancy (faster, not bit-identical to the
                                      reference's sequential order); 0 = sequential  */
@@ -283,7 +285,12 @@ int red_mlp_set_normalizer(red_mlp* net, const float* mean, const float* inv_std
  * any pointer may be NULL. */
 int red_mlp_get_params(red_mlp* net, float* params, unsigned short* bf16, float* norm);
 int red_mlp_predict(red_mlp* net, const float* x, int n, float* y);
-/* Mean squared error over (x[n][12], target[n]) and its gradient (BF16 weights, FP32 sums). */
+/* Training loss: mean squared error (default) or the pinball loss of level tau in (0, 1),
+ * mean(max(tau (t - y), (tau - 1) (t - y))), whose minimiser is the tau-quantile. */
+#define RED_MLP_LOSS_MSE      0
+#define RED_MLP_LOSS_PINBALL  1
+int red_mlp_set_loss(red_mlp* net, int kind, float tau);
+/* Training loss over (x[n][12], target[n]) and its gradient (BF16 weights, FP32 sums). */
 int red_mlp_gradient(red_mlp* net, const float* x, const float* target, int n, float* grad, float* loss);
 /* Adam (beta 0.9, 0.999, eps 1e-8) over minibatches of `batch` records in the given order,
  * `epochs` passes; losses[epoch] receives the mean minibatch loss (may be NULL). */

@@ -11,7 +11,8 @@ mathematics, not reference code, and is pinned by closed forms and finite differ
 Network: z = clip((x - mean) * inv_std, -8, 8) (FP32, as the kernels), then in float64 on the given
 (already BF16-rounded) weights
     p1 = W1 z + b1, h1 = gelu(p1), p2 = W2 h1 + b2, h2 = gelu(p2), y = W3 . h2 + b3
-with gelu(x) = x Phi(x) = x (1 + erf(x / sqrt 2)) / 2, and loss = mean (y - t)^2.
+with gelu(x) = x Phi(x) = x (1 + erf(x / sqrt 2)) / 2, and loss = mean (y - t)^2 or the
+pinball loss of level tau.
 Parameter vector layout: W1[64][12] b1[64] W2[64][64] b2[64] W3[64] b3.
 """
 from __future__ import annotations
@@ -92,16 +93,22 @@ def forward(params, Z):
     return dict(p1=p1, h1=h1, p2=p2, h2=h2, y=y)
 
 
-def loss_and_grad(params, Z, T):
-    """Mean squared error and its gradient with respect to every parameter (float64)."""
+def loss_and_grad(params, Z, T, tau=None):
+    """Loss and its gradient with respect to every parameter (float64): mean squared error,
+    or with tau the pinball loss mean(max(tau (t - y), (tau - 1) (t - y)))."""
     W1, b1, W2, b2, W3, b3 = unpack(params)
     Z = np.asarray(Z, dtype=np.float64)
     T = np.asarray(T, dtype=np.float64)
     f = forward(params, Z)
     n = Z.shape[0]
-    e = f["y"] - T
-    loss = float(np.mean(e * e))
-    dy = 2.0 * e / n
+    if tau is None:
+        e = f["y"] - T
+        loss = float(np.mean(e * e))
+        dy = 2.0 * e / n
+    else:
+        e = T - f["y"]
+        loss = float(np.mean(np.maximum(tau * e, (tau - 1.0) * e)))
+        dy = np.where(e > 0, -tau, 1.0 - tau) / n
     gW3 = f["h2"].T @ dy
     gb3 = dy.sum()
     d2 = np.outer(dy, W3) * gelu_grad(f["p2"])

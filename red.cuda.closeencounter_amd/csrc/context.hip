@@ -117,9 +117,10 @@ struct red_ctx {
     int prop_count = 0, prop_first_ok = 0;
     int hist_cap = 0;
     std::vector<float> hist;              // RED_HIST_FIELDS floats per record
-    StepFeat* d_feat = nullptr;
-    float* d_prop_out = nullptr;
-    float* h_prop_out = nullptr;          // pinned
+    float* d_learn_part = nullptr;        // per-block feature extrema
+    unsigned* d_learn_counter = nullptr;
+    float* h_prop_out = nullptr;          // pinned, written by the step-start kernel
+    float* h_prop_out_dev = nullptr;      // its device-side address
     // accepted-step history: eta[0] = latest tolerance-matching step
     double eta_hist[4] = {0.0, 0.0, 0.0, 0.0};
     int n_eta = 0;
@@ -480,7 +481,7 @@ int red_ctx_destroy(red_ctx* c)
     (void)hipFree(c->d_ctl); (void)hipFree(c->d_events); (void)hipFree(c->d_events_hc); (void)hipFree(c->d_scratch);
     (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy);
     (void)hipFree(c->d_partial);
-    (void)hipFree(c->d_feat); (void)hipFree(c->d_prop_out);
+    (void)hipFree(c->d_learn_part); (void)hipFree(c->d_learn_counter);
     for (void* p : c->user_buffers) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->h_prop_out) (void)hipHostFree(c->h_prop_out);
@@ -535,8 +536,10 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     c->gcfg.precision = p->precision;
     c->gcfg.n_fp64_src = c->cnt[0] + c->cnt[1];
     c->gcfg.block = p->block > 0 ? std::min(256, std::max(64, p->block / 64 * 64)) : 256;
+    c->gcfg.block_seq = p->block >= 256 ? (p->block >= 1024 ? 1024 : (p->block >= 512 ? 512 : 256)) : 0;
     c->gcfg.spt = p->sinks_per_thread > 0 ? p->sinks_per_thread : (p->precision == RED_PREC_MIXED ? 2 : 1);
-    c->gcfg.strategy = p->strategy > 0 ? p->strategy : 1;
+    // FP64 with the reference's sequential sums: producer/consumer sweeps by default
+    c->gcfg.strategy = p->strategy > 0 ? p->strategy : (p->precision == RED_PREC_FP64 && !p->fp64_split ? 3 : 1);
     c->gcfg.kahan_pairs = p->kahan_pairs;
     c->gcfg.split_max_sinks = 2048;
     c->gcfg.fp64_split = p->fp64_split;
@@ -551,14 +554,14 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     for (int k = 0; k < 2; k++) {
         CTX_HIP(c, dmalloc(&c->d_r[k], cap));
         CTX_HIP(c, dmalloc(&c->d_v[k], cap));
-        CTX_HIP(c, hipMemset(c->d_r[k], 0, sizeof(d4) * cap));
-        CTX_HIP(c, hipMemset(c->d_v[k], 0, sizeof(d4) * cap));
+        CTX_HIP(c, hipMemsetAsync(c->d_r[k], 0, sizeof(d4) * cap, c->stream));
+        CTX_HIP(c, hipMemsetAsync(c->d_v[k], 0, sizeof(d4) * cap, c->stream));
     }
     const int ns = c->S.n_stage;
     c->d_sr.assign(ns, nullptr); c->d_sv.assign(ns, nullptr); c->d_a.assign(ns, nullptr);
     for (int k = 0; k < ns; k++) {
         CTX_HIP(c, dmalloc(&c->d_a[k], cap));
-        CTX_HIP(c, hipMemset(c->d_a[k], 0, sizeof(d4) * cap));
+        CTX_HIP(c, hipMemsetAsync(c->d_a[k], 0, sizeof(d4) * cap, c->stream));
         if (k >= 1) {
             CTX_HIP(c, dmalloc(&c->d_sr[k], cap));
             CTX_HIP(c, dmalloc(&c->d_sv[k], cap));
@@ -569,7 +572,7 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     CTX_HIP(c, dmalloc(&c->d_id, cap));
     CTX_HIP(c, dmalloc(&c->d_type, cap));
     CTX_HIP(c, dmalloc(&c->d_ctl, 1));
-    CTX_HIP(c, hipMemset(c->d_ctl, 0, sizeof(Ctl)));
+    CTX_HIP(c, hipMemsetAsync(c->d_ctl, 0, sizeof(Ctl), c->stream));
     CTX_HIP(c, hipHostMalloc((void**)&c->h_ctl, sizeof(Ctl), hipHostMallocDefault));
     memset(c->h_ctl, 0, sizeof(Ctl));
     c->ev_cap = p->event_capacity > 0 ? p->event_capacity : std::max(4096, n);
@@ -698,25 +701,32 @@ float clampf(double x, double lo, double hi) { return (float)std::min(hi, std::m
 
 int learn_buffers(red_ctx* c)
 {
-    if (c->d_feat && c->d_prop_out && c->h_prop_out) return RED_OK;
+    if (c->d_learn_part && c->d_learn_counter && c->h_prop_out) return RED_OK;
     SetupLock lk;
-    if (!c->d_feat) CTX_HIP(c, hipMalloc(&c->d_feat, sizeof(StepFeat)));
-    if (!c->d_prop_out) CTX_HIP(c, hipMalloc(&c->d_prop_out, sizeof(float) * RED_HIST_FIELDS));
-    if (!c->h_prop_out)
+    if (!c->d_learn_part) CTX_HIP(c, hipMalloc(&c->d_learn_part, sizeof(float) * 3 * LEARN_MAX_BLOCKS));
+    if (!c->d_learn_counter) {
+        CTX_HIP(c, hipMalloc(&c->d_learn_counter, sizeof(unsigned)));
+        CTX_HIP(c, hipMemsetAsync(c->d_learn_counter, 0, sizeof(unsigned), c->stream));
+        CTX_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    if (!c->h_prop_out) {
         CTX_HIP(c, hipHostMalloc((void**)&c->h_prop_out, sizeof(float) * RED_HIST_FIELDS, hipHostMallocDefault));
+        CTX_HIP(c, hipHostGetDevicePointer((void**)&c->h_prop_out_dev, c->h_prop_out, 0));
+    }
     return RED_OK;
 }
 
-// Fills rec[0..11] with the features of the step about to start (history on the host, state
-// features of (y_n, f0) on the device) and rec[13] with the network output, if any.
-int learn_features(red_ctx* c, float* rec)
+// Launches the step-start kernel: the state features of (y_n, f0), the network (if
+// proposing) and the first trial step, written into the control block after the attempt's
+// own fields.  The feature vector and the network output reach the pinned buffer with the
+// attempt's read-back; nothing waits for them on the host.
+int learn_launch(red_ctx* c, bool propose, double eta_n)
 {
     int rc = learn_buffers(c);
     if (rc) return rc;
     ProposeArgs A;
     memset(&A, 0, sizeof(A));
-    const bool hist = c->n_eta >= 4;
-    if (hist) {
+    if (c->n_eta >= 4) {
         const double* e = c->eta_hist;
         A.x[0] = clampf(std::log10(c->last_dt), -30.0, 30.0);
         A.x[1] = clampf(std::log10(c->last_err_ratio), -30.0, 30.0);
@@ -729,19 +739,22 @@ int learn_features(red_ctx* c, float* rec)
     A.prev_log_min_sep = c->prev_log_min_sep;
     A.prev_log_max_pert = c->prev_log_max_pert;
     A.have_prev = c->have_prev_feat ? 1 : 0;
+    LearnBuffers B{c->d_learn_part, c->d_learn_counter, c->h_prop_out_dev};
     CTX_HIP(c, timed(c, RED_PROF_LEARN, [&] {
-        RED_HIP_TRY(launch_step_features(c->d_r[c->cur], c->d_a[0], c->d_id, c->nSI(), c->nMassive(), c->n,
-                                         c->d_feat, c->stream));
-        return launch_mlp_propose(hist ? c->prop_net : nullptr, A, c->d_feat, c->d_prop_out, c->stream);
+        return launch_learn_step(propose ? c->prop_net : nullptr, A, c->d_r[c->cur], c->d_a[0], c->d_id, c->nSI(),
+                                 c->nMassive(), c->n, B, c->d_ctl, propose ? 1 : 0, eta_n, c->prop_safety,
+                                 c->prop_max_log, c->stream);
     }));
-    CTX_HIP(c, hipMemcpyAsync(c->h_prop_out, c->d_prop_out, sizeof(float) * (MLP_NF + 1), hipMemcpyDeviceToHost,
-                              c->stream));
-    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    return RED_OK;
+}
+
+// After a synchronisation following learn_launch: the features and the network output.
+void learn_collect(red_ctx* c, float* rec)
+{
     for (int k = 0; k <= MLP_NF; k++) rec[k < MLP_NF ? k : 13] = c->h_prop_out[k];
     c->prev_log_min_sep = c->h_prop_out[FEAT_LOG_MIN_SEP];
     c->prev_log_max_pert = c->h_prop_out[FEAT_LOG_MAX_PERT];
     c->have_prev_feat = true;
-    return RED_OK;
 }
 
 // Accepted step: shift the history (an unusable error estimate restarts it).
@@ -818,24 +831,14 @@ int red_step(red_ctx* c, red_step_info* info)
     CTX_HIP(c, hipGetLastError());
     CTX_HIP(c, run_maybe_graph(c, &c->g_f0[c->cur], enqueue_f0));
     c->raw_valid = false;
-    // learned proposal of the first trial step (the reference controller's step otherwise)
+    // learned proposal of the first trial step (the reference controller's step otherwise):
+    // the step-start kernel runs on the stream ahead of the first attempt, which takes its
+    // step from the device; nothing waits for it on the host
     float rec[RED_HIST_FIELDS] = {0.0f};
-    bool rec_open = false, proposed = false;
-    double eta_n = 0.0;
-    if (adaptive && (c->prop_net || c->hist_cap > 0)) {
-        int rc = learn_features(c, rec);
-        if (rc) return rc;
-        if (c->n_eta >= 4) {
-            rec_open = true;
-            eta_n = c->eta_hist[0];
-            if (c->prop_net) {
-                const double y = std::min(c->prop_max_log, std::max(-c->prop_max_log, (double)rec[13]));
-                c->dt_try = c->prop_safety * eta_n * std::exp(y);
-                proposed = true;
-                c->prop_count++;
-            }
-        }
-    }
+    const bool learning = adaptive && (c->prop_net || c->hist_cap > 0);
+    const bool rec_open = learning && c->n_eta >= 4;
+    const bool proposed = rec_open && c->prop_net != nullptr;
+    const double eta_n = rec_open ? c->eta_hist[0] : 0.0;
     double max_err = 0.0;
     int iter = 0, attempts = 0;
     bool done = false;
@@ -846,6 +849,10 @@ int red_step(red_ctx* c, red_step_info* info)
         hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, c->stream, c->d_ctl, c->t, c->dt_try, iter, 1,
                            S.clear_events_each_attempt ? 1 : 0, 0);
         CTX_HIP(c, hipGetLastError());
+        if (iter == 0 && learning) {
+            int rc = learn_launch(c, proposed, eta_n);
+            if (rc) return rc;
+        }
         CTX_HIP(c, run_maybe_graph(c, &c->g_attempt[c->cur], enqueue_attempt));
         if (!adaptive) {
             iter++;
@@ -853,6 +860,11 @@ int red_step(red_ctx* c, red_step_info* info)
         }
         int rc = read_ctl(c);
         if (rc) return rc;
+        if (iter == 0 && learning) learn_collect(c, rec);
+        if (iter == 0 && proposed) {
+            c->dt_did = c->dt_try = c->h_ctl->dt;   // the step the device proposed and tried
+            c->prop_count++;
+        }
         if (c->h_ctl->nan_flag) return c->fail(RED_ERR_NAN, "non-finite error estimate");
         if (c->h_ctl->n_events > c->ev_cap)
             return c->fail(RED_ERR_EVENT_OVERFLOW, "pair events exceed the event capacity");

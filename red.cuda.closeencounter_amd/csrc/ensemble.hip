@@ -365,7 +365,7 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     E->sr.assign(E->S.n_stage, nullptr); E->sv.assign(E->S.n_stage, nullptr); E->acc.assign(E->S.n_stage, nullptr);
     for (int k = 0; k < E->S.n_stage; k++) {
         ENS_HIP(E, dmalloc(&E->acc[k], n));
-        ENS_HIP(E, hipMemset(E->acc[k], 0, sizeof(d4) * n));
+        ENS_HIP(E, hipMemsetAsync(E->acc[k], 0, sizeof(d4) * n, E->stream));
         if (k >= 1) { ENS_HIP(E, dmalloc(&E->sr[k], n)); ENS_HIP(E, dmalloc(&E->sv[k], n)); }
     }
     ENS_HIP(E, dmalloc(&E->disk_of, n));

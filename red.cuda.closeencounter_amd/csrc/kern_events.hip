@@ -11,6 +11,8 @@
 //
 // The energy kernel is not in the reference: per massive body it writes
 // (1/2 m v^2, sum_{j>i, interacting} m_i m_j / r_ij, m r x v) for a fixed-order host sum.
+#include <algorithm>
+
 #include "red_internal.hpp"
 
 namespace red {
@@ -96,124 +98,7 @@ __global__ __launch_bounds__(256) void k_energy(const d4* __restrict__ r, const 
     o[5] = 0.0;
 }
 
-// ---- state features of a step start (learned step-size proposer; not in the reference) ----
-
-constexpr int FEAT_SRC = 256;   // sources per wave in the pair pass (4 per lane)
-
-__global__ void k_feat_reset(StepFeat* f)
-{
-    f->min_q3_bits = 0x7f800000u;   // +inf
-    f->max_a_bits = 0u;
-    f->max_pert_bits = 0u;
-    f->pad = 0u;
-}
-
-__device__ __forceinline__ float wave_min(float v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ float wave_max(float v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// One wave per work item, four waves per block.  Items [0, n0): (SI sink 1..n_si-1,
-// 256-source chunk of 1..n_massive-1); [n0, n0+n1): (NSI sink, chunk of SI sources
-// 1..n_si-1); then one item per 64 bodies 1..n-1.  Lanes run over sources, so a sink with
-// many sources is spread over waves; min and max are exact in any order.  The pair
-// quantity is q^3 = d^3 3 m_star / ((m_i + m_j) ((h_i + h_j)/2)^3) with heliocentric h:
-// differences in FP64, the ratio in FP32.
-__global__ __launch_bounds__(256) void k_step_features(const d4* __restrict__ r, const d4* __restrict__ a,
-                                                       const int* __restrict__ id, int n_si, int n_massive, int n,
-                                                       long n0, int ch0, long n1, int ch1, long n_items,
-                                                       StepFeat* __restrict__ f)
-{
-    const int lane = threadIdx.x & 63;
-    long w = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (w >= n_items) return;
-    const d4 star = r[0];
-    const double ms = star.w;
-    if (w < n0 + n1) {
-        int i, src0, src1;
-        long c;
-        if (w < n0) {
-            i = 1 + (int)(w / ch0); c = w % ch0; src0 = 1; src1 = n_massive;
-        } else {
-            w -= n0;
-            i = n_si + (int)(w / ch1); c = w % ch1; src0 = 1; src1 = n_si;
-        }
-        const int j0 = src0 + (int)c * FEAT_SRC;
-        const int j1 = min(src1, j0 + FEAT_SRC);
-        float best = __uint_as_float(0x7f800000u);
-        if (id[i] > 0) {
-            const d4 ri = r[i];
-            const double hx = ri.x - star.x, hy = ri.y - star.y, hz = ri.z - star.z;
-            const float hi = sqrtf((float)(hx * hx + hy * hy + hz * hz));
-            for (int j = j0 + lane; j < j1; j += 64) {
-                if (j == i || id[j] <= 0) continue;
-                const d4 rj = r[j];
-                const double mm = ri.w + rj.w;
-                if (!(mm > 0.0)) continue;
-                const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-                const double gx = rj.x - star.x, gy = rj.y - star.y, gz = rj.z - star.z;
-                const float d2 = (float)(dx * dx + dy * dy + dz * dz);
-                const float hj = sqrtf((float)(gx * gx + gy * gy + gz * gz));
-                const float hm = 0.5f * (hi + hj);
-                const float q3 = d2 * sqrtf(d2) * (float)(3.0 * ms) / ((float)mm * hm * hm * hm);
-                best = fminf(best, q3);
-            }
-        }
-        best = wave_min(best);
-        if (lane == 0) atomicMin(&f->min_q3_bits, __float_as_uint(best));
-        return;
-    }
-    w -= n0 + n1;
-    const int i = 1 + (int)w * 64 + lane;
-    float amax = 0.0f, pmax = 0.0f;
-    if (i < n && id[i] > 0) {
-        const d4 ri = r[i];
-        const d4 ai = a[i];
-        const double hx = ri.x - star.x, hy = ri.y - star.y, hz = ri.z - star.z;
-        const double h2 = hx * hx + hy * hy + hz * hz;
-        const double s = -ms / (h2 * sqrt(h2));
-        const double kx = s * hx, ky = s * hy, kz = s * hz;
-        const double ex = ai.x - kx, ey = ai.y - ky, ez = ai.z - kz;
-        const double an = sqrt(ai.x * ai.x + ai.y * ai.y + ai.z * ai.z);
-        const double kn = sqrt(kx * kx + ky * ky + kz * kz);
-        const double en = sqrt(ex * ex + ey * ey + ez * ez);
-        amax = (float)an;
-        pmax = kn > 0.0 ? (float)(en / kn) : 0.0f;
-    }
-    amax = wave_max(amax);
-    pmax = wave_max(pmax);
-    if (lane == 0) {
-        atomicMax(&f->max_a_bits, __float_as_uint(amax));
-        atomicMax(&f->max_pert_bits, __float_as_uint(pmax));
-    }
-}
-
 }  // namespace
-
-hipError_t launch_step_features(const d4* r, const d4* a, const int* id, int n_si, int n_massive, int n,
-                                StepFeat* f, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_feat_reset, dim3(1), dim3(1), 0, s, f);
-    RED_HIP_TRY(hipGetLastError());
-    const long c0 = n_si > 1 ? n_si - 1 : 0;                 // SI sinks without the star
-    const long c1 = n_si > 1 ? n_massive - n_si : 0;         // NSI sinks (need SI sources)
-    const int ch0 = n_massive > 1 ? (n_massive - 1 + FEAT_SRC - 1) / FEAT_SRC : 0;
-    const int ch1 = n_si > 1 ? (n_si - 1 + FEAT_SRC - 1) / FEAT_SRC : 0;
-    const long n0 = c0 * ch0, n1 = c1 * ch1;
-    const long nb2 = n > 1 ? (n - 1 + 63) / 64 : 0;
-    const long items = n0 + n1 + nb2;
-    if (items <= 0) return hipSuccess;
-    const long blocks = (items + 3) / 4;
-    hipLaunchKernelGGL(k_step_features, dim3((unsigned)blocks), dim3(256), 0, s, r, a, id, n_si, n_massive, n, n0,
-                       ch0, n1, ch1, items, f);
-    return hipGetLastError();
-}
 
 hipError_t launch_ejection_scan(const d4* r, const d4* v, int* id, const int* type, int n, double ej, double hc,
                                 Ctl* ctl, red_event* buf, int cap, hipStream_t s)

@@ -192,8 +192,34 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
         ax[s] = ay[s] = az[s] = 0.0;
     }
     if (!LDS) {
-#pragma unroll 2
-        for (int j = jlo; j < jhi; ++j) {
+        // Sources in groups of G: the next group's records are requested before the current
+        // group's G independent pair terms are computed, so load latency and the long FP64
+        // sqrt/divide sequences overlap; the sums still take j in ascending order.
+        constexpr int G = 4;
+        int j = jlo;
+        d4 qn[G];
+        bool an[G];
+        if (j + G <= jhi) {
+#pragma unroll
+            for (int k = 0; k < G; ++k) { qn[k] = r[j + k]; an[k] = id[j + k] >= 0; }
+        }
+        for (; j + G <= jhi; j += G) {
+            d4 q[G];
+            bool act[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) { q[k] = qn[k]; act[k] = an[k]; }
+            if (j + 2 * G <= jhi) {
+#pragma unroll
+                for (int k = 0; k < G; ++k) { qn[k] = r[j + G + k]; an[k] = id[j + G + k] >= 0; }
+            }
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+#pragma unroll
+                for (int s = 0; s < SPT; s++)
+                    fp64_pair<OVERLAP, EV>(q[k], j + k, act[k], ii[s], xi[s], yi[s], zi[s], Ri[s], live[s], ax[s],
+                                           ay[s], az[s], P, E, ctl, r, v, mass, radius, id);
+        }
+        for (; j < jhi; ++j) {
             const bool act = id[j] >= 0;
             const d4 q = r[j];
 #pragma unroll
@@ -224,6 +250,165 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
 #pragma unroll
     for (int s = 0; s < SPT; s++)
         if (ii[s] < sw.sink1) store_acc(a, partial, parts, p, sw, ii[s], live[s], ax[s], ay[s], az[s]);
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 sweep with sequential sums, producer/consumer through LDS.  The reference adds the
+// source terms of a sink one after another in ascending j (pp_disk.cu:144-215); only those
+// additions are a serial chain -- the pair terms themselves (differences, IEEE sqrt and
+// divide) are independent.  A block of 1 + NP waves takes SQ_SG sinks: NP producer waves
+// compute the terms of the next SQ_J sources into one half of a double-buffered LDS tile
+// while the consumer wave adds the previous chunk in ascending j, one lane per (sink,
+// component) row, starting from 0.  Every term is formed and added exactly as in the fused
+// loop, so the sums are bit-identical to the reference order, while the long FP64
+// sequences run on all SIMDs.  A producer lane keeps one sink in registers and takes
+// SQ_J / SLOTS sources of every chunk (independent work for the scheduler to interleave);
+// the source records of the chunk after next are requested before a chunk is computed.
+constexpr int SQ_SG = 16;             // sinks per block
+constexpr int SQ_J = 64;              // sources per chunk
+constexpr int SQ_ROWS = 3 * SQ_SG;    // consumer lanes in use
+
+template <bool OVERLAP, bool EV, int NP>
+__global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                                 d4* __restrict__ a, const double* __restrict__ mass,
+                                                                 const double* __restrict__ radius,
+                                                                 const int* __restrict__ id, Sweep sw, PairTest P,
+                                                                 EventSink E, Ctl* __restrict__ ctl)
+{
+    constexpr int SLOTS = NP * 64 / SQ_SG;            // source slots per pass over the chunk
+    constexpr int M = (SQ_J + SLOTS - 1) / SLOTS;     // sources per producer lane and chunk
+    __shared__ double buf[2][SQ_J][SQ_ROWS];
+    const int wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int i0 = sw.sink0 + blockIdx.x * SQ_SG;
+    const int ns = min(SQ_SG, sw.sink1 - i0);
+    const int nsrc = sw.src1 - sw.src0;
+    const int nch = (nsrc + SQ_J - 1) / SQ_J;
+    double sum = 0.0;
+
+    if (wv > 0) {
+        // producer lane: sink sk, source slots slot + m * SLOTS of every chunk
+        const int t = (wv - 1) * 64 + lane;
+        const int sk = t % SQ_SG;
+        const int slot = t / SQ_SG;
+        const bool sink_in = sk < ns;
+        const int i = i0 + (sink_in ? sk : 0);
+        const d4 p = r[i];
+        const bool live = sink_in && id[i] > 0;
+        const double Ri = EV ? radius[i] : 0.0;
+        d4 qa[M];
+        int ida[M];            // raw ids: comparing here would wait for the loads
+        auto fetch = [&](int c) {
+            const int j0 = sw.src0 + c * SQ_J;
+            const int nj = min(SQ_J, sw.src1 - j0);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                // unconditional (clamped) loads: a masked load would make the compiler wait
+                // for the data to merge it; slots past the chunk are never used
+                const int jj = slot + m * SLOTS;
+                const int j = j0 + (jj < nj ? jj : 0);
+                qa[m] = r[j];
+                ida[m] = id[j];
+            }
+        };
+        auto compute = [&](int c, int bsel, const d4* q, const int* qid) {
+            const int j0 = sw.src0 + c * SQ_J;
+            const int nj = min(SQ_J, sw.src1 - j0);
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const int jj = slot + m * SLOTS;
+                if (jj < nj) {
+                    const int j = j0 + jj;
+                    const double dx = q[m].x - p.x;
+                    const double dy = q[m].y - p.y;
+                    const double dz = q[m].z - p.z;
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    const double d = sqrt(r2);
+                    const bool act = qid[m] >= 0;
+                    const bool use = sink_in && (OVERLAP ? (act && j != i) : act);
+                    const double sc = use ? q[m].w / (d * r2) : 0.0;
+                    buf[bsel][jj][3 * sk + 0] = sc * dx;
+                    buf[bsel][jj][3 * sk + 1] = sc * dy;
+                    buf[bsel][jj][3 * sk + 2] = sc * dz;
+                    if (EV && use && live) pair_test(P, E, ctl, i, j, d, Ri, radius[j], r, v, mass, id);
+                }
+            }
+        };
+        d4 q[M];
+        int qid[M];
+        if (nch > 0) {
+            fetch(0);
+#pragma unroll
+            for (int m = 0; m < M; ++m) { q[m] = qa[m]; qid[m] = ida[m]; }
+            if (nch > 1) fetch(1);
+            compute(0, 0, q, qid);
+        }
+        __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            if (c + 1 < nch) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) { q[m] = qa[m]; qid[m] = ida[m]; }
+                if (c + 2 < nch) fetch(c + 2);
+                compute(c + 1, (c + 1) & 1, q, qid);
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    // consumer wave
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        const int b = c & 1;
+        if (lane < 3 * ns) {
+            const int nj = min(SQ_J, nsrc - c * SQ_J);
+            if (nj == SQ_J) {
+                // full chunk: the reads of the next 16 terms are in flight while the current
+                // 16 are added
+                constexpr int GQ = 16;
+                double x[GQ], y[GQ];
+#pragma unroll
+                for (int k = 0; k < GQ; ++k) x[k] = buf[b][k][lane];
+#pragma unroll
+                for (int g = 0; g < SQ_J / GQ; ++g) {
+                    if (g + 1 < SQ_J / GQ) {
+#pragma unroll
+                        for (int k = 0; k < GQ; ++k) y[k] = buf[b][(g + 1) * GQ + k][lane];
+                    }
+#pragma unroll
+                    for (int k = 0; k < GQ; ++k) sum += x[k];
+#pragma unroll
+                    for (int k = 0; k < GQ; ++k) x[k] = y[k];
+                }
+            } else {
+                for (int jj = 0; jj < nj; ++jj) sum += buf[b][jj][lane];
+            }
+        }
+        __syncthreads();
+    }
+    if (lane < 3 * ns) {
+        const int i = i0 + lane / 3;
+        const int comp = lane % 3;
+        double* o = reinterpret_cast<double*>(a + i);
+        o[comp] = id[i] > 0 ? sum : 0.0;
+        if (comp == 0) o[3] = 0.0;
+    }
+}
+
+template <bool OV, bool EV>
+hipError_t go_fp64_seq(int np, const Sweep& sw, const GravBuffers& B, const PairTest& P, const EventSink& E,
+                       hipStream_t s)
+{
+    dim3 g((sw.sink1 - sw.sink0 + SQ_SG - 1) / SQ_SG);
+    if (np <= 3)
+        hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 3>), g, dim3(256), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id, sw,
+                           P, E, B.ctl);
+    else if (np <= 7)
+        hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 7>), g, dim3(512), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id, sw,
+                           P, E, B.ctl);
+    else
+        hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 15>), g, dim3(1024), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
+                           sw, P, E, B.ctl);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------------
@@ -547,6 +732,16 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
     const int nsink = sw.sink1 - sw.sink0;
     const int nsrc = sw.src1 - sw.src0;
     if (nsink <= 0) return hipSuccess;
+    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0) {
+        const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
+        const bool ev = P.enabled != 0;
+        // producer waves per block: the block size if one was asked for, else many producers
+        // when there are few blocks to spread over the chip
+        const int nblk = (nsink + SQ_SG - 1) / SQ_SG;
+        const int np = C.block_seq > 0 ? C.block_seq / 64 - 1 : (nblk >= 128 ? 7 : 15);
+        if (ov) return ev ? go_fp64_seq<true, true>(np, sw, B, P, E, s) : go_fp64_seq<true, false>(np, sw, B, P, E, s);
+        return ev ? go_fp64_seq<false, true>(np, sw, B, P, E, s) : go_fp64_seq<false, false>(np, sw, B, P, E, s);
+    }
     const bool use_split = nsrc > 0 && nsink <= C.split_max_sinks && nsrc >= 1024 &&
                            split_scratch_bytes(sw) <= B.scratch_bytes;
     if (use_split) {

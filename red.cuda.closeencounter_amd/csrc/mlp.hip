@@ -15,9 +15,13 @@
 //   p1[u] = b1[u] + sum_f W1[u][f] z[f];     h1 = gelu(p1)
 //   p2[u] = b2[u] + sum_v W2[u][v] h1[v];    h2 = gelu(p2)
 //   y     = b3    + sum_u W3[u] h2[u]
-// Loss: mean squared error over the batch.  Gradients are taken at the BF16 weights
+// Loss: mean squared error, or the pinball (quantile) loss of level tau,
+//   L = mean(max(tau (t - y), (tau - 1) (t - y))),
+// whose minimiser is the tau-quantile of the target given the input.  Gradients are taken
+// at the BF16 weights
 // (the FP32 master receives them), per 32-sample block, summed in block order: results
 // are deterministic.  Adam updates the master copy and re-rounds the BF16 image.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -121,26 +125,30 @@ __global__ __launch_bounds__(256) void k_mlp_predict(const unsigned short* __res
     for (int s = threadIdx.x; s < ns; s += blockDim.x) Y[s0 + s] = L.out[s];
 }
 
-// Per-block gradient sums of mean((y - t)^2) over n samples (the 1/n is applied here) into
-// G[block][NPARAM] and the block's sum of squared errors into Lsum[block].
+// Per-block gradient sums of the mean loss over n samples (the 1/n is applied here) into
+// G[block][NPARAM] and the block's sum of per-sample losses into Lsum[block].
 __global__ __launch_bounds__(256) void k_mlp_grad(const unsigned short* __restrict__ p, const float* __restrict__ norm,
                                                   const float* __restrict__ X, const float* __restrict__ T, int n,
-                                                  float* __restrict__ G, float* __restrict__ Lsum)
+                                                  int pinball, float tau, float* __restrict__ G,
+                                                  float* __restrict__ Lsum)
 {
     __shared__ Lds L;
     const int s0 = blockIdx.x * SB;
     const int ns = min(SB, n - s0);
     load_block(L, p, norm, X, s0, ns);
     forward(L, ns);
-    const float two_over_n = 2.0f / (float)n;
-    for (int s = threadIdx.x; s < ns; s += blockDim.x) L.dy[s] = two_over_n * (L.out[s] - T[s0 + s]);
+    const float inv_n = 1.0f / (float)n;
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        const float e = T[s0 + s] - L.out[s];
+        L.dy[s] = pinball ? (e > 0.0f ? -tau : 1.0f - tau) * inv_n : 2.0f * inv_n * (L.out[s] - T[s0 + s]);
+    }
     __syncthreads();
     float* g = G + (size_t)blockIdx.x * NPARAM;
     if (threadIdx.x == 0) {
         float se = 0.0f, db3 = 0.0f;
         for (int s = 0; s < ns; s++) {
-            const float e = L.out[s] - T[s0 + s];
-            se = fmaf(e, e, se);
+            const float e = T[s0 + s] - L.out[s];
+            se += pinball ? fmaxf(tau * e, (tau - 1.0f) * e) : e * e;
             db3 += L.dy[s];
         }
         Lsum[blockIdx.x] = se;
@@ -228,23 +236,145 @@ __global__ void k_mlp_to_bf16(const float* __restrict__ w, unsigned short* __res
     if (k < NPARAM) wb[k] = to_bf16(w[k]);
 }
 
-// One proposal inside the step loop: completes the feature vector from the device-side
-// state features, writes it to out[0..11] and, with a network, its output to out[12].
-__global__ __launch_bounds__(64) void k_mlp_propose(const unsigned short* __restrict__ p, const float* __restrict__ norm,
-                                                    red::ProposeArgs A, const red::StepFeat* __restrict__ f,
-                                                    float* __restrict__ out)
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+constexpr int FEAT_SRC = 64;    // sources per work item of the pair pass
+
+// The step-start kernel of the proposer.
+//
+// State features.  Work items, one wave each: [0, n0): (64 SI sinks of 1..n_si-1, 64-source
+// chunk of 1..n_massive-1); [n0, n0+n1): (64 NSI sinks, chunk of SI sources 1..n_si-1); then
+// one item per 64 bodies 1..n-1.  Lanes run over sinks and every source of a chunk reaches
+// the wave as one broadcast record, as in the force sweeps; waves take items grid-stride and
+// keep running extrema (exact in any order).  The pair quantity is
+// q^3 = d^3 3 m_star / ((m_i + m_j) ((h_i + h_j)/2)^3), heliocentric h: FP64 differences,
+// FP32 ratio.  Per body: |a| and |a - a_kep| / |a_kep|, a_kep = -m_star (r - r_star)/|r - r_star|^3.
+// Each block stores its extrema in its own slot; the last block to finish (counter) reduces
+// the slots, completes the feature vector, runs the network and applies the proposal.
+__global__ __launch_bounds__(256) void k_learn_step(const red::d4* __restrict__ r, const red::d4* __restrict__ a,
+                                                    const int* __restrict__ id, int n_si, int n_massive, int n,
+                                                    long n0, int ch0, long n1, int ch1, long n_items,
+                                                    float* __restrict__ part, unsigned* __restrict__ counter,
+                                                    float* __restrict__ out, const unsigned short* __restrict__ wb,
+                                                    const float* __restrict__ norm, red::ProposeArgs A,
+                                                    red::Ctl* __restrict__ ctl, int apply, double eta_n,
+                                                    double safety, double max_log)
 {
     __shared__ Lds L;
+    __shared__ float red3[3][4];
     __shared__ float x[NF];
+    __shared__ int last;
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
+    const red::d4 star = r[0];
+    const double ms = star.w;
+    float best = __uint_as_float(0x7f800000u), amax = 0.0f, pmax = 0.0f;
+    for (long w = (long)blockIdx.x * (blockDim.x >> 6) + wid; w < n_items; w += nwaves) {
+        if (w < n0 + n1) {
+            int i, i1, src0, src1;
+            long c;
+            if (w < n0) {
+                i = 1 + (int)(w / ch0) * 64 + lane; i1 = n_si; c = w % ch0; src0 = 1; src1 = n_massive;
+            } else {
+                const long u = w - n0;
+                i = n_si + (int)(u / ch1) * 64 + lane; i1 = n_massive; c = u % ch1; src0 = 1; src1 = n_si;
+            }
+            const bool live = i < i1 && id[i] > 0;
+            const red::d4 ri = r[live ? i : 0];
+            const int j0 = src0 + (int)c * FEAT_SRC;
+            const int j1 = min(src1, j0 + FEAT_SRC);
+            const double hx = ri.x - star.x, hy = ri.y - star.y, hz = ri.z - star.z;
+            const float hi = sqrtf((float)(hx * hx + hy * hy + hz * hz));
+            for (int j = j0; j < j1; ++j) {
+                const red::d4 rj = r[j];
+                const int idj = id[j];
+                const double mm = ri.w + rj.w;
+                if (!live || j == i || idj <= 0 || !(mm > 0.0)) continue;
+                const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
+                const double gx = rj.x - star.x, gy = rj.y - star.y, gz = rj.z - star.z;
+                const float d2 = (float)(dx * dx + dy * dy + dz * dz);
+                const float hj = sqrtf((float)(gx * gx + gy * gy + gz * gz));
+                const float hm = 0.5f * (hi + hj);
+                const float q3 = d2 * sqrtf(d2) * (float)(3.0 * ms) / ((float)mm * hm * hm * hm);
+                best = fminf(best, q3);
+            }
+        } else {
+            const int i = 1 + (int)(w - n0 - n1) * 64 + lane;
+            if (i < n && id[i] > 0) {
+                const red::d4 ri = r[i];
+                const red::d4 ai = a[i];
+                const double hx = ri.x - star.x, hy = ri.y - star.y, hz = ri.z - star.z;
+                const double h2 = hx * hx + hy * hy + hz * hz;
+                const double s = -ms / (h2 * sqrt(h2));
+                const double kx = s * hx, ky = s * hy, kz = s * hz;
+                const double ex = ai.x - kx, ey = ai.y - ky, ez = ai.z - kz;
+                const double an = sqrt(ai.x * ai.x + ai.y * ai.y + ai.z * ai.z);
+                const double kn = sqrt(kx * kx + ky * ky + kz * kz);
+                const double en = sqrt(ex * ex + ey * ey + ez * ez);
+                amax = fmaxf(amax, (float)an);
+                if (kn > 0.0) pmax = fmaxf(pmax, (float)(en / kn));
+            }
+        }
+    }
+    best = wave_min(best);
+    amax = wave_max(amax);
+    pmax = wave_max(pmax);
+    if (lane == 0) { red3[0][wid] = best; red3[1][wid] = amax; red3[2][wid] = pmax; }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            best = fminf(best, red3[0][k]);
+            amax = fmaxf(amax, red3[1][k]);
+            pmax = fmaxf(pmax, red3[2][k]);
+        }
+        part[3 * blockIdx.x + 0] = best;
+        part[3 * blockIdx.x + 1] = amax;
+        part[3 * blockIdx.x + 2] = pmax;
+        __threadfence();
+        const unsigned done = atomicAdd(counter, 1u);
+        last = done == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    // last block: every other block's slot is visible (release fence before its increment);
+    // all its threads read the slots and the block reduces them
+    __threadfence();
+    {
+        float q3 = __uint_as_float(0x7f800000u), ma = 0.0f, mp = 0.0f;
+        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+            q3 = fminf(q3, __hip_atomic_load(&part[3 * b + 0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            ma = fmaxf(ma, __hip_atomic_load(&part[3 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            mp = fmaxf(mp, __hip_atomic_load(&part[3 * b + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        }
+        q3 = wave_min(q3);
+        ma = wave_max(ma);
+        mp = wave_max(mp);
+        if (lane == 0) { red3[0][wid] = q3; red3[1][wid] = ma; red3[2][wid] = mp; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float q3 = red3[0][0], ma = red3[1][0], mp = red3[2][0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            q3 = fminf(q3, red3[0][k]);
+            ma = fmaxf(ma, red3[1][k]);
+            mp = fmaxf(mp, red3[2][k]);
+        }
+        *counter = 0u;   // ready for the next launch on this stream
         for (int k = 0; k < NF; k++) x[k] = A.x[k];
-        const float minq3 = __uint_as_float(f->min_q3_bits);
-        const float maxa = __uint_as_float(f->max_a_bits);
-        const float pert = __uint_as_float(f->max_pert_bits);
         // log10(d / R_H) = log10(q^3) / 3; every value is clamped to keep the features finite
-        const float lq = fminf(fmaxf(log10f(fmaxf(minq3, 1e-30f)) / 3.0f, -10.0f), 10.0f);
-        const float la = fminf(fmaxf(log10f(fmaxf(maxa, 1e-30f)), -30.0f), 30.0f);
-        const float lp = fminf(fmaxf(log10f(fmaxf(pert, 1e-30f)), -30.0f), 30.0f);
+        const float lq = fminf(fmaxf(log10f(fmaxf(q3, 1e-30f)) / 3.0f, -10.0f), 10.0f);
+        const float la = fminf(fmaxf(log10f(fmaxf(ma, 1e-30f)), -30.0f), 30.0f);
+        const float lp = fminf(fmaxf(log10f(fmaxf(mp, 1e-30f)), -30.0f), 30.0f);
         x[red::FEAT_LOG_MIN_SEP] = lq;
         x[red::FEAT_D_LOG_MIN_SEP] = A.have_prev ? lq - A.prev_log_min_sep : 0.0f;
         x[red::FEAT_LOG_MAX_ACC] = la;
@@ -254,10 +384,16 @@ __global__ __launch_bounds__(64) void k_mlp_propose(const unsigned short* __rest
         out[NF] = 0.0f;
     }
     __syncthreads();
-    if (!p) return;
-    load_block(L, p, norm, x, 0, 1);
+    if (!wb) return;
+    load_block(L, wb, norm, x, 0, 1);
     forward(L, 1);
-    if (threadIdx.x == 0) out[NF] = L.out[0];
+    if (threadIdx.x == 0) {
+        out[NF] = L.out[0];
+        if (apply) {
+            const double y = fmin(max_log, fmax(-max_log, (double)L.out[0]));
+            ctl->dt = safety * eta_n * exp(y);
+        }
+    }
 }
 
 }  // namespace
@@ -273,6 +409,8 @@ struct red_mlp {
     float *X = nullptr, *T = nullptr, *P = nullptr;
     int cap_n = 0, cap_blocks = 0, cap_losses = 0;
     long step = 0;
+    int pinball = 0;               // RED_MLP_LOSS_*
+    float tau = 0.5f;
     int hipfail(hipError_t e, const char* where)
     {
         err = std::string(where) + ": " + hipGetErrorString(e);
@@ -288,13 +426,25 @@ struct red_mlp {
 
 namespace red {
 
-hipError_t launch_mlp_propose(const red_mlp* M, const ProposeArgs& A, const StepFeat* f, float* out, hipStream_t s)
+int mlp_device(const red_mlp* M) { return M->device; }
+
+hipError_t launch_learn_step(const red_mlp* net, const ProposeArgs& A, const d4* r, const d4* a, const int* id,
+                             int n_si, int n_massive, int n, const LearnBuffers& B, Ctl* ctl, int apply,
+                             double eta_n, double safety, double max_log, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mlp_propose, dim3(1), dim3(64), 0, s, M ? M->wb : nullptr, M ? M->norm : nullptr, A, f, out);
+    const long c0 = n_si > 1 ? (n_si - 1 + 63) / 64 : 0;                // SI sink groups (no star)
+    const long c1 = n_si > 1 ? (n_massive - n_si + 63) / 64 : 0;        // NSI sink groups
+    const int ch0 = n_massive > 1 ? (n_massive - 1 + FEAT_SRC - 1) / FEAT_SRC : 0;
+    const int ch1 = n_si > 1 ? (n_si - 1 + FEAT_SRC - 1) / FEAT_SRC : 0;
+    const long n0 = c0 * ch0, n1 = c1 * ch1;
+    const long nb2 = n > 1 ? (n - 1 + 63) / 64 : 0;
+    const long items = n0 + n1 + nb2;
+    const long blocks = std::max<long>(1, std::min<long>((items + 3) / 4, LEARN_MAX_BLOCKS));
+    hipLaunchKernelGGL(k_learn_step, dim3((unsigned)blocks), dim3(256), 0, s, r, a, id, n_si, n_massive, n, n0, ch0,
+                       n1, ch1, items, B.part, B.counter, B.out, net ? net->wb : nullptr, net ? net->norm : nullptr, A,
+                       ctl, apply, eta_n, safety, max_log);
     return hipGetLastError();
 }
-
-int mlp_device(const red_mlp* M) { return M->device; }
 
 }  // namespace red
 
@@ -372,8 +522,8 @@ int red_mlp_create(int device, const float* params, red_mlp** out)
     MLP_HIP(M, hipMalloc(&M->m, sizeof(float) * NPARAM));
     MLP_HIP(M, hipMalloc(&M->v, sizeof(float) * NPARAM));
     MLP_HIP(M, hipMalloc(&M->g, sizeof(float) * NPARAM));
-    MLP_HIP(M, hipMemset(M->m, 0, sizeof(float) * NPARAM));
-    MLP_HIP(M, hipMemset(M->v, 0, sizeof(float) * NPARAM));
+    MLP_HIP(M, hipMemsetAsync(M->m, 0, sizeof(float) * NPARAM, M->stream));
+    MLP_HIP(M, hipMemsetAsync(M->v, 0, sizeof(float) * NPARAM, M->stream));
     std::vector<float> w(NPARAM, 0.0f), nrm(2 * NF, 0.0f);
     if (params) memcpy(w.data(), params, sizeof(float) * NPARAM);
     for (int f = 0; f < NF; f++) nrm[NF + f] = 1.0f;
@@ -400,6 +550,22 @@ int red_mlp_set_normalizer(red_mlp* M, const float* mean, const float* inv_std)
     MLP_HIP(M, hipSetDevice(M->device));
     MLP_HIP(M, hipMemcpyAsync(M->norm, nrm, sizeof(nrm), hipMemcpyHostToDevice, M->stream));
     MLP_HIP(M, hipStreamSynchronize(M->stream));
+    return RED_OK;
+}
+
+int red_mlp_set_loss(red_mlp* M, int kind, float tau)
+{
+    if (!M) return RED_ERR_ARG;
+    if (kind != RED_MLP_LOSS_MSE && kind != RED_MLP_LOSS_PINBALL) {
+        M->err = "unknown loss";
+        return RED_ERR_ARG;
+    }
+    if (kind == RED_MLP_LOSS_PINBALL && !(tau > 0.0f && tau < 1.0f)) {
+        M->err = "the quantile level must lie in (0, 1)";
+        return RED_ERR_ARG;
+    }
+    M->pinball = kind == RED_MLP_LOSS_PINBALL;
+    M->tau = tau;
     return RED_OK;
 }
 
@@ -439,7 +605,8 @@ int red_mlp_gradient(red_mlp* M, const float* X, const float* T, int n, float* g
     if (rc) return rc;
     MLP_HIP(M, hipMemcpyAsync(M->X, X, sizeof(float) * (size_t)n * NF, hipMemcpyHostToDevice, M->stream));
     MLP_HIP(M, hipMemcpyAsync(M->T, T, sizeof(float) * (size_t)n, hipMemcpyHostToDevice, M->stream));
-    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, M->stream, M->wb, M->norm, M->X, M->T, n, M->G, M->Ls);
+    hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, M->stream, M->wb, M->norm, M->X, M->T, n, M->pinball, M->tau,
+                       M->G, M->Ls);
     MLP_HIP(M, hipGetLastError());
     hipLaunchKernelGGL(k_mlp_reduce, pgrid(), dim3(PB), 0, M->stream, M->G, nb, M->g, M->Ls, M->loss, n);
     MLP_HIP(M, hipGetLastError());
@@ -470,7 +637,7 @@ int red_mlp_train(red_mlp* M, const float* X, const float* T, int n, int epochs,
             const int nbs = std::min(batch, n - s);
             const int nb = (nbs + SB - 1) / SB;
             hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, M->stream, M->wb, M->norm, M->X + (size_t)s * NF,
-                               M->T + s, nbs, M->G, M->Ls);
+                               M->T + s, nbs, M->pinball, M->tau, M->G, M->Ls);
             MLP_HIP(M, hipGetLastError());
             hipLaunchKernelGGL(k_mlp_reduce, pgrid(), dim3(PB), 0, M->stream, M->G, nb, M->g, M->Ls, M->loss + li, nbs);
             MLP_HIP(M, hipGetLastError());

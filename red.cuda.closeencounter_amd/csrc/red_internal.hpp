@@ -99,6 +99,7 @@ struct GravConfig {
     int split_max_sinks;           // sweeps with fewer sinks use the split path
     int fp64_split;                // FP64 sums may be partitioned over sources
     int source_parts;              // forced part count, 0 = automatic
+    int block_seq;                 // producer/consumer block size (256, 512, 1024), 0 = automatic
 };
 
 // Number of source parts a sweep is cut into.
@@ -179,26 +180,28 @@ enum {
     FEAT_LOG_MAX_PERT = 10,
     FEAT_D_LOG_MAX_PERT = 11
 };
-// State features of one step start, reduced by bit-pattern atomics (non-negative floats
-// order like their bits): min over pairs of (d / R_H)^3, max |a|, max |a - a_kep| / |a_kep|.
-struct StepFeat {
-    unsigned min_q3_bits;
-    unsigned max_a_bits;
-    unsigned max_pert_bits;
-    unsigned pad;
-};
 struct ProposeArgs {
     float x[MLP_NF];            // history features; device slots are overwritten
     float prev_log_min_sep;
     float prev_log_max_pert;
     int have_prev;
 };
-// Resets f and reduces the state features of (r, a): pairs SI x massive and NSI x SI without
-// the star (index 0), bodies 1..n-1 with id > 0.
-hipError_t launch_step_features(const d4* r, const d4* a, const int* id, int n_si, int n_massive, int n,
-                                StepFeat* f, hipStream_t s);
-hipError_t launch_mlp_propose(const red_mlp* M, const ProposeArgs& A, const StepFeat* f, float* out, hipStream_t s);
+// Device buffers of the fused step-start kernel.
+constexpr int LEARN_MAX_BLOCKS = 512;
+struct LearnBuffers {
+    float* part;                // 3 floats per block: min (d/R_H)^3, max |a|, max |a - a_kep|/|a_kep|
+    unsigned* counter;          // blocks finished; the last block resets it to 0
+    float* out;                 // host-mapped: features [0..11], network output [12]
+};
+// One launch at the start of a step: reduces the state features of (r, a) -- pairs SI x
+// massive and NSI x SI without the star (index 0), bodies 1..n-1 with id > 0 -- completes the
+// feature vector from A, runs the network (if net) and, with apply, sets
+// ctl->dt = safety * eta_n * exp(clamp(y, +-max_log)).
+hipError_t launch_learn_step(const red_mlp* net, const ProposeArgs& A, const d4* r, const d4* a, const int* id,
+                             int n_si, int n_massive, int n, const LearnBuffers& B, Ctl* ctl, int apply,
+                             double eta_n, double safety, double max_log, hipStream_t s);
 int mlp_device(const red_mlp* M);
+
 
 }  // namespace red
 

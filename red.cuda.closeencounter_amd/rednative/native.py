@@ -123,6 +123,7 @@ SIGNATURES = {
     "red_mlp_last_error": (ctypes.c_char_p, [_P]),
     "red_mlp_set_normalizer": (_I, [_P, _P, _P]),
     "red_mlp_get_params": (_I, [_P, _P, _P, _P]),
+    "red_mlp_set_loss": (_I, [_P, _I, ctypes.c_float]),
     "red_mlp_predict": (_I, [_P, _P, _I, _P]),
     "red_mlp_gradient": (_I, [_P, _P, _P, _I, _P, _P]),
     "red_mlp_train": (_I, [_P, _P, _P, _I, _I, _I, ctypes.c_float, _P]),
@@ -587,6 +588,11 @@ class Mlp:
         m = np.ascontiguousarray(mean, dtype=np.float32)
         s = np.ascontiguousarray(inv_std, dtype=np.float32)
         self._chk(self.L.red_mlp_set_normalizer(self.h, _ptr(m), _ptr(s)))
+
+    LOSS_MSE, LOSS_PINBALL = 0, 1
+
+    def set_loss(self, kind: int, tau: float = 0.5):
+        self._chk(self.L.red_mlp_set_loss(self.h, int(kind), ctypes.c_float(tau)))
 
     def params(self):
         """(FP32 master parameters, BF16 image as uint16, normaliser[24])."""

@@ -5,11 +5,14 @@ dt <- 0.9 dt (tol/err)^(1/order) (red.cuda/src/red.cuda/int_rungekutta8.cu:776-7
 
 Target.  For an attempt with step dt and error err, eta = dt (tol/err)^(1/order) is the
 step that would have met the tolerance exactly.  The reference controller proposes
-0.9 eta_n for step n+1, i.e. it assumes eta does not change.  The network predicts
-y = ln(eta_(n+1) / eta_n) from the 12 features of a step record (recent eta ratios, the
-last error ratio and rejections, pair events, and state features: closest approach in
-mutual Hill radii, the largest acceleration and the largest non-Keplerian acceleration
-ratio, with their changes), and the library proposes dt = safety * eta_n * exp(y).  The
+0.9 eta_n for step n+1, i.e. it assumes eta does not change and keeps a fixed margin
+(the error then sits near 0.9^order = 0.43 of the tolerance for RKF7(8)).  The network
+predicts a low quantile (1 %) of y = ln(eta_(n+1) / eta_n) from the 12 features of a step
+record (recent eta ratios, the last error ratio and rejections, pair events, and state
+features: closest approach in mutual Hill radii, the largest acceleration and the largest
+non-Keplerian acceleration ratio, with their changes) -- trained with the pinball loss --
+and the library proposes dt = safety * eta_n * exp(y): close to the tolerance where the
+history and the state say the step will not shrink, smaller ahead of an encounter.  The
 embedded error estimate still accepts or rejects every attempt and a rejected attempt is
 retried with the reference controller's step, so the network only decides where a step
 starts.  Records are written by the library itself (red_history_*), one per step.
@@ -19,6 +22,7 @@ the library's own HIP forward/backward/Adam kernels (red_mlp_train).
 """
 from __future__ import annotations
 
+import os
 import time
 from dataclasses import dataclass
 from typing import Iterable, List, Optional
@@ -28,6 +32,8 @@ import numpy as np
 from . import native
 
 NF = native.MLP_FEATURES
+# network trained by train_proposer (RKF7(8) step records, see data/README)
+DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "proposer_rkf78.npz")
 LABEL, OUTPUT, TRIED, FIRST_OK = 12, 13, 14, 15
 DAY = 0.01720209895        # internal time units per day
 YEAR_DAYS = 365.25
@@ -70,11 +76,16 @@ def collect(bodies, n_steps: int, t_stop: Optional[float] = None, explore: Optio
     zero = native.Mlp(np.zeros(native.mlp_param_count(), np.float32), device=device) if explore else None
     rng = np.random.default_rng(seed)
     k = 0
-    while k < n_steps and (t_stop is None or d.t <= t_stop):
-        if zero is not None:
-            d.set_proposer(zero, safety=float(rng.uniform(*explore)), max_log_change=1.0)
-        d.step()
-        k += 1
+    try:
+        while k < n_steps and (t_stop is None or d.t <= t_stop):
+            if zero is not None:
+                d.set_proposer(zero, safety=float(rng.uniform(*explore)), max_log_change=1.0)
+            d.step()
+            k += 1
+    except native.RedError as e:
+        # a disk that fails (e.g. the reference controller's infinite step after an error
+        # estimate of 0) ends its history; the records up to the failure stay valid
+        print(f"collect: disk stopped after {k} steps: {e}", flush=True)
     H = d.history()
     d.close()
     if zero is not None:
@@ -98,9 +109,16 @@ def collect_many(disks: List, n_steps: int, threads: int = 8, progress=None, **k
     return np.concatenate(out) if out else np.zeros((0, native.HIST_FIELDS), np.float32)
 
 
+# The network is trained for a low quantile of the step change, so that a proposal
+# dt = SAFETY * eta_n * exp(y) is rejected rarely; SAFETY is a small fixed margin.
+QUANTILE = 0.01
+SAFETY = 0.995
+
+
 def train(records: np.ndarray, epochs: int = 20, batch: int = 4096, lr: float = 2e-3, seed: int = 0,
-          device: int = 0, net: Optional[native.Mlp] = None):
-    """Trains a network on the records' features and labels; returns (net, epoch losses)."""
+          device: int = 0, net: Optional[native.Mlp] = None, quantile: Optional[float] = QUANTILE):
+    """Trains a network on the records' features and labels with the pinball loss of level
+    `quantile` (None: mean squared error); returns (net, epoch losses)."""
     R = np.asarray(records, np.float32)
     R = R[np.all(np.isfinite(R), axis=1)]
     rng = np.random.default_rng(seed)
@@ -110,6 +128,10 @@ def train(records: np.ndarray, epochs: int = 20, batch: int = 4096, lr: float = 
         net = native.Mlp(init_params(seed), device=device)
         mean, istd = normalizer(X)
         net.set_normalizer(mean, istd)
+    if quantile is None:
+        net.set_loss(native.Mlp.LOSS_MSE)
+    else:
+        net.set_loss(native.Mlp.LOSS_PINBALL, quantile)
     losses = net.train(X, T, epochs=epochs, batch=batch, lr=lr)
     return net, losses
 
@@ -143,7 +165,7 @@ class RunStats:
         return self.sim_years / max(self.seconds, 1e-12) * 3600.0
 
 
-def run(bodies, t_years: float, net: Optional[native.Mlp] = None, safety: float = 0.9,
+def run(bodies, t_years: float, net: Optional[native.Mlp] = None, safety: float = SAFETY,
         max_log_change: float = 1.0, max_steps: int = 10 ** 9, warm_steps: int = 0, **kw) -> RunStats:
     """Integrates for t_years (from the disk's start) with or without the proposer and
     reports steps, attempts, rejections, wall time and the energy / angular-momentum drift."""

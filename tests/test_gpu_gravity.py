@@ -2,8 +2,8 @@
 
 FP64 mode must equal the oracle bit for bit (same operation order, no fused
 multiply-adds, correctly rounded sqrt and division on gfx950), for every launch path:
-broadcast and LDS-tile sweeps, 1/2/4 sinks per work-item, and the split path used for
-sweeps with few sinks.  Mixed mode must stay within a relative tolerance of 2e-6 per
+broadcast and LDS-tile sweeps, 1/2/4 sinks per work-item, the split path used for
+sweeps with few sinks, and the producer/consumer sweep (the FP64 default).  Mixed mode must stay within a relative tolerance of 2e-6 per
 body of the FP64 acceleration (FP32 r^-3 carries about 1e-7 relative error per pair)
 and must produce the same event list as FP64 on the same state.
 """
@@ -34,7 +34,8 @@ def _events_equal(ev_o, ev_g):
             assert_bitwise(np.array(eo[k], dtype=np.float64), np.array(eg[k], dtype=np.float64), k)
 
 
-@pytest.mark.parametrize("spt,strategy,block", [(1, 1, 256), (2, 1, 256), (4, 1, 128), (1, 2, 256), (2, 2, 64)])
+@pytest.mark.parametrize("spt,strategy,block", [(1, 1, 256), (2, 1, 256), (4, 1, 128), (1, 2, 256), (2, 2, 64),
+                                                (1, 3, 256)])
 def test_fp64_allsi_bit_exact(oracle_mod, native_mod, spt, strategy, block):
     print(f"case allsi n=3001 spt={spt} strategy={strategy} block={block}", flush=True)
     b = generate.to_internal(generate.disk_allsi(3001, seed=11))
@@ -51,11 +52,13 @@ def test_fp64_typed_disk_bit_exact(oracle_mod, native_mod):
     assert_bitwise(d.accel(1.0, b.r, b.v), o.accel(1.0, b.r, b.v), "accel")
 
 
-def test_fp64_split_path_bit_exact(oracle_mod, native_mod):
-    # 3 SI sinks against 6003 sources and 3000 NI sinks: the SI sweep takes the split path
-    print("case disk_mixed 3+3000+3000", flush=True)
+@pytest.mark.parametrize("strategy", [1, 3])
+def test_fp64_split_path_bit_exact(oracle_mod, native_mod, strategy):
+    # 3 SI sinks against 6003 sources and 3000 NI sinks: with the broadcast strategy the SI
+    # sweep takes the split path; the producer/consumer strategy takes every sweep
+    print(f"case disk_mixed 3+3000+3000 strategy={strategy}", flush=True)
     b = generate.to_internal(generate.disk_mixed(n_pl=3000, n_tp=3000, seed=5))
-    o, d = _pair(oracle_mod, native_mod, b)
+    o, d = _pair(oracle_mod, native_mod, b, strategy=strategy)
     assert_bitwise(d.accel(2.0, b.r, b.v), o.accel(2.0, b.r, b.v), "accel")
 
 

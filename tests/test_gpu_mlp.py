@@ -55,6 +55,22 @@ def test_gradient_matches_oracle(native_mod, n):
     assert not bad.any(), (np.argwhere(bad)[:5].ravel(), g[bad][:5], gref[bad][:5])
 
 
+@pytest.mark.parametrize("tau", [0.01, 0.5])
+def test_pinball_gradient_matches_oracle(native_mod, tau):
+    net, p, nrm = make_net(native_mod, seed=70)
+    net.set_loss(native_mod.Mlp.LOSS_PINBALL, tau)
+    rng = np.random.default_rng(71)
+    X = rng.normal(size=(500, mo.NF)).astype(np.float32)
+    T = rng.normal(size=500).astype(np.float32)
+    loss, g = net.gradient(X, T)
+    lref, gref = mo.loss_and_grad(mo.bf16_round(p), mo.standardize(X, nrm), T, tau=tau)
+    print(f"tau={tau} loss {loss} vs {lref}")
+    assert abs(loss - lref) <= 1e-5 * (1.0 + abs(lref))
+    scale = np.abs(gref).max()
+    bad = np.abs(g - gref) > 1e-4 * np.abs(gref) + 2e-6 * scale
+    assert not bad.any(), (np.argwhere(bad)[:5].ravel(), g[bad][:5], gref[bad][:5])
+
+
 def test_adam_step_and_bf16_image(native_mod):
     net, p, nrm = make_net(native_mod, seed=40)
     rng = np.random.default_rng(41)

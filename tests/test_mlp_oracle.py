@@ -68,6 +68,35 @@ def test_gradient_matches_finite_differences():
         assert abs(fd - g[k]) <= 1e-7 + 1e-5 * abs(g[k]), (k, fd, g[k])
 
 
+def test_pinball_gradient_matches_finite_differences():
+    p = random_net(4)
+    rng = np.random.default_rng(5)
+    Z = rng.normal(size=(9, mo.NF))
+    T = rng.normal(size=9)
+    tau = 0.1
+    loss, g = mo.loss_and_grad(p, Z, T, tau=tau)
+    e = T - mo.forward(p, Z)["y"]
+    assert loss == pytest.approx(np.mean(np.where(e > 0, tau * e, (tau - 1) * e)))
+    h = 1e-7
+    for k in list(range(mo.OFF["W3"][0], mo.NPARAM)) + list(range(0, mo.OFF["W2"][1], 97)):
+        q = p.copy(); q[k] += h
+        lp = mo.loss_and_grad(q, Z, T, tau=tau)[0]
+        q[k] -= 2 * h
+        lm = mo.loss_and_grad(q, Z, T, tau=tau)[0]
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-7 + 1e-5 * abs(g[k]), (k, fd, g[k])
+
+
+def test_pinball_minimiser_is_the_quantile():
+    # the constant minimising the mean pinball loss over a sample is its tau-quantile
+    rng = np.random.default_rng(6)
+    t = rng.normal(size=2001)
+    grid = np.linspace(-3, 3, 6001)
+    for tau in (0.05, 0.5, 0.9):
+        L = [np.mean(np.maximum(tau * (t - c), (tau - 1) * (t - c))) for c in grid]
+        assert abs(grid[int(np.argmin(L))] - np.quantile(t, tau, method="inverted_cdf")) < 2e-3
+
+
 def test_output_closed_form_without_hidden_weights():
     rng = np.random.default_rng(3)
     W1 = np.zeros((mo.NH, mo.NF)); W2 = np.zeros((mo.NH, mo.NH))
