@@ -103,13 +103,13 @@ def cpu_baseline(bodies_full, target_s=10.0):
                        f"{ntp} of its 65533 test particles: {p} pairs in {dt:.2f} s")
 
 
-def proposer_section(native, years):
+def proposer_section(native, years, use_graph=True):
     """Learned step-size proposer on the 16384-body disk (RKF7(8), tol 1e-12, FP64):
     reference controller against the shipped network, same span of simulated time."""
     import numpy as np
     from rednative import generate, proposer
     b = generate.to_internal(generate.disk_powerlaw(n=16383, seed=202))
-    kw = dict(integrator=4, adaptive=True, tolerance=1e-12, dt_day=0.01, use_graph=True)
+    kw = dict(integrator=4, adaptive=True, tolerance=1e-12, dt_day=0.01, use_graph=use_graph)
     net = proposer.load(proposer.DEFAULT_MODEL)
     out = {"workload": "1 star + 16383 protoplanets/planetesimals (m >= 1e-10 M_sun self-interacting), "
                        "power-law masses, RKF7(8) tol 1e-12, FP64 sequential sums",
@@ -157,6 +157,34 @@ def ensemble_section(native, days, sample):
             "autotuned_block": block, "autotuned_jtile": jtile}
 
 
+def copy_calibration(mib):
+    """One device-to-device copy of `mib` MiB (the runtime's copy kernel): the reference
+    point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0        # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a); hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n}))
+
+
+def load_traffic(path):
+    """HBM bytes per launch of the test-particle sweep from the committed counter summary
+    (two rocprofv3 --pmc runs, FETCH_SIZE then WRITE_SIZE, scaled by a copy of known size)."""
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return float(t["hbm_bytes_per_launch"]), t
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,24 +193,41 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kahan", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
+    ap.add_argument("--pmc-summary", default=os.path.join(ROOT, "profiles", "pmc_grav_mixed_ni_n131072.json"))
     ap.add_argument("--no-proposer", action="store_true")
     ap.add_argument("--proposer-years", type=float, default=5.0)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-days", type=float, default=20.0)
+    ap.add_argument("--no-graph", action="store_true", help="proposer section without HIP graphs")
+    ap.add_argument("--only", choices=["proposer", "ensemble"], default=None,
+                    help="run one auxiliary section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the force sweep would shard by sinks with an all-gather of source positions).", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+
     from rednative import generate, native
     native.load_library()
+    if args.only == "proposer":
+        print(json.dumps({"proposer": proposer_section(native, args.proposer_years, not args.no_graph)}))
+        return
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": ensemble_section(native, args.ensemble_days, 16)}))
+        return
 
     bodies = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
     pairs_eval, parts = pair_counts(bodies.counts)
 
     mixed = run_path(native, bodies, native.PREC_MIXED, args.steps, args.warmup, kahan=args.kahan)
-    fp64 = run_path(native, bodies, native.PREC_FP64, args.fp64_steps, max(1, args.warmup // 2))
+    fp64 = None if args.no_fp64 else run_path(native, bodies, native.PREC_FP64, args.fp64_steps,
+                                                   max(1, args.warmup // 2))
 
     def summarize(r, steps):
         evals = steps + 5 * r["tries"]            # f0 once per step, 5 stages per attempt
@@ -193,14 +238,16 @@ def main():
                     dE_over_E=abs((r["E1"] - r["E0"]) / r["E0"]), sim_days=r["sim_t"] / GAUSS)
 
     sm = summarize(mixed, args.steps)
-    s64 = summarize(fp64, args.fp64_steps)
+    s64 = summarize(fp64, args.fp64_steps) if fp64 else None
 
     ni_ms, ni_n = mixed["prof"]["ni"]
     avg_ni_s = (ni_ms / max(ni_n, 1)) * 1e-3
     ni_pairs = bodies.counts[6] * sum(bodies.counts[:6])
     achieved = FLOP_PER_PAIR * ni_pairs / avg_ni_s / 1e12
-    ni64_ms, ni64_n = fp64["prof"]["ni"]
-    achieved64 = FLOP_PER_PAIR * ni_pairs / ((ni64_ms / max(ni64_n, 1)) * 1e-3) / 1e12
+    if fp64:
+        ni64_ms, ni64_n = fp64["prof"]["ni"]
+        achieved64 = FLOP_PER_PAIR * ni_pairs / ((ni64_ms / max(ni64_n, 1)) * 1e-3) / 1e12
+    traffic, traffic_src = load_traffic(args.pmc_summary)
 
     cpu = None if args.no_cpu else cpu_baseline(bodies)
     prop = None if args.no_proposer else proposer_section(native, args.proposer_years)
@@ -226,15 +273,17 @@ def main():
         "dE_over_E_timed": sm["dE_over_E"],
         "sim_days_timed": sm["sim_days"],
         "attempts_timed": sm["attempts"],
-        "fp64": {"pairs_per_s": s64["pairs_per_s"], "sim_yr_per_hr": s64["sim_yr_per_hr"],
+        "fp64": None if not s64 else
+                {"pairs_per_s": s64["pairs_per_s"], "sim_yr_per_hr": s64["sim_yr_per_hr"],
                  "ms_per_step": s64["ms_per_step"], "dE_over_E_timed": s64["dE_over_E"],
                  "steps": args.fp64_steps, "ni_sweep_tflops_20flop": achieved64,
                  "ni_sweep_frac_fp64_peak": achieved64 / PEAK_FP64_TFLOPS},
-        "speedup_sim_yr_per_hr_vs_fp64": sm["sim_yr_per_hr"] / s64["sim_yr_per_hr"],
-        "speedup_pairs_per_s_vs_fp64": sm["pairs_per_s"] / s64["pairs_per_s"],
+        "speedup_sim_yr_per_hr_vs_fp64": sm["sim_yr_per_hr"] / s64["sim_yr_per_hr"] if s64 else None,
+        "speedup_pairs_per_s_vs_fp64": sm["pairs_per_s"] / s64["pairs_per_s"] if s64 else None,
         "roofline": {"bound": "valu", "kernel": "k_grav_mixed_bcast (test-particle sweep)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "traffic_source": None if not traffic_src else traffic_src.get("source"),
                      "launch_ms": avg_ni_s * 1e3, "pairs_per_launch": ni_pairs,
                      "flop_per_pair": FLOP_PER_PAIR},
         "profile_ms": {k: v[0] for k, v in mixed["prof"].items() if v[1]},
