@@ -95,7 +95,8 @@ typedef struct red_params {
     int    kahan_pairs;           /* mixed: compensate every pair, not every tile   */
     int    device;                /* HIP device ordinal                             */
     int    block;                 /* gravity launch shape, 0 = autotuned/default    */
-    int    sinks_per_thread;      /* 0 = default                                    */
+    int    sinks_per_thread;      /* 0 = default; strategy 3: sinks per block
+                                     (4, 8 or 16)                                   */
     int    strategy;              /* 0 = auto, 1 = scalar-broadcast, 2 = LDS tiles,
                                      3 = FP64 producer/consumer (sequential sums;
                                      the FP64 default)                              */

@@ -624,14 +624,22 @@ struct Disk {
     }
 
     // NEW: E = sum 1/2 m v^2 - sum_{interacting i<j} m_i m_j / r_ij ; L = sum m r x v (G = 1 units)
+    // Compensated (Neumaier) sums: millions of tiny pair terms added to a potential
+    // dominated by the star would otherwise lose their low bits.
+    static void nsum(double& s, double& c, double x) {
+        const double t = s + x;
+        if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
     void energy(double* E, double* L) const {
-        double T = 0.0, U = 0.0, Lx = 0.0, Ly = 0.0, Lz = 0.0;
+        double T = 0.0, U = 0.0, cT = 0.0, cU = 0.0, Lx = 0.0, Ly = 0.0, Lz = 0.0;
         int nm = n_massive(), nsi = n_SI();
         for (int i = 0; i < nm; i++) {
             if (md[i].id <= 0) continue;
             const vec_t& r = y[0][i]; const vec_t& v = y[1][i];
             double m = p[i].mass;
-            T += 0.5 * m * ((v.x * v.x) + (v.y * v.y) + (v.z * v.z));
+            nsum(T, cT, 0.5 * m * ((v.x * v.x) + (v.y * v.y) + (v.z * v.z)));
             Lx += m * (r.y * v.z - r.z * v.y);
             Ly += m * (r.z * v.x - r.x * v.z);
             Lz += m * (r.x * v.y - r.y * v.x);
@@ -639,11 +647,11 @@ struct Disk {
             for (int j = i + 1; j < jend; j++) {
                 if (md[j].id <= 0) continue;
                 double dx = y[0][j].x - r.x, dy = y[0][j].y - r.y, dz = y[0][j].z - r.z;
-                U += m * p[j].mass / sqrt((dx * dx) + (dy * dy) + (dz * dz));
+                nsum(U, cU, m * p[j].mass / sqrt((dx * dx) + (dy * dy) + (dz * dz)));
             }
         }
         // NSI sinks interact with SI sources; count each SI-NSI pair once (done above for i < nsi)
-        *E = T - U;
+        *E = (T + cT) - (U + cU);
         L[0] = Lx; L[1] = Ly; L[2] = Lz;
     }
 };

@@ -87,6 +87,8 @@ struct red_ctx {
     BodyUpdate* d_updates = nullptr;
     int updates_cap = 0;
     double* d_energy = nullptr;
+    double* d_energy_part = nullptr;      // potential partials, n_si x chunks
+    size_t energy_part_cap = 0;
     // integrator state (integrator.h:359-383)
     Scheme S;
     double t = 0.0, dt_try = 0.0, dt_did = 0.0, last_err = 0.0;
@@ -479,7 +481,7 @@ int red_ctx_destroy(red_ctx* c)
     for (auto p : c->d_a) (void)hipFree(p);
     (void)hipFree(c->d_mass); (void)hipFree(c->d_radius); (void)hipFree(c->d_id); (void)hipFree(c->d_type);
     (void)hipFree(c->d_ctl); (void)hipFree(c->d_events); (void)hipFree(c->d_events_hc); (void)hipFree(c->d_scratch);
-    (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy);
+    (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy); (void)hipFree(c->d_energy_part);
     (void)hipFree(c->d_partial);
     (void)hipFree(c->d_learn_part); (void)hipFree(c->d_learn_counter);
     for (void* p : c->user_buffers) (void)hipFree(p);
@@ -537,6 +539,7 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     c->gcfg.n_fp64_src = c->cnt[0] + c->cnt[1];
     c->gcfg.block = p->block > 0 ? std::min(256, std::max(64, p->block / 64 * 64)) : 256;
     c->gcfg.block_seq = p->block >= 256 ? (p->block >= 1024 ? 1024 : (p->block >= 512 ? 512 : 256)) : 0;
+    c->gcfg.seq_sinks = p->sinks_per_thread >= 4 ? p->sinks_per_thread : 0;
     c->gcfg.spt = p->sinks_per_thread > 0 ? p->sinks_per_thread : (p->precision == RED_PREC_MIXED ? 2 : 1);
     // FP64 with the reference's sequential sums: producer/consumer sweeps by default
     c->gcfg.strategy = p->strategy > 0 ? p->strategy : (p->precision == RED_PREC_FP64 && !p->fp64_split ? 3 : 1);
@@ -1125,16 +1128,28 @@ int red_counters(red_ctx* c, int out[8])
 int red_energy(red_ctx* c, double* E, double L[3])
 {
     if (!c) return RED_ERR_ARG;
-    const int nm = c->nMassive();
-    std::vector<double> o(6 * (size_t)std::max(nm, 1));
-    CTX_HIP(c, launch_energy(c->d_r[c->cur], c->d_v[c->cur], c->d_mass, c->d_id, nm, c->nSI(), c->d_energy, c->stream));
-    CTX_HIP(c, hipStreamSynchronize(c->stream));
+    const int nm = c->nMassive(), nsi = c->nSI();
+    const int nch = energy_chunks(nm);
+    const size_t np = (size_t)std::max(nsi, 0) * nch;
+    if (np > c->energy_part_cap) {
+        SetupLock lk;
+        (void)hipFree(c->d_energy_part);
+        c->d_energy_part = nullptr;
+        c->energy_part_cap = 0;
+        CTX_HIP(c, dmalloc(&c->d_energy_part, np));
+        c->energy_part_cap = np;
+    }
+    std::vector<double> o(6 * (size_t)std::max(nm, 1)), u(std::max<size_t>(np, 1));
+    CTX_HIP(c, launch_energy(c->d_r[c->cur], c->d_v[c->cur], c->d_mass, c->d_id, nm, nsi, c->d_energy,
+                             c->d_energy_part, c->stream));
     if (nm > 0) CTX_HIP(c, stream_copy(c->stream, o.data(), c->d_energy, sizeof(double) * 6 * nm, hipMemcpyDeviceToHost));
+    if (np > 0) CTX_HIP(c, stream_copy(c->stream, u.data(), c->d_energy_part, sizeof(double) * np, hipMemcpyDeviceToHost));
     double T = 0.0, U = 0.0, lx = 0.0, ly = 0.0, lz = 0.0;
     for (int i = 0; i < nm; i++) {
-        T += o[6 * i]; U += o[6 * i + 1];
+        T += o[6 * i];
         lx += o[6 * i + 2]; ly += o[6 * i + 3]; lz += o[6 * i + 4];
     }
+    for (size_t k = 0; k < np; k++) U += u[k];
     if (E) *E = T - U;
     if (L) { L[0] = lx; L[1] = ly; L[2] = lz; }
     return RED_OK;

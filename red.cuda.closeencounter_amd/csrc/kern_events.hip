@@ -9,8 +9,9 @@
 // The merge scatter applies the host-computed results of pp_disk::handle_collision_pair
 // (pp_disk.cu:1151-1202) in one launch instead of seven small copies per pair.
 //
-// The energy kernel is not in the reference: per massive body it writes
-// (1/2 m v^2, sum_{j>i, interacting} m_i m_j / r_ij, m r x v) for a fixed-order host sum.
+// The energy kernels are not in the reference: per massive body (1/2 m v^2, m r x v), and
+// the potential sum_{j>i, interacting} m_i m_j / r_ij in partials per (SI body, source
+// chunk), all added on the host in a fixed order.
 #include <algorithm>
 
 #include "red_internal.hpp"
@@ -69,33 +70,58 @@ __global__ void k_apply_updates(const BodyUpdate* __restrict__ u, int nu, d4* __
     }
 }
 
+// Per massive body: kinetic energy and angular momentum (o[1], the potential, is filled by
+// the pair kernel below).
 __global__ __launch_bounds__(256) void k_energy(const d4* __restrict__ r, const d4* __restrict__ v,
                                                 const double* __restrict__ mass, const int* __restrict__ id,
-                                                int n_massive, int n_si, double* __restrict__ out)
+                                                int n_massive, double* __restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_massive) return;
     double* o = out + 6 * (size_t)i;
-    if (id[i] <= 0) {
-        for (int c = 0; c < 6; c++) o[c] = 0.0;
-        return;
-    }
+    for (int c = 0; c < 6; c++) o[c] = 0.0;
+    if (id[i] <= 0) return;
     const d4 q = r[i], w = v[i];
     const double m = mass[i];
-    double U = 0.0;
-    const int jend = i < n_si ? n_massive : n_si;   // NSI-NSI pairs do not interact
-    for (int j = i + 1; j < jend; j++) {
-        if (id[j] <= 0) continue;
-        const d4 p = r[j];
-        const double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-        U += m * mass[j] / sqrt((dx * dx) + (dy * dy) + (dz * dz));
-    }
     o[0] = 0.5 * m * ((w.x * w.x) + (w.y * w.y) + (w.z * w.z));
-    o[1] = U;
     o[2] = m * (q.y * w.z - q.z * w.y);
     o[3] = m * (q.z * w.x - q.x * w.z);
     o[4] = m * (q.x * w.y - q.y * w.x);
-    o[5] = 0.0;
+}
+
+// Potential energy of the interacting pairs: every pair has a self-interacting body i
+// (SI x massive, NSI-NSI pairs do not interact), counted once as (i, j > i).  Block
+// (chunk, i) sums m_i m_j / r_ij over sources j of one ECH-source chunk and reduces in LDS
+// in a fixed order; the host adds the partials in (i, chunk) order.
+constexpr int ECH = 4096;
+
+__global__ __launch_bounds__(256) void k_energy_pairs(const d4* __restrict__ r, const double* __restrict__ mass,
+                                                      const int* __restrict__ id, int n_massive,
+                                                      double* __restrict__ part)
+{
+    __shared__ double red[256];
+    const int i = blockIdx.y;
+    const int nch = gridDim.x;
+    const int j0 = blockIdx.x * ECH;
+    const int j1 = min(n_massive, j0 + ECH);
+    double U = 0.0;
+    if (id[i] > 0) {
+        const d4 q = r[i];
+        const double m = mass[i];
+        for (int j = max(j0, i + 1) + threadIdx.x; j < j1; j += blockDim.x) {
+            if (id[j] <= 0) continue;
+            const d4 p = r[j];
+            const double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+            U += m * mass[j] / sqrt((dx * dx) + (dy * dy) + (dz * dz));
+        }
+    }
+    red[threadIdx.x] = U;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)i * nch + blockIdx.x] = red[0];
 }
 
 }  // namespace
@@ -116,12 +142,17 @@ hipError_t launch_apply_updates(const BodyUpdate* u, int nu, d4* r, d4* v, doubl
     return hipGetLastError();
 }
 
+int energy_chunks(int n_massive) { return (n_massive + ECH - 1) / ECH; }
+
 hipError_t launch_energy(const d4* r, const d4* v, const double* mass, const int* id, int n_massive, int n_si,
-                         double* out6, hipStream_t s)
+                         double* out6, double* part, hipStream_t s)
 {
     if (n_massive <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_energy, dim3((n_massive + 255) / 256), dim3(256), 0, s, r, v, mass, id, n_massive, n_si,
-                       out6);
+    hipLaunchKernelGGL(k_energy, dim3((n_massive + 255) / 256), dim3(256), 0, s, r, v, mass, id, n_massive, out6);
+    RED_HIP_TRY(hipGetLastError());
+    if (n_si <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_energy_pairs, dim3(energy_chunks(n_massive), n_si), dim3(256), 0, s, r, mass, id, n_massive,
+                       part);
     return hipGetLastError();
 }
 

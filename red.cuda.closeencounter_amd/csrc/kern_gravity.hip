@@ -256,27 +256,30 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
 // FP64 sweep with sequential sums, producer/consumer through LDS.  The reference adds the
 // source terms of a sink one after another in ascending j (pp_disk.cu:144-215); only those
 // additions are a serial chain -- the pair terms themselves (differences, IEEE sqrt and
-// divide) are independent.  A block of 1 + NP waves takes SQ_SG sinks: NP producer waves
-// compute the terms of the next SQ_J sources into one half of a double-buffered LDS tile
-// while the consumer wave adds the previous chunk in ascending j, one lane per (sink,
-// component) row, starting from 0.  Every term is formed and added exactly as in the fused
-// loop, so the sums are bit-identical to the reference order, while the long FP64
+// divide) are independent.  A block of 1 + NP waves takes SG sinks: NP producer waves
+// compute the terms of the next J = 1024 / SG sources into one half of a double-buffered
+// LDS tile while the consumer wave adds the previous chunk in ascending j, one lane per
+// (sink, component) row, starting from 0.  Every term is formed and added exactly as in the
+// fused loop, so the sums are bit-identical to the reference order, while the long FP64
 // sequences run on all SIMDs.  A producer lane keeps one sink in registers and takes
-// SQ_J / SLOTS sources of every chunk (independent work for the scheduler to interleave);
-// the source records of the chunk after next are requested before a chunk is computed.
-constexpr int SQ_SG = 16;             // sinks per block
-constexpr int SQ_J = 64;              // sources per chunk
-constexpr int SQ_ROWS = 3 * SQ_SG;    // consumer lanes in use
+// J / SLOTS sources of every chunk (independent work for the scheduler to interleave); the
+// source records of the chunk after next are requested before a chunk is computed.  Longer
+// chunks (fewer sinks per block) spread the per-chunk barrier and producer latency over
+// more sources; the tile stays 2 x 1024 x 3 doubles.
+constexpr int SQ_TILE = 1024;         // sinks x sources per chunk
 
-template <bool OVERLAP, bool EV, int NP>
+template <bool OVERLAP, bool EV, int NP, int SQ_SG>
 __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __restrict__ r, const d4* __restrict__ v,
                                                                  d4* __restrict__ a, const double* __restrict__ mass,
                                                                  const double* __restrict__ radius,
                                                                  const int* __restrict__ id, Sweep sw, PairTest P,
                                                                  EventSink E, Ctl* __restrict__ ctl)
 {
+    constexpr int SQ_J = SQ_TILE / SQ_SG;             // sources per chunk
+    constexpr int SQ_ROWS = 3 * SQ_SG;                // consumer lanes in use
     constexpr int SLOTS = NP * 64 / SQ_SG;            // source slots per pass over the chunk
     constexpr int M = (SQ_J + SLOTS - 1) / SLOTS;     // sources per producer lane and chunk
+    static_assert(SQ_ROWS <= 64, "one consumer wave");
     __shared__ double buf[2][SQ_J][SQ_ROWS];
     const int wv = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
@@ -394,21 +397,30 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
     }
 }
 
+template <bool OV, bool EV, int SG>
+hipError_t go_fp64_seq_sg(int np, const Sweep& sw, const GravBuffers& B, const PairTest& P, const EventSink& E,
+                          hipStream_t s)
+{
+    dim3 g((sw.sink1 - sw.sink0 + SG - 1) / SG);
+    if (np <= 3)
+        hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 3, SG>), g, dim3(256), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
+                           sw, P, E, B.ctl);
+    else if (np <= 7)
+        hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 7, SG>), g, dim3(512), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
+                           sw, P, E, B.ctl);
+    else
+        hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 15, SG>), g, dim3(1024), 0, s, B.r, B.v, B.a, B.mass, B.radius,
+                           B.id, sw, P, E, B.ctl);
+    return hipGetLastError();
+}
+
 template <bool OV, bool EV>
-hipError_t go_fp64_seq(int np, const Sweep& sw, const GravBuffers& B, const PairTest& P, const EventSink& E,
+hipError_t go_fp64_seq(int np, int sg, const Sweep& sw, const GravBuffers& B, const PairTest& P, const EventSink& E,
                        hipStream_t s)
 {
-    dim3 g((sw.sink1 - sw.sink0 + SQ_SG - 1) / SQ_SG);
-    if (np <= 3)
-        hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 3>), g, dim3(256), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id, sw,
-                           P, E, B.ctl);
-    else if (np <= 7)
-        hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 7>), g, dim3(512), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id, sw,
-                           P, E, B.ctl);
-    else
-        hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 15>), g, dim3(1024), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
-                           sw, P, E, B.ctl);
-    return hipGetLastError();
+    if (sg >= 16) return go_fp64_seq_sg<OV, EV, 16>(np, sw, B, P, E, s);
+    if (sg >= 8) return go_fp64_seq_sg<OV, EV, 8>(np, sw, B, P, E, s);
+    return go_fp64_seq_sg<OV, EV, 4>(np, sw, B, P, E, s);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -735,12 +747,14 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
     if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0) {
         const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
         const bool ev = P.enabled != 0;
-        // producer waves per block: the block size if one was asked for, else many producers
-        // when there are few blocks to spread over the chip
-        const int nblk = (nsink + SQ_SG - 1) / SQ_SG;
+        // sinks per block (chunk length 1024 / sg) and producer waves per block: as asked for,
+        // else long chunks for few sinks (the serial chain per sink is the critical path) and
+        // many producers when there are few blocks to spread over the chip
+        const int sg = C.seq_sinks > 0 ? C.seq_sinks : (nsink <= 4096 ? 4 : 8);
+        const int nblk = (nsink + sg - 1) / sg;
         const int np = C.block_seq > 0 ? C.block_seq / 64 - 1 : (nblk >= 128 ? 7 : 15);
-        if (ov) return ev ? go_fp64_seq<true, true>(np, sw, B, P, E, s) : go_fp64_seq<true, false>(np, sw, B, P, E, s);
-        return ev ? go_fp64_seq<false, true>(np, sw, B, P, E, s) : go_fp64_seq<false, false>(np, sw, B, P, E, s);
+        if (ov) return ev ? go_fp64_seq<true, true>(np, sg, sw, B, P, E, s) : go_fp64_seq<true, false>(np, sg, sw, B, P, E, s);
+        return ev ? go_fp64_seq<false, true>(np, sg, sw, B, P, E, s) : go_fp64_seq<false, false>(np, sg, sw, B, P, E, s);
     }
     const bool use_split = nsrc > 0 && nsink <= C.split_max_sinks && nsrc >= 1024 &&
                            split_scratch_bytes(sw) <= B.scratch_bytes;

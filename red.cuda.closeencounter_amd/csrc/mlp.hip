@@ -253,11 +253,13 @@ constexpr int FEAT_SRC = 64;    // sources per work item of the pair pass
 //
 // State features.  Work items, one wave each: [0, n0): (64 SI sinks of 1..n_si-1, 64-source
 // chunk of 1..n_massive-1); [n0, n0+n1): (64 NSI sinks, chunk of SI sources 1..n_si-1); then
-// one item per 64 bodies 1..n-1.  Lanes run over sinks and every source of a chunk reaches
-// the wave as one broadcast record, as in the force sweeps; waves take items grid-stride and
-// keep running extrema (exact in any order).  The pair quantity is
-// q^3 = d^3 3 m_star / ((m_i + m_j) ((h_i + h_j)/2)^3), heliocentric h: FP64 differences,
-// FP32 ratio.  Per body: |a| and |a - a_kep| / |a_kep|, a_kep = -m_star (r - r_star)/|r - r_star|^3.
+// one item per 64 bodies 1..n-1.  Lanes run over sinks; each lane stages one source of the
+// chunk and the wave broadcasts them with readlane, so the pair loop issues no memory
+// operations; waves take items grid-stride and keep running extrema (exact in any order).
+// The pair quantity is q^3 = d^3 3 m_star / ((m_i + m_j) ((h_i + h_j)/2)^3), heliocentric h,
+// from FP32 star-relative positions (the star subtracted in FP64) and the hardware square
+// root and reciprocal: a few 1e-4 relative in d at 1e-3 AU separations 5 AU out, ample for
+// a logarithmic feature.  Per body: |a| and |a - a_kep| / |a_kep|, a_kep = -m_star (r - r_star)/|r - r_star|^3.
 // Each block stores its extrema in its own slot; the last block to finish (counter) reduces
 // the slots, completes the feature vector, runs the network and applies the proposal.
 __global__ __launch_bounds__(256) void k_learn_step(const red::d4* __restrict__ r, const red::d4* __restrict__ a,
@@ -289,24 +291,36 @@ __global__ __launch_bounds__(256) void k_learn_step(const red::d4* __restrict__ 
                 const long u = w - n0;
                 i = n_si + (int)(u / ch1) * 64 + lane; i1 = n_massive; c = u % ch1; src0 = 1; src1 = n_si;
             }
+            const int j0 = src0 + (int)c * FEAT_SRC;
+            const int nj = min(src1 - j0, FEAT_SRC);
+            // lane k stages source j0 + k: star-relative FP32 position, heliocentric distance,
+            // mass (-1 when inactive); the loop below broadcasts it with readlane
+            float sx = 0.0f, sy = 0.0f, sz = 0.0f, sh = 0.0f, sm = -1.0f;
+            if (lane < nj) {
+                const red::d4 q = r[j0 + lane];
+                sx = (float)(q.x - star.x); sy = (float)(q.y - star.y); sz = (float)(q.z - star.z);
+                sh = sqrtf(sx * sx + sy * sy + sz * sz);
+                sm = id[j0 + lane] > 0 ? (float)q.w : -1.0f;
+            }
             const bool live = i < i1 && id[i] > 0;
             const red::d4 ri = r[live ? i : 0];
-            const int j0 = src0 + (int)c * FEAT_SRC;
-            const int j1 = min(src1, j0 + FEAT_SRC);
-            const double hx = ri.x - star.x, hy = ri.y - star.y, hz = ri.z - star.z;
-            const float hi = sqrtf((float)(hx * hx + hy * hy + hz * hz));
-            for (int j = j0; j < j1; ++j) {
-                const red::d4 rj = r[j];
-                const int idj = id[j];
-                const double mm = ri.w + rj.w;
-                if (!live || j == i || idj <= 0 || !(mm > 0.0)) continue;
-                const double dx = rj.x - ri.x, dy = rj.y - ri.y, dz = rj.z - ri.z;
-                const double gx = rj.x - star.x, gy = rj.y - star.y, gz = rj.z - star.z;
-                const float d2 = (float)(dx * dx + dy * dy + dz * dz);
-                const float hj = sqrtf((float)(gx * gx + gy * gy + gz * gz));
-                const float hm = 0.5f * (hi + hj);
-                const float q3 = d2 * sqrtf(d2) * (float)(3.0 * ms) / ((float)mm * hm * hm * hm);
-                best = fminf(best, q3);
+            const float xi = (float)(ri.x - star.x), yi = (float)(ri.y - star.y), zi = (float)(ri.z - star.z);
+            const float hi = sqrtf(xi * xi + yi * yi + zi * zi);
+            const float mi = (float)ri.w;
+            const float three_ms = (float)(3.0 * ms);
+            for (int k = 0; k < nj; ++k) {
+                const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sx), k));
+                const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sy), k));
+                const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sz), k));
+                const float qh = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sh), k));
+                const float qm = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sm), k));
+                const float dx = qx - xi, dy = qy - yi, dz = qz - zi;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                const float mm = qm + mi;
+                const float hm = 0.5f * (hi + qh);
+                const float q3 = d2 * __builtin_amdgcn_sqrtf(d2) * three_ms * __builtin_amdgcn_rcpf(mm * hm * hm * hm);
+                const bool ok = live && qm >= 0.0f && j0 + k != i && mm > 0.0f;
+                best = ok ? fminf(best, q3) : best;
             }
         } else {
             const int i = 1 + (int)(w - n0 - n1) * 64 + lane;

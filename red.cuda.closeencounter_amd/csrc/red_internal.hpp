@@ -100,6 +100,7 @@ struct GravConfig {
     int fp64_split;                // FP64 sums may be partitioned over sources
     int source_parts;              // forced part count, 0 = automatic
     int block_seq;                 // producer/consumer block size (256, 512, 1024), 0 = automatic
+    int seq_sinks;                 // producer/consumer sinks per block (4, 8, 16), 0 = automatic
 };
 
 // Number of source parts a sweep is cut into.
@@ -165,9 +166,12 @@ struct BodyUpdate {
 hipError_t launch_apply_updates(const BodyUpdate* u, int nu, d4* r, d4* v, double* mass, double* radius,
                                 int* id, hipStream_t s);
 
-// Energy / angular momentum partials per massive body.
+// Energy / angular momentum partials: per massive body (1/2 m v^2, 0, m r x v, 0) into out6,
+// and the potential of the interacting pairs per (SI body, source chunk) into part
+// (n_si x energy_chunks(n_massive) values, to be added in that order).
+int energy_chunks(int n_massive);
 hipError_t launch_energy(const d4* r, const d4* v, const double* mass, const int* id, int n_massive, int n_si,
-                         double* out6, hipStream_t s);
+                         double* out6, double* part, hipStream_t s);
 
 // ---- learned step-size proposer ----
 constexpr int MLP_NF = RED_MLP_FEATURES, MLP_NH = 64;

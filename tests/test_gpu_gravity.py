@@ -35,7 +35,8 @@ def _events_equal(ev_o, ev_g):
 
 
 @pytest.mark.parametrize("spt,strategy,block", [(1, 1, 256), (2, 1, 256), (4, 1, 128), (1, 2, 256), (2, 2, 64),
-                                                (1, 3, 256)])
+                                                (1, 3, 256), (4, 3, 512), (8, 3, 1024), (16, 3, 256),
+                                                (16, 3, 1024)])
 def test_fp64_allsi_bit_exact(oracle_mod, native_mod, spt, strategy, block):
     print(f"case allsi n=3001 spt={spt} strategy={strategy} block={block}", flush=True)
     b = generate.to_internal(generate.disk_allsi(3001, seed=11))

