@@ -185,9 +185,13 @@ void refresh_pair_test(red_ctx* c)
     const double ms = mass_of_star(c);
     P.three_mstar = 3.0 * (ms > 0 ? ms : 1.0);
     // the mixed sweep screens only sources past the FP64 head (star, giants)
-    double rmax = 0.0;
-    for (int i = std::max(0, c->cnt[0] + c->cnt[1]); i < c->n; i++) rmax = std::max(rmax, c->h_radius[i]);
+    double rmax = 0.0, mmax = 0.0;
+    for (int i = std::max(0, c->cnt[0] + c->cnt[1]); i < c->n; i++) {
+        rmax = std::max(rmax, c->h_radius[i]);
+        mmax = std::max(mmax, c->h_mass[i]);
+    }
     P.rmax = rmax;
+    P.mmax = mmax;
     P.close_encounter = c->prm.close_encounter;
     P.enabled = c->prm.inner_steps && (P.radii_f > 0.0 || P.hill_f > 0.0);
 }
@@ -508,7 +512,6 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     if (p->integrator < RED_EULER || p->integrator > RED_RKF78) { delete c; return RED_ERR_UNSUPPORTED; }
     if (p->frame_center != RED_FRAME_BARY) { delete c; return RED_ERR_UNSUPPORTED; }
     if (!p->inner_steps && (p->radii_enhance_factor > 0.0 || p->encounter_hill_factor > 0.0)) { delete c; return RED_ERR_UNSUPPORTED; }
-    if (p->precision == RED_PREC_MIXED && p->encounter_hill_factor > 0.0) { delete c; return RED_ERR_UNSUPPORTED; }
     if (n > 0 && (!b->id || !b->type || !b->mass || !b->radius || !b->density || !b->cd || !b->r || !b->v)) {
         delete c;
         return RED_ERR_ARG;

@@ -463,8 +463,16 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
         cx[s] = cy[s] = cz[s] = 0.0;
         if (EV) {
             // conservative screen: FP32 r^2 is within ~5e-7 relative of the FP64 value
-            const double cut = P.radii_f * (Ri[s] + P.rmax) * 1.001;
-            cut2[s] = (float)(cut * cut);
+            double cut = P.radii_f * (Ri[s] + P.rmax);
+            if (P.hill_f > 0.0) {
+                // d < f c (|r_i| + |r_j|)/2 with c = ((m_i + m_j)/(3 M_star))^(1/3) and
+                // |r_j| <= |r_i| + d implies d < f c |r_i| / (1 - f c / 2), c taken at m_max
+                const double fc = P.hill_f * cbrt((q.w + P.mmax) / P.three_mstar);
+                const double hr = sqrt((q.x * q.x) + (q.y * q.y) + (q.z * q.z));
+                cut = fc < 1.0 ? fmax(cut, fc * hr / (1.0 - 0.5 * fc)) : 1e18;
+            }
+            cut *= 1.001;
+            cut2[s] = (float)fmin(cut * cut, 3.0e38);
         }
     }
     // FP64 head (the star and giant planets), processed by part 0

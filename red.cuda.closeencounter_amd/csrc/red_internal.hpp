@@ -51,6 +51,7 @@ struct PairTest {
     double hill_f;                 // close encounter when d < hill_f * R_Hill
     double three_mstar;            // 3 M_star, for R_Hill
     double rmax;                   // largest body radius (mixed-precision screen)
+    double mmax;                   // largest source mass (mixed-precision Hill screen)
     int close_encounter;           // 1: report CLOSE_ENCOUNTER, keep (i, j) order
     int enabled;                   // any pair test active
 };
