@@ -83,7 +83,9 @@ typedef struct red_params {
     int    error_check_for_tp;    /* include test particles in the error norm       */
     int    close_encounter;       /* report pair events instead of merging          */
     int    inner_steps;           /* detect pair events inside the force sweep      */
-    int    frame_center;          /* RED_FRAME_BARY (RED_FRAME_ASTRO: not supported) */
+    int    frame_center;          /* RED_FRAME_BARY, or RED_FRAME_ASTRO: star = body 0
+                                     at the origin, FP64, self-interacting bodies only
+                                     (pp_disk.cu:409-454, number_of_bodies.cpp:205-209) */
     double hit_centrum;           /* AU, 0 = off                                    */
     double ejection;              /* AU, 0 = off                                    */
     double radii_enhance_factor;  /* collision when d < f (R_i + R_j), 0 = off      */
@@ -202,6 +204,9 @@ int red_energy(red_ctx* ctx, double* E, double L[3]);
 /* Host-side transforms (tools.cpp:122-178, pp_disk.cu:1512-1551): barycentric shift
  * (sums run from the last body down, as the reference), and t*k / v/k scaling. */
 int red_host_transform_to_bc(int n, const double* mass, double* r, double* v);
+/* Astro-centric shift (tools.cpp:180-236): subtracts body 0's position and velocity from
+ * every body, from the last down; RED_ERR_NO_STAR unless body 0 has type 0 (STAR). */
+int red_host_transform_to_ac(int n, const int* type, double* r, double* v);
 int red_host_transform_velocity(int n, double* v);
 
 /* Kernel timing (not in the reference, which has none beyond nvprof): with profiling on,

@@ -194,6 +194,8 @@ void refresh_pair_test(red_ctx* c)
     P.mmax = mmax;
     P.close_encounter = c->prm.close_encounter;
     P.enabled = c->prm.inner_steps && (P.radii_f > 0.0 || P.hill_f > 0.0);
+    // the astro-centric start term uses the star's current mass (it grows by hit-centrum merges)
+    if (c->gcfg.astro) c->gcfg.astro_mstar = c->h_mass[0];
 }
 
 // Stage state records: stage 0 = current state, stage k >= 1 = S_k, RK4's stage 4 = y_(n+1).
@@ -241,7 +243,8 @@ hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const
     if (c->gcfg.precision == RED_PREC_MIXED)
         RED_HIP_TRY(timed(c, RED_PROF_PACK, [&] { return launch_pack_mixed(r, c->d_id, c->d_src_mixed, c->nMassive(), c->stream); }));
     const int nsi = c->nSI(), nm = c->nMassive(), n = c->n;
-    const Sweep sw[3] = {{0, nsi, 0, nm}, {nsi, nm, 0, nsi}, {nm, n, 0, nm}};
+    const int s0 = c->gcfg.astro ? 1 : 0;   // the astro-centric SI sweep skips the star (number_of_bodies.cpp:205-209)
+    const Sweep sw[3] = {{s0, nsi, s0, nm}, {nsi, nm, 0, nsi}, {nm, n, 0, nm}};
     for (int k = 0; k < 3; k++) {
         if (sw[k].sink1 <= sw[k].sink0) continue;
         RED_HIP_TRY(timed(c, RED_PROF_SI + k, [&] { return launch_sweep(sw[k], B, c->gcfg, c->ptest, E, c->stream); }));
@@ -510,7 +513,7 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
         n += b->counts[k];
     }
     if (p->integrator < RED_EULER || p->integrator > RED_RKF78) { delete c; return RED_ERR_UNSUPPORTED; }
-    if (p->frame_center != RED_FRAME_BARY) { delete c; return RED_ERR_UNSUPPORTED; }
+    if (p->frame_center != RED_FRAME_BARY && p->frame_center != RED_FRAME_ASTRO) { delete c; return RED_ERR_ARG; }
     if (!p->inner_steps && (p->radii_enhance_factor > 0.0 || p->encounter_hill_factor > 0.0)) { delete c; return RED_ERR_UNSUPPORTED; }
     if (n > 0 && (!b->id || !b->type || !b->mass || !b->radius || !b->density || !b->cd || !b->r || !b->v)) {
         delete c;
@@ -549,6 +552,19 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     c->gcfg.kahan_pairs = p->kahan_pairs;
     c->gcfg.split_max_sinks = 2048;
     c->gcfg.fp64_split = p->fp64_split;
+    if (p->frame_center == RED_FRAME_ASTRO) {
+        // The reference's NSI and NI sweeps run the SI code with bounds that keep the star as
+        // source 0 at the origin (number_of_bodies.cpp:227-257, pp_disk.cu:574-583): its
+        // indirect term is 0/0 there, so only self-interacting disks are defined.
+        if (c->nNSI() > 0 || c->nNI() > 0 || p->precision != RED_PREC_FP64) {
+            int rc = c->fail(RED_ERR_UNSUPPORTED, "frame_center = astro needs FP64 and self-interacting bodies only");
+            *out = c;
+            return rc;
+        }
+        // the star is body 0 and rests at the origin (tools.cpp:180-199 tests body 0's type)
+        if (n == 0 || c->h_type[0] != BT_STAR) { delete c; return RED_ERR_NO_STAR; }
+        c->gcfg.astro = 1;
+    }
     c->gcfg.source_parts = p->source_parts;
     refresh_pair_test(c);
 
@@ -1222,6 +1238,17 @@ int red_copy_to_host(red_ctx* c, void* host, const void* dev, unsigned long long
     if (!c || (bytes && (!dev || !host))) return RED_ERR_ARG;
     CTX_HIP(c, hipStreamSynchronize(c->stream));
     CTX_HIP(c, stream_copy(c->stream, host, dev, bytes, hipMemcpyDeviceToHost));
+    return RED_OK;
+}
+
+int red_host_transform_to_ac(int n, const int* type, double* r, double* v)
+{
+    if (n < 0 || (n > 0 && (!type || !r || !v))) return RED_ERR_ARG;
+    if (n == 0) return RED_OK;
+    if (type[0] != 0) return RED_ERR_NO_STAR;   // the star leads the body list
+    const double R0[3] = {r[0], r[1], r[2]}, V0[3] = {v[0], v[1], v[2]};
+    for (int j = n - 1; j >= 0; j--)
+        for (int k = 0; k < 3; k++) { r[3 * j + k] -= R0[k]; v[3 * j + k] -= V0[k]; }
     return RED_OK;
 }
 

@@ -330,7 +330,8 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     if (!out || !disks || !p || n_disks <= 0) return RED_ERR_ARG;
     *out = nullptr;
     SetupLock lk;
-    if (p->integrator < RED_EULER || p->integrator > RED_RKF78 || p->precision != RED_PREC_FP64)
+    if (p->integrator < RED_EULER || p->integrator > RED_RKF78 || p->precision != RED_PREC_FP64 ||
+        p->frame_center != RED_FRAME_BARY)
         return RED_ERR_UNSUPPORTED;
     red_ensemble* E = new red_ensemble();
     E->prm = *p;

@@ -424,6 +424,56 @@ hipError_t go_fp64_seq(int np, int sg, const Sweep& sw, const GravBuffers& B, co
 }
 
 // ---------------------------------------------------------------------------------------
+// Astro-centric frame (the reference's CPU path, pp_disk.cu:409-454; its GPU kernel has no
+// such term).  The star sits at the origin and is neither sink nor source of the SI sweep
+// (number_of_bodies.cpp:205-209, 218-221); every sink starts from the star's pull
+// -(m_star + m_i) r_i / |r_i|^3 and each active source j != i adds its direct term, then
+// subtracts its indirect term m_j r_j / |r_j|^3, in the reference's operation order.  The
+// star's own row (below sink0) is written as zero.
+template <bool EV>
+__global__ __launch_bounds__(256) void k_grav_fp64_astro(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                         d4* __restrict__ a, const double* __restrict__ mass,
+                                                         const double* __restrict__ radius,
+                                                         const int* __restrict__ id, Sweep sw, double mstar,
+                                                         PairTest P, EventSink E, Ctl* __restrict__ ctl)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sw.sink1) return;
+    if (i < sw.sink0) {   // the star rests at the origin of this frame
+        a[i] = d4{0.0, 0.0, 0.0, 0.0};
+        return;
+    }
+    const d4 qi = r[i];
+    const double mu = mstar + mass[i];
+    const double r2 = (qi.x * qi.x) + (qi.y * qi.y) + (qi.z * qi.z);
+    double ax = -mu / (r2 * sqrt(r2)) * qi.x;
+    double ay = -mu / (r2 * sqrt(r2)) * qi.y;
+    double az = -mu / (r2 * sqrt(r2)) * qi.z;
+    if (id[i] > 0) {
+        const double Ri = EV ? radius[i] : 0.0;
+        for (int j = sw.src0; j < sw.src1; ++j) {
+            if (i == j || id[j] < 0) continue;
+            const d4 q = r[j];
+            const double dx = q.x - qi.x;
+            const double dy = q.y - qi.y;
+            const double dz = q.z - qi.z;
+            const double dr2 = (dx * dx) + (dy * dy) + (dz * dz);
+            const double d = sqrt(dr2);
+            const double sc = mass[j] / (d * dr2);
+            ax += sc * dx;
+            ay += sc * dy;
+            az += sc * dz;
+            const double rj2 = (q.x * q.x) + (q.y * q.y) + (q.z * q.z);
+            ax -= mass[j] / (rj2 * sqrt(rj2)) * q.x;
+            ay -= mass[j] / (rj2 * sqrt(rj2)) * q.y;
+            az -= mass[j] / (rj2 * sqrt(rj2)) * q.z;
+            if (EV) pair_test(P, E, ctl, i, j, d, Ri, radius[j], r, v, mass, id);
+        }
+    }
+    a[i] = d4{ax, ay, az, 0.0};
+}
+
+// ---------------------------------------------------------------------------------------
 // Mixed-precision broadcast sweep.
 __device__ __forceinline__ void kahan_add(double& sum, double& comp, double x)
 {
@@ -752,6 +802,16 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
     const int nsink = sw.sink1 - sw.sink0;
     const int nsrc = sw.src1 - sw.src0;
     if (nsink <= 0) return hipSuccess;
+    if (C.astro) {
+        dim3 g((sw.sink1 + 255) / 256), b(256);   // also clears the star's row below sink0
+        if (P.enabled)
+            hipLaunchKernelGGL(k_grav_fp64_astro<true>, g, b, 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id, sw,
+                               C.astro_mstar, P, E, B.ctl);
+        else
+            hipLaunchKernelGGL(k_grav_fp64_astro<false>, g, b, 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id, sw,
+                               C.astro_mstar, P, E, B.ctl);
+        return hipGetLastError();
+    }
     if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0) {
         const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
         const bool ev = P.enabled != 0;

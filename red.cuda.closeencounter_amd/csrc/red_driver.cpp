@@ -430,7 +430,8 @@ int main(int argc, const char** argv, const char** env)
         if (n > 0 && B.r[0] == 0.0 && B.r[1] == 0.0 && B.r[2] == 0.0 && B.v[0] == 0.0 && B.v[1] == 0.0 && B.v[2] == 0.0) {
             if (P.frame_center == RED_FRAME_BARY) red_host_transform_to_bc(n, B.mass.data(), B.r.data(), B.v.data());
         } else if (P.frame_center == RED_FRAME_ASTRO) {
-            throw Error{"frame_center = astro is not supported on the GPU path"};
+            if (red_host_transform_to_ac(n, B.type.data(), B.r.data(), B.v.data()) != RED_OK)
+                throw Error{"No star is included!"};
         }
         for (auto& e : B.epoch) e *= GAUSS;
         red_host_transform_velocity(n, B.v.data());

@@ -102,6 +102,8 @@ struct GravConfig {
     int source_parts;              // forced part count, 0 = automatic
     int block_seq;                 // producer/consumer block size (256, 512, 1024), 0 = automatic
     int seq_sinks;                 // producer/consumer sinks per block (4, 8, 16), 0 = automatic
+    int astro;                     // astro-centric frame (FP64, self-interacting bodies only)
+    double astro_mstar;            // the star's mass for the astro-centric start term
 };
 
 // Number of source parts a sweep is cut into.

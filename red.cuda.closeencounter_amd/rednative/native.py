@@ -102,6 +102,7 @@ SIGNATURES = {
     "red_counters": (_I, [_P, _IP]),
     "red_energy": (_I, [_P, _DP, _DP]),
     "red_host_transform_to_bc": (_I, [_I, _P, _P, _P]),
+    "red_host_transform_to_ac": (_I, [_I, _P, _P, _P]),
     "red_host_transform_velocity": (_I, [_I, _P]),
     "red_set_profiling": (_I, [_P, _I]),
     "red_get_profile": (_I, [_P, _DP, _IP, _I]),
@@ -172,6 +173,15 @@ def event_dict(e: RedEvent) -> dict:
         if not isinstance(v, (int, float)):
             d[k] = tuple(v)
     return d
+
+
+def host_transform_to_ac(btype: np.ndarray, r: np.ndarray, v: np.ndarray) -> None:
+    """Astro-centric shift in place (tools.cpp:209-236); body 0 must be the star."""
+    L = load_library()
+    t = np.ascontiguousarray(btype, np.int32)
+    rc = L.red_host_transform_to_ac(len(t), _ptr(t), _ptr(r), _ptr(v))
+    if rc:
+        raise RedError(rc, "red_host_transform_to_ac")
 
 
 def host_transform_to_bc(mass: np.ndarray, r: np.ndarray, v: np.ndarray) -> None:

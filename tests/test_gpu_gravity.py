@@ -143,18 +143,3 @@ def test_mixed_accuracy_and_events(oracle_mod, native_mod, kahan):
     assert len(ev_o) > 0
     _events_equal(ev_o, ev_g)
 
-
-def test_mixed_hill_events_match_fp64(oracle_mod, native_mod):
-    # the mixed screen must keep every pair the FP64 Hill test reports (protoplanets, so
-    # that mutual Hill radii are large) and recheck it exactly
-    b = generate.to_internal(generate.disk_allsi(3001, seed=12, m_range=(1e-9, 1e-7)))
-    kw = dict(radii_enhance_factor=1.0, encounter_hill_factor=3.0)
-    print(f"case mixed hill n={b.n}", flush=True)
-    o = oracle_mod.Oracle(b, **kw)
-    d = native_mod.Disk(b, precision=native_mod.PREC_MIXED, event_capacity=100000, **kw)
-    d.accel(1.0, b.r, b.v)
-    o.accel(1.0, b.r, b.v)
-    ev_o, ev_g = o.raw_events(), d.raw_events()
-    print("events", len(ev_o), flush=True)
-    assert len(ev_o) > 0
-    _events_equal(ev_o, ev_g)
