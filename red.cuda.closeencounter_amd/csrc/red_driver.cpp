@@ -8,7 +8,14 @@
 //
 // Additions: two optional parameter keys, `dt` (initial step in days; the reference
 // hard-codes 0.001 in main.cu:141) and `encounter_hill_factor` (close encounters at that
-// many mutual Hill radii), and the `-mixed` switch (mixed-precision force sweeps).
+// many mutual Hill radii), the `-mixed` switch (mixed-precision force sweeps) and
+// `-restart <result file>`: the run resumes from the last snapshot of a text result file.
+// The reference has no explicit resume; its text snapshot has the body file's 16 columns
+// with t in the epoch column (pp_disk.cu:1707-1722 against 1576-1612), so "last snapshot
+// + counts line + start_time" is its implicit restart, which -restart performs: bodies and
+// t from the snapshot, the stop time of the parameter file's schedule kept.  The step
+// controller's state is not in the snapshot (nor in the reference's): the first step is
+// the `dt` parameter.
 // `-cpu` is refused: this build has one execution path, the GPU one.
 #include <algorithm>
 #include <cctype>
@@ -71,7 +78,7 @@ struct Options {
     int id_dev = 0;
     bool gpu = true, verbose = false, ups = false, ef = false, mixed = false;
     int n_tpb = 64;
-    std::string input_dir, printout_dir, param_file, gasdisk_file, body_file;
+    std::string input_dir, printout_dir, param_file, gasdisk_file, body_file, restart;
     std::string info = "info", event = "event", log = "log", result = "result";
 };
 
@@ -92,6 +99,7 @@ void usage()
                  "     -v      | --verbose                      : verbose mode\n"
                  "     -ef                                      : prefix output names with the run configuration\n"
                  "     -mixed                                   : mixed-precision force sweeps\n"
+                 "     -restart <filename>                      : resume from the last snapshot of a text result file\n"
                  "     -gpu    | --gpu                          : execute on the GPU (the only path of this build)\n"
                  "     -h      | --help                         : print this help\n";
 }
@@ -138,6 +146,8 @@ Options parse_options(int argc, const char** argv)
             o.ef = true;
         } else if (p == "-mixed") {
             o.mixed = true;
+        } else if (p == "-restart") {
+            o.restart = next(i, p);
         } else if (p == "--cpu" || p == "-cpu") {
             throw Error{"-cpu: this build integrates on the GPU only"};
         } else if (p == "--gpu" || p == "-gpu") {
@@ -150,7 +160,7 @@ Options parse_options(int argc, const char** argv)
         }
     }
     if (o.param_file.empty()) throw Error{"Missing filename for -p | --parameter!"};
-    if (o.body_file.empty()) throw Error{"Missing filename for -ic | --initial_conditions!"};
+    if (o.body_file.empty() && o.restart.empty()) throw Error{"Missing filename for -ic | --initial_conditions!"};
     return o;
 }
 
@@ -269,6 +279,51 @@ Bodies load_bodies(const std::string& path)
         if (name.length() > 30) name = name.substr(0, 30);
         B.names[k] = name;
     }
+    return B;
+}
+
+// Last snapshot of a text result file (pp_disk.cu:1707-1722): the trailing lines that share
+// the time column.  Returns the bodies (counts from the body types, which must be in the
+// body file's type order) and the snapshot time in days.
+Bodies load_restart(const std::string& path, double* t_day)
+{
+    std::ifstream in(path.c_str());
+    if (!in) throw Error{"Cannot open " + path + "."};
+    std::vector<std::vector<std::string>> rows;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::vector<std::string> tok;
+        std::string w;
+        while (ls >> w) tok.push_back(w);
+        if (tok.empty()) continue;
+        if (tok.size() != 16) throw Error{"Invalid result line in " + path + ": " + line};
+        rows.push_back(tok);
+    }
+    if (rows.empty()) throw Error{"No snapshot in " + path + "."};
+    const double t = atof(rows.back()[3].c_str());
+    size_t first = rows.size();
+    while (first > 0 && atof(rows[first - 1][3].c_str()) == t) first--;
+    Bodies B;
+    int last_type = 0;
+    for (size_t k = first; k < rows.size(); k++) {
+        const auto& x = rows[k];
+        auto f = [&](int c) { return atof(x[c].c_str()); };
+        const int type = atoi(x[2].c_str());
+        if (type < 0 || type > 6 || type < last_type) throw Error{"Snapshot bodies out of type order in " + path + "."};
+        last_type = type;
+        B.counts[type]++;
+        B.id.push_back(atoi(x[0].c_str()));
+        B.names.push_back(x[1].substr(0, 30));
+        B.type.push_back(type);
+        B.epoch.push_back(f(3));
+        B.mass.push_back(f(4)); B.radius.push_back(f(5)); B.density.push_back(f(6)); B.cd.push_back(f(7));
+        B.mig_type.push_back(atoi(x[8].c_str()));
+        B.mig_stop.push_back(f(9));
+        for (int c = 0; c < 3; c++) B.r.push_back(f(10 + c));
+        for (int c = 0; c < 3; c++) B.v.push_back(f(13 + c));
+    }
+    *t_day = t;
     return B;
 }
 
@@ -423,7 +478,10 @@ int main(int argc, const char** argv, const char** env)
     try {
         Options opt = parse_options(argc, argv);
         Params P = load_parameters(combine_path(opt.input_dir, opt.param_file), opt.verbose);
-        Bodies B = load_bodies(combine_path(opt.input_dir, opt.body_file));
+        double restart_day = 0.0;
+        Bodies B = opt.restart.empty() ? load_bodies(combine_path(opt.input_dir, opt.body_file))
+                                       : load_restart(combine_path(opt.input_dir, opt.restart), &restart_day);
+        if (!opt.restart.empty()) P.start_time = restart_day * GAUSS;   // stop_time keeps the schedule
         const int n = B.n();
         // options::create_pp_disk (options.cpp:197-235): barycentric frame when the star rests
         // at the origin, then epochs * k and v / k

@@ -20,7 +20,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import Dict, List, Optional
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
@@ -224,6 +224,38 @@ def read_parameters(path: str) -> Dict:
 def parse_result_ascii(path: str) -> List[List[str]]:
     with open(path) as f:
         return [ln.split() for ln in f if ln.strip()]
+
+
+def read_result_snapshot(path: str) -> Tuple[float, BodySet]:
+    """Last snapshot of a text result file (the trailing lines sharing the time column) as a
+    body list plus its time in days: the reference's implicit restart, since its snapshot has
+    the body file's 16 columns with t in the epoch column (pp_disk.cu:1707-1722 against
+    1576-1612).  Counts follow the body types, which must be in type order."""
+    rows = [r for r in parse_result_ascii(path)]
+    if not rows:
+        raise ValueError(f"No snapshot in {path}.")
+    for r in rows:
+        if len(r) != 16:
+            raise ValueError(f"Invalid result line in {path}: {' '.join(r)}")
+    t = float(rows[-1][3])
+    k = len(rows)
+    while k > 0 and float(rows[k - 1][3]) == t:
+        k -= 1
+    rec = rows[k:]
+    types = [int(x[2]) for x in rec]
+    if any(b < a for a, b in zip(types, types[1:])) or min(types) < 0 or max(types) > 6:
+        raise ValueError(f"Snapshot bodies out of type order in {path}.")
+    counts = [types.count(c) for c in range(7)]
+    f64 = lambda col: np.array([float(x[col]) for x in rec], dtype=np.float64)
+    i32 = lambda col: np.array([int(x[col]) for x in rec], dtype=np.int32)
+    return t, BodySet(
+        counts=counts,
+        ids=i32(0), names=[x[1][:30] for x in rec], types=i32(2), epoch=f64(3),
+        mass=f64(4), radius=f64(5), density=f64(6), cd=f64(7),
+        mig_type=i32(8), mig_stop_at=f64(9),
+        r=np.stack([f64(10), f64(11), f64(12)], axis=1),
+        v=np.stack([f64(13), f64(14), f64(15)], axis=1),
+    )
 
 
 def format_result_line(id_, name, btype, t_day, p4, mig_type, mig_stop, r, v_day) -> str:

@@ -219,3 +219,67 @@ def to_internal(b: BodySet, transform_bc: Optional[bool] = None) -> BodySet:
     c.epoch = c.epoch * GAUSS
     c.v = c.v / GAUSS
     return c
+
+
+# ---- command line: body and parameter files for the benchmark workloads -------------------
+# Each workload is written in the reference's input layout, so that the drop-in driver runs
+# it as `red -iDir <dir> -p parameters.txt -ic disk.txt` (add -mixed for the mixed disk).
+# Collisions merge at the physical radii (collision_factor 1), close encounters are reported
+# at 3 mutual Hill radii, bodies beyond 100 AU are ejected.
+_RULES = "collision_factor = 1\nencounter_hill_factor = 3\nejection = 100\n"
+WORKLOADS = {
+    # 256 bodies, fixed-step RK4, dt = 1 day, 10000 steps (the loop steps while t <= stop)
+    "small": dict(make=lambda seed: disk_small(seed=seed), seed=101,
+                  params="integrator = rk4\ndt = 1\nlength = {!r}\noutput_interval = 1\n".format(9999.5 / 365.25)),
+    # 16384 bodies, RKF7(8) at tolerance 1e-12, 1000 yr
+    "powerlaw": dict(make=lambda seed: disk_powerlaw(seed=seed), seed=202,
+                     params="integrator = rkf8\ntolerance = 1e-12\nlength = 1000\noutput_interval = 10\n"),
+    # 131072 bodies (65536 planetesimals, 65533 test particles), RK5(4) at 1e-10, 1000 yr
+    "mixed": dict(make=lambda seed: disk_mixed(seed=seed), seed=303,
+                  params="integrator = rk5\ntolerance = 1e-10\nlength = 1000\noutput_interval = 10\n"),
+}
+
+
+def write_workload(name: str, out_dir: str, seed: Optional[int] = None, n_disks: int = 512) -> List[str]:
+    """Writes the body file(s) and parameters.txt of a workload; returns the paths written.
+    'ensemble' writes disk_000.txt ... (512 disks, RKF7(8) at 1e-11, 100 yr each)."""
+    import os
+    from .files import write_body_file
+    os.makedirs(out_dir, exist_ok=True)
+    written = []
+    if name == "ensemble":
+        disks = ensemble(n_disks=n_disks, seed=404 if seed is None else seed)
+        for k, b in enumerate(disks):
+            p = os.path.join(out_dir, f"disk_{k:03d}.txt")
+            write_body_file(p, b)
+            written.append(p)
+        params = "integrator = rkf8\ntolerance = 1e-11\nlength = 100\noutput_interval = 10\n"
+    else:
+        w = WORKLOADS[name]
+        p = os.path.join(out_dir, "disk.txt")
+        write_body_file(p, w["make"](w["seed"] if seed is None else seed))
+        written.append(p)
+        params = w["params"]
+    p = os.path.join(out_dir, "parameters.txt")
+    with open(p, "w") as f:
+        f.write(f"name = {name}\n" + params + _RULES)
+    written.append(p)
+    return written
+
+
+def main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(description="Writes a benchmark disk in the reference's body-file layout.")
+    ap.add_argument("workload", choices=sorted(list(WORKLOADS) + ["ensemble"]))
+    ap.add_argument("--out", required=True, help="output directory")
+    ap.add_argument("--seed", type=int, default=None, help="seed (default: the workload's own)")
+    ap.add_argument("--disks", type=int, default=512, help="number of disks of the ensemble")
+    a = ap.parse_args(argv)
+    for p in write_workload(a.workload, a.out, a.seed, a.disks):
+        print(p)
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

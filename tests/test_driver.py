@@ -154,3 +154,75 @@ def test_driver_astro_frame_matches_python_loop(tmp_path, native_mod):
     for row, i, x in zip(last, ids, rr):
         assert int(row[0]) == i
         assert [float(v) for v in row[10:13]] == [float(f"{c:.16e}") for c in x]
+
+
+def _write_snapshots(path, snaps):
+    lines = []
+    for t_day, b in snaps:
+        for k in range(b.n):
+            lines.append(files.format_result_line(b.ids[k], b.names[k], b.types[k], t_day, b.params4()[k],
+                                                  b.mig_type[k], b.mig_stop_at[k], b.r[k], b.v[k]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def test_result_snapshot_reader(tmp_path):
+    b0 = generate.disk_small(20, seed=4)
+    b1 = generate.disk_small(12, seed=5)
+    _write_snapshots(tmp_path / "r.txt", [(0.0, b0), (36.5, b1)])
+    t, b = files.read_result_snapshot(str(tmp_path / "r.txt"))
+    assert t == 36.5 and b.n == b1.n and list(b.counts) == list(b1.counts)
+    assert np.array_equal(b.ids, b1.ids) and np.array_equal(b.r, b1.r) and np.array_equal(b.v, b1.v)
+    assert np.array_equal(b.mass, b1.mass)
+    bad = generate.disk_small(12, seed=5)
+    bad.types = bad.types[::-1].copy()
+    _write_snapshots(tmp_path / "bad.txt", [(1.0, bad)])
+    with pytest.raises(ValueError, match="type order"):
+        files.read_result_snapshot(str(tmp_path / "bad.txt"))
+
+
+def test_restart_errors(tmp_path):
+    (tmp_path / "p.txt").write_text("integrator = rk4\nlength = 0.1\n")
+    r = run_red(["-iDir", str(tmp_path), "-p", "p.txt", "-restart", "nothere.txt"])
+    assert r.returncode != 0 and "Cannot open" in r.stderr
+    (tmp_path / "r.txt").write_text("1 star 0 0.0 1.0\n")
+    r = run_red(["-iDir", str(tmp_path), "-p", "p.txt", "-restart", "r.txt"])
+    assert r.returncode != 0 and "Invalid result line" in r.stderr
+    bad = generate.disk_small(12, seed=5)
+    bad.types = bad.types[::-1].copy()
+    _write_snapshots(tmp_path / "bad.txt", [(1.0, bad)])
+    r = run_red(["-iDir", str(tmp_path), "-p", "p.txt", "-restart", "bad.txt"])
+    assert r.returncode != 0 and "type order" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_restart_from_snapshot(tmp_path, native_mod):
+    """A run stopped at 0.1 yr and resumed with -restart from its last text snapshot ends
+    where the uninterrupted 0.2 yr run ends.  Fixed-step RK4, so the step sequence does not
+    depend on the controller state the snapshot lacks; the snapshot's v * k and t / k
+    round trips leave differences of a few ulps, hence the tolerance."""
+    b = generate.disk_allsi(100, seed=23, m_range=(1e-10, 1e-8))
+    for sub in ("full", "half"):
+        (tmp_path / sub).mkdir()
+        files.write_body_file(str(tmp_path / sub / "disk.txt"), b)
+    prm = "integrator = rk4\noutput_interval = 0.1\ndt = 0.5\nlength = {}\n"
+    (tmp_path / "full" / "p.txt").write_text(prm.format(0.2))
+    (tmp_path / "half" / "p1.txt").write_text(prm.format(0.1))
+    (tmp_path / "half" / "p.txt").write_text(prm.format(0.2))
+    print("case driver restart", flush=True)
+    r = run_red(["-iDir", str(tmp_path / "full"), "-p", "p.txt", "-ic", "disk.txt"])
+    assert r.returncode == 0, r.stderr
+    r = run_red(["-iDir", str(tmp_path / "half"), "-p", "p1.txt", "-ic", "disk.txt"])
+    assert r.returncode == 0, r.stderr
+    t_mid, _ = files.read_result_snapshot(str(tmp_path / "half" / "result.txt"))
+    assert t_mid > 36.0
+    r = run_red(["-iDir", str(tmp_path / "half"), "-p", "p.txt", "-restart", "result.txt", "-result", "resumed"])
+    assert r.returncode == 0, r.stderr
+    t_full, full = files.read_result_snapshot(str(tmp_path / "full" / "result.txt"))
+    t_res, res = files.read_result_snapshot(str(tmp_path / "half" / "resumed.txt"))
+    first = files.parse_result_ascii(str(tmp_path / "half" / "resumed.txt"))[0]
+    assert float(first[3]) == t_mid                     # resumed at the snapshot's time
+    assert abs(t_res - t_full) <= 1e-12 * t_full, (t_res, t_full)
+    assert np.array_equal(res.ids, full.ids)
+    assert np.allclose(res.r, full.r, rtol=0, atol=1e-12)
+    assert np.allclose(res.v, full.v, rtol=0, atol=1e-13)
