@@ -362,8 +362,10 @@ struct Disk {
             if (!inner_steps || !(i > 0 && i > j)) continue;
             bool collide = d < thr[TH_RADII] * (p[i].radius + p[j].radius);
             bool hill = false;
-            if (!collide && hill_factor > 0.0) {
+            if (!collide && hill_factor > 0.0 && j >= n_s) {
                 // NEW: mutual Hill radius R_H = ((m_i+m_j)/(3 M_star))^(1/3) * (|r_i|+|r_j|)/2
+                // between two bodies orbiting the star (stars lead the list; a body is always
+                // within a few such radii of its star, so star pairs are not tested)
                 double ri = sqrt((r[i].x * r[i].x) + (r[i].y * r[i].y) + (r[i].z * r[i].z));
                 double rj = sqrt((r[j].x * r[j].x) + (r[j].y * r[j].y) + (r[j].z * r[j].z));
                 double rh = cbrt_ieee((p[i].mass + p[j].mass) / (3.0 * mstar)) * ((ri + rj) * 0.5);

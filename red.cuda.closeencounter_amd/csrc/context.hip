@@ -192,7 +192,15 @@ void refresh_pair_test(red_ctx* c)
     }
     P.rmax = rmax;
     P.mmax = mmax;
+    double rall = 0.0, mns = 0.0;
+    for (int i = 0; i < c->n; i++) {
+        rall = std::max(rall, c->h_radius[i]);
+        if (i >= c->cnt[0]) mns = std::max(mns, c->h_mass[i]);
+    }
+    P.rmax_all = rall;
+    P.mmax_ns = mns;
     P.close_encounter = c->prm.close_encounter;
+    P.n_star = c->cnt[0];
     P.enabled = c->prm.inner_steps && (P.radii_f > 0.0 || P.hill_f > 0.0);
     // the astro-centric start term uses the star's current mass (it grows by hit-centrum merges)
     if (c->gcfg.astro) c->gcfg.astro_mstar = c->h_mass[0];

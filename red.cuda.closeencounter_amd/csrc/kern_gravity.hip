@@ -102,7 +102,7 @@ __device__ __forceinline__ void pair_test(const PairTest& P, const EventSink& E,
     if (!(0 < i && i > j)) return;
     bool collide = d < P.radii_f * (Ri + Rj);
     bool hill = false;
-    if (!collide && P.hill_f > 0.0) {
+    if (!collide && P.hill_f > 0.0 && j >= P.n_star) {   // both bodies orbit the star
         const d4 qi = r[i], qj = r[j];
         double ri = sqrt((qi.x * qi.x) + (qi.y * qi.y) + (qi.z * qi.z));
         double rj = sqrt((qj.x * qj.x) + (qj.y * qj.y) + (qj.z * qj.z));
@@ -112,6 +112,21 @@ __device__ __forceinline__ void pair_test(const PairTest& P, const EventSink& E,
     if (collide || hill) append_pair_event(P, E, ctl, i, j, d, collide, r, v, mass, id);
 }
 
+
+// Distance below which pair_test can report an event for sink i, so that farther pairs skip
+// it without changing any result: the collision reach with the largest body radius, and the
+// Hill reach with the largest non-star mass, using |r_j| <= |r_i| + d (which turns
+// d < f c (|r_i| + |r_j|) / 2 into d < f c |r_i| / (1 - f c / 2)); 0.1 % margin for rounding.
+__device__ __forceinline__ double pair_screen(const PairTest& P, double Ri, double mi, const d4& qi)
+{
+    double cut = P.radii_f * (Ri + P.rmax_all);
+    if (P.hill_f > 0.0) {
+        const double fc = P.hill_f * cbrt((mi + P.mmax_ns) / P.three_mstar);
+        const double hr = sqrt((qi.x * qi.x) + (qi.y * qi.y) + (qi.z * qi.z));
+        cut = fc < 1.0 ? fmax(cut, fc * hr / (1.0 - 0.5 * fc)) : 1e300;
+    }
+    return cut * 1.001;
+}
 
 // ---------------------------------------------------------------------------------------
 // Source-range partition: part p of `parts` covers [lo, hi) of [j0, j1), slices rounded
@@ -299,6 +314,7 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
         const d4 p = r[i];
         const bool live = sink_in && id[i] > 0;
         const double Ri = EV ? radius[i] : 0.0;
+        const double scr = EV ? pair_screen(P, Ri, mass[i], p) : 0.0;
         d4 qa[M];
         int ida[M];            // raw ids: comparing here would wait for the loads
         auto fetch = [&](int c) {
@@ -333,7 +349,7 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
                     buf[bsel][jj][3 * sk + 0] = sc * dx;
                     buf[bsel][jj][3 * sk + 1] = sc * dy;
                     buf[bsel][jj][3 * sk + 2] = sc * dz;
-                    if (EV && use && live) pair_test(P, E, ctl, i, j, d, Ri, radius[j], r, v, mass, id);
+                    if (EV && use && live && d < scr) pair_test(P, E, ctl, i, j, d, Ri, radius[j], r, v, mass, id);
                 }
             }
         };
@@ -713,6 +729,43 @@ __global__ __launch_bounds__(256) void k_split_sum(const double* __restrict__ T,
     }
 }
 
+// Sums of the split-sweep terms where the summation order is free (mixed precision,
+// source-partitioned FP64): every row (sink, component) is cut into nseg contiguous
+// segments, one block per segment with coalesced loads and a tree in LDS; k_split_sum_fin
+// then adds each row's segments in segment order, so the result is run-to-run identical.
+__global__ __launch_bounds__(256) void k_split_sum_seg(const double* __restrict__ T, double* __restrict__ seg,
+                                                       Sweep sw, int nseg)
+{
+    __shared__ double red_lds[256];
+    const int nsrc = sw.src1 - sw.src0;
+    const int row = blockIdx.x, g = blockIdx.y;
+    const int len = (nsrc + nseg - 1) / nseg;
+    const int b = g * len, e = min(nsrc, b + len);
+    const double* t = T + (size_t)row * nsrc;
+    double acc = 0.0;
+    for (int j = b + (int)threadIdx.x; j < e; j += 256) acc += t[j];
+    red_lds[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red_lds[threadIdx.x] += red_lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) seg[(size_t)row * nseg + g] = red_lds[0];
+}
+
+__global__ __launch_bounds__(256) void k_split_sum_fin(const double* __restrict__ seg, d4* __restrict__ a,
+                                                       const int* __restrict__ id, Sweep sw, int nseg)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= (sw.sink1 - sw.sink0) * 3) return;
+    double acc = 0.0;
+    for (int k = 0; k < nseg; ++k) acc += seg[(size_t)row * nseg + k];
+    const int i = sw.sink0 + row / 3, c = row % 3;
+    double* o = reinterpret_cast<double*>(a + i);
+    o[c] = id[i] > 0 ? acc : 0.0;
+    if (c == 0) o[3] = 0.0;
+}
+
 template <int SPT, bool OV, bool EV>
 hipError_t go_fp64(int strategy, dim3 g, dim3 b, const Sweep& sw, const GravBuffers& B, int parts,
                    const PairTest& P, const EventSink& E, hipStream_t s)
@@ -837,10 +890,18 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
                                E, B.ctl, T);
         RED_HIP_TRY(hipGetLastError());
         dim3 g2(nsink * 3);
-        if (C.precision == RED_PREC_FP64 && !C.fp64_split)
+        const int nrow = nsink * 3;
+        const int nseg = std::min(64, std::max(1, nsrc / 2048));
+        if (C.precision == RED_PREC_FP64 && !C.fp64_split) {
             hipLaunchKernelGGL(k_split_sum<true>, g2, dim3(64), 0, s, T, B.a, B.id, sw);
-        else
+        } else if (B.partial && (size_t)nrow * nseg <= B.partial_elems * 4) {
+            double* seg = reinterpret_cast<double*>(B.partial);
+            hipLaunchKernelGGL(k_split_sum_seg, dim3(nrow, nseg), dim3(256), 0, s, T, seg, sw, nseg);
+            RED_HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(k_split_sum_fin, dim3((nrow + 255) / 256), dim3(256), 0, s, seg, B.a, B.id, sw, nseg);
+        } else {
             hipLaunchKernelGGL(k_split_sum<false>, g2, dim3(256), 0, s, T, B.a, B.id, sw);
+        }
         return hipGetLastError();
     }
     if (nsrc <= 0) {

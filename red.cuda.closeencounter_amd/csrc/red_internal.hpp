@@ -52,6 +52,9 @@ struct PairTest {
     double three_mstar;            // 3 M_star, for R_Hill
     double rmax;                   // largest body radius (mixed-precision screen)
     double mmax;                   // largest source mass (mixed-precision Hill screen)
+    double rmax_all;               // largest radius of any body (FP64 screen)
+    double mmax_ns;                // largest mass of any non-star body (FP64 screen)
+    int n_star;                    // bodies [0, n_star) are stars: no Hill test with them
     int close_encounter;           // 1: report CLOSE_ENCOUNTER, keep (i, j) order
     int enabled;                   // any pair test active
 };

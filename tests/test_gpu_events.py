@@ -115,9 +115,10 @@ def test_driver_loop_mixed_events_match_fp64(native_mod):
 
 
 def test_hill_close_encounters_reported(oracle_mod, native_mod):
+    # pairs within 10 mutual Hill radii in the ring (pairs with the star are never tested)
     b = planted_disk(seed=44)
     kw = dict(integrator=2, dt_day=0.5, hit_centrum=0.0, ejection=0.0, radii_enhance_factor=1.0,
-              encounter_hill_factor=3.0)
+              encounter_hill_factor=10.0)
     o = oracle_mod.Oracle(b, **kw)
     d = native_mod.Disk(b, event_capacity=20000, **kw)
     n_events, cnt = lockstep(o, d, 20)

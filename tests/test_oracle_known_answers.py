@@ -216,3 +216,27 @@ def test_energy_conserved_rkf78(oracle_mod):
         drift.append(abs((E1 - E0) / E0))
         assert np.linalg.norm(L1 - L0) / np.linalg.norm(L0) < (1e-6 if tol > 1e-12 else 1e-8)
     assert drift[1] < 5e-9 and drift[1] < drift[0] / 10, drift
+
+
+@pytest.mark.parametrize("frac,expect", [(0.99, True), (1.01, False)])
+def test_mutual_hill_criterion_by_hand(oracle_mod, frac, expect):
+    """Star + two protoplanets placed frac x f R_H apart, with R_H = ((m_1 + m_2) / (3 M))^(1/3)
+    (|r_1| + |r_2|) / 2: one close encounter below the reach, none above; the star, always
+    within a few such radii of each body, is never paired."""
+    M, m1, m2, f = 1.0, 3e-7, 2e-7, 4.0
+    rh0 = ((m1 + m2) / (3.0 * M)) ** (1.0 / 3.0)
+    # body 1 at x = 1; body 2 on the x axis at 1 + s, so |r_1| + |r_2| = 2 + s and
+    # s = frac f rh0 (2 + s) / 2
+    k = frac * f * rh0 / 2.0
+    s = 2.0 * k / (1.0 - k)
+    r = [[0, 0, 0], [1.0, 0, 0], [1.0 + s, 0, 0]]
+    v = [[0, 0, 0], [0, 1.0, 0], [0, 1.0, 0]]
+    b = _bodies(r, v, [M, m1, m2], types=[0, 3, 3], radius=[1e-6, 1e-9, 1e-9])
+    o = oracle_mod.Oracle(b, encounter_hill_factor=f, radii_enhance_factor=1.0, close_encounter=True)
+    o.accel(0.0, b.r, b.v)
+    ev = o.raw_events()
+    pairs = [(e["idx1"], e["idx2"]) for e in ev]
+    assert all(0 not in p for p in pairs)
+    assert pairs == ([(2, 1)] if expect else [])
+    if expect:
+        assert ev[0]["d"] == (1.0 + s) - 1.0
