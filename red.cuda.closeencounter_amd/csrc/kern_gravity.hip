@@ -277,11 +277,12 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
 // (sink, component) row, starting from 0.  Every term is formed and added exactly as in the
 // fused loop, so the sums are bit-identical to the reference order, while the long FP64
 // sequences run on all SIMDs.  A producer lane keeps one sink in registers and takes
-// J / SLOTS sources of every chunk (independent work for the scheduler to interleave); the
+// M sources of every chunk (independent work for the scheduler to interleave); the chunk
+// holds exactly M x SLOTS sources, so no producer lane idles on a full chunk, and the
 // source records of the chunk after next are requested before a chunk is computed.  Longer
 // chunks (fewer sinks per block) spread the per-chunk barrier and producer latency over
-// more sources; the tile stays 2 x 1024 x 3 doubles.
-constexpr int SQ_TILE = 1024;         // sinks x sources per chunk
+// more sources; the tile stays within 2 x 1024 x 3 doubles.
+constexpr int SQ_TILE = 1024;         // sinks x sources per chunk, at most
 
 template <bool OVERLAP, bool EV, int NP, int SQ_SG>
 __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __restrict__ r, const d4* __restrict__ v,
@@ -290,10 +291,11 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
                                                                  const int* __restrict__ id, Sweep sw, PairTest P,
                                                                  EventSink E, Ctl* __restrict__ ctl)
 {
-    constexpr int SQ_J = SQ_TILE / SQ_SG;             // sources per chunk
     constexpr int SQ_ROWS = 3 * SQ_SG;                // consumer lanes in use
     constexpr int SLOTS = NP * 64 / SQ_SG;            // source slots per pass over the chunk
-    constexpr int M = (SQ_J + SLOTS - 1) / SLOTS;     // sources per producer lane and chunk
+    constexpr int M = SQ_TILE / SQ_SG / SLOTS > 0 ? SQ_TILE / SQ_SG / SLOTS : 1;   // sources per lane and chunk
+    constexpr int SQ_J = M * SLOTS;                   // sources per chunk
+    static_assert(SQ_J * SQ_SG <= SQ_TILE, "tile");
     static_assert(SQ_ROWS <= 64, "one consumer wave");
     __shared__ double buf[2][SQ_J][SQ_ROWS];
     const int wv = threadIdx.x >> 6;
@@ -383,7 +385,8 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
             if (nj == SQ_J) {
                 // full chunk: the reads of the next 16 terms are in flight while the current
                 // 16 are added
-                constexpr int GQ = 16;
+                constexpr int GQ = SQ_J % 16 == 0 ? 16 : (SQ_J % 8 == 0 ? 8 : 4);
+                static_assert(SQ_J % GQ == 0, "consumer groups");
                 double x[GQ], y[GQ];
 #pragma unroll
                 for (int k = 0; k < GQ; ++k) x[k] = buf[b][k][lane];
@@ -687,10 +690,9 @@ __global__ __launch_bounds__(256) void k_split_terms(const d4* __restrict__ r, c
     }
 }
 
-// Split sweep, phase 2: one block per (sink, component).  SEQ: a single work-item adds
-// the terms in ascending j (the reference order); otherwise contiguous chunks are summed
-// in parallel and the chunk sums added in order.
-template <bool SEQ>
+// Split sweep, phase 2 where the order is free and no segment buffer is at hand: one block
+// per (sink, component); contiguous runs are summed in parallel and the run sums added in
+// order.
 __global__ __launch_bounds__(256) void k_split_sum(const double* __restrict__ T, d4* __restrict__ a,
                                                    const int* __restrict__ id, Sweep sw)
 {
@@ -700,31 +702,67 @@ __global__ __launch_bounds__(256) void k_split_sum(const double* __restrict__ T,
     const int i = sw.sink0 + row / 3;
     const int c = row % 3;
     const double* t = T + (size_t)row * nsrc;
-    double sum = 0.0;
-    if (SEQ) {
-        if (threadIdx.x == 0) {
-            int j = 0;
-            for (; j + 4 <= nsrc; j += 4) {
-                const double t0 = t[j], t1 = t[j + 1], t2 = t[j + 2], t3 = t[j + 3];
-                sum += t0; sum += t1; sum += t2; sum += t3;
-            }
-            for (; j < nsrc; ++j) sum += t[j];
-        }
-    } else {
-        const int per = (nsrc + blockDim.x - 1) / blockDim.x;
-        const int b = threadIdx.x * per;
-        const int e = min(nsrc, b + per);
-        double s = 0.0;
-        for (int j = b; j < e; ++j) s += t[j];
-        part[threadIdx.x] = s;
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int k = 0; k < (int)blockDim.x; ++k) sum += part[k];
-    }
+    const int per = (nsrc + blockDim.x - 1) / blockDim.x;
+    const int b = threadIdx.x * per;
+    const int e = min(nsrc, b + per);
+    double acc = 0.0;
+    for (int j = b; j < e; ++j) acc += t[j];
+    part[threadIdx.x] = acc;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        const bool live = id[i] > 0;
+        double sum = 0.0;
+        for (int k = 0; k < (int)blockDim.x; ++k) sum += part[k];
         double* o = reinterpret_cast<double*>(a + i);
-        o[c] = live ? sum : 0.0;
+        o[c] = id[i] > 0 ? sum : 0.0;
+        if (c == 0) o[3] = 0.0;
+    }
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int l)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Split sweep, phase 2 in the reference order, one wave per (sink, component) row: lane k
+// loads term 64 b + k of block b (one coalesced load per block, issued SPLIT_PF blocks
+// ahead) and the wave adds the block's terms lane by lane in ascending j, starting from 0,
+// so the serial chain of additions does not wait on memory.  Loads past the row end are
+// clamped to its last term and never added.
+constexpr int SPLIT_PF = 4;
+
+__global__ __launch_bounds__(64) void k_split_chain(const double* __restrict__ T, d4* __restrict__ a,
+                                                    const int* __restrict__ id, Sweep sw)
+{
+    const int nsrc = sw.src1 - sw.src0;
+    const int row = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double* t = T + (size_t)row * nsrc;
+    const int nb = (nsrc + 63) / 64;
+    double buf[SPLIT_PF];
+#pragma unroll
+    for (int k = 0; k < SPLIT_PF; ++k) buf[k] = t[min(k * 64 + lane, nsrc - 1)];
+    double sum = 0.0;
+    for (int b0 = 0; b0 < nb; b0 += SPLIT_PF) {
+#pragma unroll
+        for (int k = 0; k < SPLIT_PF; ++k) {
+            const double cur = buf[k];
+            buf[k] = t[min((b0 + k + SPLIT_PF) * 64 + lane, nsrc - 1)];
+            const int cnt = nsrc - (b0 + k) * 64;
+            if (cnt >= 64) {
+#pragma unroll
+                for (int l = 0; l < 64; ++l) sum += readlane_f64(cur, l);
+            } else {
+                for (int l = 0; l < cnt; ++l) sum += readlane_f64(cur, l);
+            }
+        }
+    }
+    if (lane == 0) {
+        const int i = sw.sink0 + row / 3, c = row % 3;
+        double* o = reinterpret_cast<double*>(a + i);
+        o[c] = id[i] > 0 ? sum : 0.0;
         if (c == 0) o[3] = 0.0;
     }
 }
@@ -865,7 +903,11 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
                                C.astro_mstar, P, E, B.ctl);
         return hipGetLastError();
     }
-    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0) {
+    // FP64 sequential sums over many sources for a handful of sinks: the producer/consumer
+    // sweep would run a few blocks, so the terms are formed by the whole chip and each
+    // (sink, component) row is added by a wave of its own
+    const bool few_sinks = nsink <= 64 && nsrc >= 4096 && split_scratch_bytes(sw) <= B.scratch_bytes;
+    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0 && !few_sinks) {
         const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
         const bool ev = P.enabled != 0;
         // sinks per block (chunk length 1024 / sg) and producer waves per block: as asked for,
@@ -893,14 +935,14 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
         const int nrow = nsink * 3;
         const int nseg = std::min(64, std::max(1, nsrc / 2048));
         if (C.precision == RED_PREC_FP64 && !C.fp64_split) {
-            hipLaunchKernelGGL(k_split_sum<true>, g2, dim3(64), 0, s, T, B.a, B.id, sw);
+            hipLaunchKernelGGL(k_split_chain, g2, dim3(64), 0, s, T, B.a, B.id, sw);
         } else if (B.partial && (size_t)nrow * nseg <= B.partial_elems * 4) {
             double* seg = reinterpret_cast<double*>(B.partial);
             hipLaunchKernelGGL(k_split_sum_seg, dim3(nrow, nseg), dim3(256), 0, s, T, seg, sw, nseg);
             RED_HIP_TRY(hipGetLastError());
             hipLaunchKernelGGL(k_split_sum_fin, dim3((nrow + 255) / 256), dim3(256), 0, s, seg, B.a, B.id, sw, nseg);
         } else {
-            hipLaunchKernelGGL(k_split_sum<false>, g2, dim3(256), 0, s, T, B.a, B.id, sw);
+            hipLaunchKernelGGL(k_split_sum, g2, dim3(256), 0, s, T, B.a, B.id, sw);
         }
         return hipGetLastError();
     }
