@@ -80,6 +80,11 @@ struct red_ctx {
     red_event* d_events_hc = nullptr;
     void* d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    double* d_seg = nullptr;              // segment sums of free-order split sweeps
+    size_t seg_elems = 0;
+    // the SI sweep runs on a second stream beside the NSI and NI sweeps (fork / join events)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     SrcMixed* d_src_mixed = nullptr;
     d4* d_partial = nullptr;
     size_t partial_elems = 0;
@@ -221,16 +226,28 @@ const d4* stage_v(red_ctx* c, int k)
 }
 
 // Records HIP events around fn() on the context stream when profiling is on.
+// The SI, NSI and NI sweeps (number_of_bodies.cpp:196-258); the astro-centric SI sweep
+// leaves the star out (number_of_bodies.cpp:218-221).
+void sweeps_of(const red_ctx* c, Sweep sw[3])
+{
+    const int nsi = c->nSI(), nm = c->nMassive(), n = c->n;
+    const int s0 = c->gcfg.astro ? 1 : 0;
+    sw[0] = Sweep{s0, nsi, s0, nm};
+    sw[1] = Sweep{nsi, nm, 0, nsi};
+    sw[2] = Sweep{nm, n, 0, nm};
+}
+
 template <typename F>
-hipError_t timed(red_ctx* c, int cls, F fn)
+hipError_t timed(red_ctx* c, int cls, F fn, hipStream_t s = nullptr)
 {
     if (!c->profiling) return fn();
+    if (!s) s = c->stream;
     red_ctx::ProfRec rec{cls, nullptr, nullptr};
     RED_HIP_TRY(hipEventCreate(&rec.a));
     RED_HIP_TRY(hipEventCreate(&rec.b));
-    RED_HIP_TRY(hipEventRecord(rec.a, c->stream));
+    RED_HIP_TRY(hipEventRecord(rec.a, s));
     hipError_t e = fn();
-    RED_HIP_TRY(hipEventRecord(rec.b, c->stream));
+    RED_HIP_TRY(hipEventRecord(rec.b, s));
     c->prof.push_back(rec);
     return e;
 }
@@ -246,16 +263,37 @@ hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const
     B.src_mixed = c->d_src_mixed;
     B.partial = c->d_partial;
     B.partial_elems = c->partial_elems;
+    B.seg = c->d_seg;
+    B.seg_elems = c->seg_elems;
     EventSink E;
     E.buf = c->d_events; E.cap = c->ev_cap; E.stage = stage; E.c_stage = c_stage;
     if (c->gcfg.precision == RED_PREC_MIXED)
         RED_HIP_TRY(timed(c, RED_PROF_PACK, [&] { return launch_pack_mixed(r, c->d_id, c->d_src_mixed, c->nMassive(), c->stream); }));
-    const int nsi = c->nSI(), nm = c->nMassive(), n = c->n;
-    const int s0 = c->gcfg.astro ? 1 : 0;   // the astro-centric SI sweep skips the star (number_of_bodies.cpp:205-209)
-    const Sweep sw[3] = {{s0, nsi, s0, nm}, {nsi, nm, 0, nsi}, {nm, n, 0, nm}};
+    Sweep sw[3];
+    sweeps_of(c, sw);
+    // The three sweeps write disjoint rows of a and append events atomically (the list is
+    // sorted afterwards), so the SI sweep -- few sinks, long serial sums -- can run on the
+    // side stream while the NSI and NI sweeps fill the chip, unless two of them would share
+    // the split scratch.
+    // Without graph replay the fork costs two event records and two waits per evaluation on
+    // the host, which only pays off for large sweeps.
+    const double main_pairs = (double)(sw[1].sink1 - sw[1].sink0) * (sw[1].src1 - sw[1].src0) +
+                              (double)(sw[2].sink1 - sw[2].sink0) * (sw[2].src1 - sw[2].src0);
+    const bool fork = c->side && sw[0].sink1 > sw[0].sink0 && main_pairs > 0.0 &&
+                      (c->prm.use_graph || main_pairs >= 16777216.0) &&
+                      !(sweep_wants_split(sw[1], c->gcfg) || sweep_wants_split(sw[2], c->gcfg));
+    if (fork) {
+        RED_HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
+        RED_HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    }
     for (int k = 0; k < 3; k++) {
         if (sw[k].sink1 <= sw[k].sink0) continue;
-        RED_HIP_TRY(timed(c, RED_PROF_SI + k, [&] { return launch_sweep(sw[k], B, c->gcfg, c->ptest, E, c->stream); }));
+        hipStream_t st = (fork && k == 0) ? c->side : c->stream;
+        RED_HIP_TRY(timed(c, RED_PROF_SI + k, [&] { return launch_sweep(sw[k], B, c->gcfg, c->ptest, E, st); }, st));
+    }
+    if (fork) {
+        RED_HIP_TRY(hipEventRecord(c->ev_join, c->side));
+        RED_HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
     return hipSuccess;
 }
@@ -497,11 +535,15 @@ int red_ctx_destroy(red_ctx* c)
     (void)hipFree(c->d_mass); (void)hipFree(c->d_radius); (void)hipFree(c->d_id); (void)hipFree(c->d_type);
     (void)hipFree(c->d_ctl); (void)hipFree(c->d_events); (void)hipFree(c->d_events_hc); (void)hipFree(c->d_scratch);
     (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy); (void)hipFree(c->d_energy_part);
-    (void)hipFree(c->d_partial);
+    (void)hipFree(c->d_partial); (void)hipFree(c->d_seg);
     (void)hipFree(c->d_learn_part); (void)hipFree(c->d_learn_counter);
     for (void* p : c->user_buffers) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->h_prop_out) (void)hipHostFree(c->h_prop_out);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return RED_OK;
@@ -615,20 +657,30 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
         CTX_HIP(c, dmalloc(&c->d_partial, c->partial_elems));
     }
     CTX_HIP(c, dmalloc(&c->d_energy, 6 * (size_t)cap));
-    // split-sweep scratch: the largest few-sink sweep
+    // split-sweep scratch: the largest sweep that forms its terms there; segment sums for
+    // the free-order ones
     {
-        const int nsi = c->nSI(), nm = c->nMassive();
-        size_t need = 0;
-        Sweep sw[3] = {{0, nsi, 0, nm}, {nsi, nm, 0, nsi}, {nm, n, 0, nm}};
-        for (auto& s : sw) {
-            const int ns_ = s.sink1 - s.sink0;
-            if (ns_ > 0 && ns_ <= c->gcfg.split_max_sinks && s.src1 - s.src0 >= 1024)
-                need = std::max(need, split_scratch_bytes(s));
+        size_t need = 0, rows = 0;
+        Sweep sws[3];
+        sweeps_of(c, sws);
+        for (const Sweep& s : sws) {
+            if (!sweep_wants_split(s, c->gcfg)) continue;
+            need = std::max(need, split_scratch_bytes(s));
+            rows = std::max(rows, (size_t)(s.sink1 - s.sink0) * 3);
         }
         const size_t limit = (size_t)4 << 30;
         if (need > limit) need = 0;   // too large: those sweeps use the broadcast path
         if (need > 0) CTX_HIP(c, hipMalloc(&c->d_scratch, need));
         c->scratch_bytes = need;
+        if (need > 0 && !(p->precision == RED_PREC_FP64 && !p->fp64_split && c->gcfg.strategy == 3)) {
+            c->seg_elems = rows * SPLIT_SEG_MAX;
+            CTX_HIP(c, hipMalloc(&c->d_seg, sizeof(double) * c->seg_elems));
+        }
+    }
+    if (c->nSI() > 0 && c->nNSI() + c->nNI() > 0) {
+        CTX_HIP(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        CTX_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        CTX_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     }
     // upload the state (w of r = mass)
     {

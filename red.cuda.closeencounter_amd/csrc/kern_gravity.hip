@@ -332,26 +332,48 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
                 ida[m] = id[j];
             }
         };
+        // one term: formed for every slot (the quotient is masked afterwards, so the M
+        // dependency chains of a lane interleave), written when the slot is in the chunk
+        auto term = [&](int bsel, int jj, int j, const d4& qm, int idm, bool in, double& dd, bool& hit) {
+            const double dx = qm.x - p.x;
+            const double dy = qm.y - p.y;
+            const double dz = qm.z - p.z;
+            const double r2 = (dx * dx + dy * dy) + dz * dz;
+            const double d = sqrt(r2);
+            const bool act = idm >= 0;
+            const bool use = sink_in && (OVERLAP ? (act && j != i) : act);
+            const double qt = qm.w / (d * r2);
+            const double sc = use ? qt : 0.0;
+            if (in) {
+                buf[bsel][jj][3 * sk + 0] = sc * dx;
+                buf[bsel][jj][3 * sk + 1] = sc * dy;
+                buf[bsel][jj][3 * sk + 2] = sc * dz;
+            }
+            dd = d;
+            hit = EV && in && use && live && d < scr;
+        };
         auto compute = [&](int c, int bsel, const d4* q, const int* qid) {
             const int j0 = sw.src0 + c * SQ_J;
             const int nj = min(SQ_J, sw.src1 - j0);
+            double dd[M];
+            bool hit[M];
+            if (nj == SQ_J) {   // full chunk (uniform): straight-line code
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-                const int jj = slot + m * SLOTS;
-                if (jj < nj) {
-                    const int j = j0 + jj;
-                    const double dx = q[m].x - p.x;
-                    const double dy = q[m].y - p.y;
-                    const double dz = q[m].z - p.z;
-                    const double r2 = (dx * dx + dy * dy) + dz * dz;
-                    const double d = sqrt(r2);
-                    const bool act = qid[m] >= 0;
-                    const bool use = sink_in && (OVERLAP ? (act && j != i) : act);
-                    const double sc = use ? q[m].w / (d * r2) : 0.0;
-                    buf[bsel][jj][3 * sk + 0] = sc * dx;
-                    buf[bsel][jj][3 * sk + 1] = sc * dy;
-                    buf[bsel][jj][3 * sk + 2] = sc * dz;
-                    if (EV && use && live && d < scr) pair_test(P, E, ctl, i, j, d, Ri, radius[j], r, v, mass, id);
+                for (int m = 0; m < M; ++m)
+                    term(bsel, slot + m * SLOTS, j0 + slot + m * SLOTS, q[m], qid[m], true, dd[m], hit[m]);
+            } else {
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+                    term(bsel, slot + m * SLOTS, j0 + slot + m * SLOTS, q[m], qid[m], slot + m * SLOTS < nj, dd[m],
+                         hit[m]);
+            }
+            if (EV) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) {
+                    if (hit[m]) {
+                        const int j = j0 + slot + m * SLOTS;
+                        pair_test(P, E, ctl, i, j, dd[m], Ri, radius[j], r, v, mass, id);
+                    }
                 }
             }
         };
@@ -383,8 +405,9 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
         if (lane < 3 * ns) {
             const int nj = min(SQ_J, nsrc - c * SQ_J);
             if (nj == SQ_J) {
-                // full chunk: the reads of the next 16 terms are in flight while the current
-                // 16 are added
+                // full chunk: the reads of the next GQ terms are in flight while the current
+                // GQ are added (scheduling barriers keep the compiler from sinking the reads
+                // next to their uses, which would put the LDS latency on the serial chain)
                 constexpr int GQ = SQ_J % 16 == 0 ? 16 : (SQ_J % 8 == 0 ? 8 : 4);
                 static_assert(SQ_J % GQ == 0, "consumer groups");
                 double x[GQ], y[GQ];
@@ -396,8 +419,10 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
 #pragma unroll
                         for (int k = 0; k < GQ; ++k) y[k] = buf[b][(g + 1) * GQ + k][lane];
                     }
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int k = 0; k < GQ; ++k) sum += x[k];
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int k = 0; k < GQ; ++k) x[k] = y[k];
                 }
@@ -887,6 +912,18 @@ int sweep_parts(const Sweep& sw, const GravConfig& C, size_t partial_elems)
     return parts;
 }
 
+bool sweep_wants_split(const Sweep& sw, const GravConfig& C)
+{
+    const int nsink = sw.sink1 - sw.sink0;
+    const int nsrc = sw.src1 - sw.src0;
+    if (nsink <= 0 || nsrc <= 0 || C.astro) return false;
+    // FP64 sequential sums over many sources for a handful of sinks: the producer/consumer
+    // sweep would run a few blocks, so the terms are formed by the whole chip and each
+    // (sink, component) row is added by a wave of its own
+    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3) return nsink <= 64 && nsrc >= 4096;
+    return nsink <= C.split_max_sinks && nsrc >= 1024;
+}
+
 hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig& C, const PairTest& P,
                         const EventSink& E, hipStream_t s)
 {
@@ -903,11 +940,8 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
                                C.astro_mstar, P, E, B.ctl);
         return hipGetLastError();
     }
-    // FP64 sequential sums over many sources for a handful of sinks: the producer/consumer
-    // sweep would run a few blocks, so the terms are formed by the whole chip and each
-    // (sink, component) row is added by a wave of its own
-    const bool few_sinks = nsink <= 64 && nsrc >= 4096 && split_scratch_bytes(sw) <= B.scratch_bytes;
-    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0 && !few_sinks) {
+    const bool use_split = sweep_wants_split(sw, C) && split_scratch_bytes(sw) <= B.scratch_bytes;
+    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0 && !use_split) {
         const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
         const bool ev = P.enabled != 0;
         // sinks per block (chunk length 1024 / sg) and producer waves per block: as asked for,
@@ -919,8 +953,6 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
         if (ov) return ev ? go_fp64_seq<true, true>(np, sg, sw, B, P, E, s) : go_fp64_seq<true, false>(np, sg, sw, B, P, E, s);
         return ev ? go_fp64_seq<false, true>(np, sg, sw, B, P, E, s) : go_fp64_seq<false, false>(np, sg, sw, B, P, E, s);
     }
-    const bool use_split = nsrc > 0 && nsink <= C.split_max_sinks && nsrc >= 1024 &&
-                           split_scratch_bytes(sw) <= B.scratch_bytes;
     if (use_split) {
         double* T = (double*)B.scratch;
         dim3 g1((nsrc + 255) / 256, (nsink + SPLIT_SG - 1) / SPLIT_SG);
@@ -933,11 +965,11 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
         RED_HIP_TRY(hipGetLastError());
         dim3 g2(nsink * 3);
         const int nrow = nsink * 3;
-        const int nseg = std::min(64, std::max(1, nsrc / 2048));
+        const int nseg = std::min(SPLIT_SEG_MAX, std::max(1, nsrc / 2048));
         if (C.precision == RED_PREC_FP64 && !C.fp64_split) {
             hipLaunchKernelGGL(k_split_chain, g2, dim3(64), 0, s, T, B.a, B.id, sw);
-        } else if (B.partial && (size_t)nrow * nseg <= B.partial_elems * 4) {
-            double* seg = reinterpret_cast<double*>(B.partial);
+        } else if (B.seg && (size_t)nrow * nseg <= B.seg_elems) {
+            double* seg = B.seg;
             hipLaunchKernelGGL(k_split_sum_seg, dim3(nrow, nseg), dim3(256), 0, s, T, seg, sw, nseg);
             RED_HIP_TRY(hipGetLastError());
             hipLaunchKernelGGL(k_split_sum_fin, dim3((nrow + 255) / 256), dim3(256), 0, s, seg, B.a, B.id, sw, nseg);

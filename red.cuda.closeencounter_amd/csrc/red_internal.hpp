@@ -91,6 +91,8 @@ struct GravBuffers {
     SrcMixed* src_mixed;           // packed sources for the mixed sweep (cap entries)
     d4* partial;                   // per-part sums of source-partitioned sweeps
     size_t partial_elems;
+    double* seg;                   // segment sums of split sweeps (own buffer: a split
+    size_t seg_elems;              // sweep may run beside a partitioned one)
 };
 
 struct GravConfig {
@@ -117,6 +119,9 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
                         const EventSink& E, hipStream_t s);
 // Scratch bytes the split path needs for a sweep.
 size_t split_scratch_bytes(const Sweep& sw);
+// whether launch_sweep forms the sweep's terms in the split scratch (given room for them)
+bool sweep_wants_split(const Sweep& sw, const GravConfig& C);
+constexpr int SPLIT_SEG_MAX = 64;   // segments per row of a free-order split sum
 // Packs SrcMixed records for bodies [0, n).
 hipError_t launch_pack_mixed(const d4* r, const int* id, SrcMixed* out, int n, hipStream_t s);
 
