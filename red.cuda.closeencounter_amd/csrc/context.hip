@@ -275,12 +275,13 @@ hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const
     // sorted afterwards), so the SI sweep -- few sinks, long serial sums -- can run on the
     // side stream while the NSI and NI sweeps fill the chip, unless two of them would share
     // the split scratch.
-    // Without graph replay the fork costs two event records and two waits per evaluation on
-    // the host, which only pays off for large sweeps.
+    // The fork costs two event records and two waits per evaluation (on the host without
+    // graph replay, as graph edges with it), which only pays off when the other sweeps are
+    // large enough to hide the SI sweep under them.
     const double main_pairs = (double)(sw[1].sink1 - sw[1].sink0) * (sw[1].src1 - sw[1].src0) +
                               (double)(sw[2].sink1 - sw[2].sink0) * (sw[2].src1 - sw[2].src0);
-    const bool fork = c->side && sw[0].sink1 > sw[0].sink0 && main_pairs > 0.0 &&
-                      (c->prm.use_graph || main_pairs >= 16777216.0) &&
+    const bool fork = c->side && sw[0].sink1 > sw[0].sink0 &&
+                      main_pairs >= (c->prm.use_graph ? 4194304.0 : 16777216.0) &&
                       !(sweep_wants_split(sw[1], c->gcfg) || sweep_wants_split(sw[2], c->gcfg));
     if (fork) {
         RED_HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
