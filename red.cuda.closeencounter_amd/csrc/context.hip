@@ -85,6 +85,10 @@ struct red_ctx {
     // the SI sweep runs on a second stream beside the NSI and NI sweeps (fork / join events)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // the proposer's pair features run on their own stream beside the step's f0
+    hipStream_t learn_side = nullptr;
+    hipEvent_t ev_learn_fork = nullptr, ev_learn_join = nullptr;
+    LearnPlan learn_plan_cur{};
     SrcMixed* d_src_mixed = nullptr;
     d4* d_partial = nullptr;
     size_t partial_elems = 0;
@@ -541,6 +545,10 @@ int red_ctx_destroy(red_ctx* c)
     for (void* p : c->user_buffers) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->h_prop_out) (void)hipHostFree(c->h_prop_out);
+    if (c->learn_side) (void)hipStreamSynchronize(c->learn_side);
+    if (c->ev_learn_fork) (void)hipEventDestroy(c->ev_learn_fork);
+    if (c->ev_learn_join) (void)hipEventDestroy(c->ev_learn_join);
+    if (c->learn_side) (void)hipStreamDestroy(c->learn_side);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -784,8 +792,13 @@ float clampf(double x, double lo, double hi) { return (float)std::min(hi, std::m
 
 int learn_buffers(red_ctx* c)
 {
-    if (c->d_learn_part && c->d_learn_counter && c->h_prop_out) return RED_OK;
+    if (c->d_learn_part && c->d_learn_counter && c->h_prop_out && c->learn_side) return RED_OK;
     SetupLock lk;
+    if (!c->learn_side) {
+        CTX_HIP(c, hipStreamCreateWithFlags(&c->learn_side, hipStreamNonBlocking));
+        CTX_HIP(c, hipEventCreateWithFlags(&c->ev_learn_fork, hipEventDisableTiming));
+        CTX_HIP(c, hipEventCreateWithFlags(&c->ev_learn_join, hipEventDisableTiming));
+    }
     if (!c->d_learn_part) CTX_HIP(c, hipMalloc(&c->d_learn_part, sizeof(float) * 3 * LEARN_MAX_BLOCKS));
     if (!c->d_learn_counter) {
         CTX_HIP(c, hipMalloc(&c->d_learn_counter, sizeof(unsigned)));
@@ -823,11 +836,30 @@ int learn_launch(red_ctx* c, bool propose, double eta_n)
     A.prev_log_max_pert = c->prev_log_max_pert;
     A.have_prev = c->have_prev_feat ? 1 : 0;
     LearnBuffers B{c->d_learn_part, c->d_learn_counter, c->h_prop_out_dev};
+    CTX_HIP(c, hipStreamWaitEvent(c->stream, c->ev_learn_join, 0));   // the pair launch of learn_pairs
     CTX_HIP(c, timed(c, RED_PROF_LEARN, [&] {
-        return launch_learn_step(propose ? c->prop_net : nullptr, A, c->d_r[c->cur], c->d_a[0], c->d_id, c->nSI(),
-                                 c->nMassive(), c->n, B, c->d_ctl, propose ? 1 : 0, eta_n, c->prop_safety,
-                                 c->prop_max_log, c->stream);
+        return launch_learn_tail(propose ? c->prop_net : nullptr, A, c->learn_plan_cur, c->d_r[c->cur], c->d_a[0],
+                                 c->d_id, c->nSI(), c->nMassive(), c->n, B, c->d_ctl, propose ? 1 : 0, eta_n,
+                                 c->prop_safety, c->prop_max_log, c->stream);
     }));
+    return RED_OK;
+}
+
+// At the start of a step, before f0: the pair features need only y_n, so their launch runs
+// on the proposer's stream beside the force evaluation (learn_launch joins it).
+int learn_pairs(red_ctx* c)
+{
+    int rc = learn_buffers(c);
+    if (rc) return rc;
+    c->learn_plan_cur = learn_plan(c->nSI(), c->nMassive(), c->n);
+    LearnBuffers B{c->d_learn_part, c->d_learn_counter, c->h_prop_out_dev};
+    CTX_HIP(c, hipEventRecord(c->ev_learn_fork, c->stream));
+    CTX_HIP(c, hipStreamWaitEvent(c->learn_side, c->ev_learn_fork, 0));
+    CTX_HIP(c, timed(c, RED_PROF_LEARN, [&] {
+        return launch_learn_pairs(c->learn_plan_cur, c->d_r[c->cur], c->d_id, c->nSI(), c->nMassive(), c->n, B,
+                                  c->learn_side);
+    }, c->learn_side));
+    CTX_HIP(c, hipEventRecord(c->ev_learn_join, c->learn_side));
     return RED_OK;
 }
 
@@ -909,16 +941,20 @@ int red_step(red_ctx* c, red_step_info* info)
     if (!c) return RED_ERR_ARG;
     const Scheme& S = c->S;
     const bool adaptive = c->prm.adaptive && S.err_form >= 0;
+    const bool learning = adaptive && (c->prop_net || c->hist_cap > 0);
     // f0 = f(t, y_n) (int_rungekutta8.cu:715-724)
     hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, c->stream, c->d_ctl, c->t, c->dt_try, 0, 1, 0, 0);
     CTX_HIP(c, hipGetLastError());
+    if (learning) {
+        int rc = learn_pairs(c);
+        if (rc) return rc;
+    }
     CTX_HIP(c, run_maybe_graph(c, &c->g_f0[c->cur], enqueue_f0));
     c->raw_valid = false;
     // learned proposal of the first trial step (the reference controller's step otherwise):
     // the step-start kernel runs on the stream ahead of the first attempt, which takes its
     // step from the device; nothing waits for it on the host
     float rec[RED_HIST_FIELDS] = {0.0f};
-    const bool learning = adaptive && (c->prop_net || c->hist_cap > 0);
     const bool rec_open = learning && c->n_eta >= 4;
     const bool proposed = rec_open && c->prop_net != nullptr;
     const double eta_n = rec_open ? c->eta_hist[0] : 0.0;

@@ -114,6 +114,88 @@ __device__ void forward(Lds& L, int ns)
     __syncthreads();
 }
 
+// The forward pass of one sample with the weights read straight from global memory into
+// registers (one row of W1 and W2 per unit, issued before the inputs are known so that the
+// loads overlap the feature reduction): the same multiply-adds in the same order as
+// forward(), so the output is bit-identical, with a few hundred bytes of LDS instead of a
+// 32-sample tile (the step-start kernel carries this LDS in every block of its grid).
+struct Lds1 {
+    float z[NF];
+    float h1[NH], h2[NH], w3[NH];
+    float out;
+};
+
+struct RowW {
+    uint4 w2[NH / 8];          // W2[u][0..63], 8 BF16 per uint4
+    uint2 w1[NF / 4];          // W1[u][0..11], 4 BF16 per uint2
+    unsigned short b1, b2, w3;
+};
+static_assert(NH % 8 == 0 && NF % 4 == 0 && OFF_W2 % 8 == 0 && OFF_W1 % 4 == 0, "row loads");
+
+__device__ __forceinline__ void load_row(RowW& R, const unsigned short* __restrict__ p, int u)
+{
+    const uint4* w2 = reinterpret_cast<const uint4*>(p + OFF_W2 + u * NH);
+#pragma unroll
+    for (int k = 0; k < NH / 8; k++) R.w2[k] = w2[k];
+    const uint2* w1 = reinterpret_cast<const uint2*>(p + OFF_W1 + u * NF);
+#pragma unroll
+    for (int k = 0; k < NF / 4; k++) R.w1[k] = w1[k];
+    R.b1 = p[OFF_B1 + u];
+    R.b2 = p[OFF_B2 + u];
+    R.w3 = p[OFF_W3 + u];
+}
+
+__device__ __forceinline__ float bf_lo(unsigned v) { return __uint_as_float(v << 16); }
+__device__ __forceinline__ float bf_hi(unsigned v) { return __uint_as_float(v & 0xffff0000u); }
+
+// All threads call it; threads u < NH hold row u in R (loaded by load_row).
+__device__ void forward_one(Lds1& S, const RowW& R, const unsigned short* __restrict__ p,
+                            const float* __restrict__ norm, const float* x)
+{
+    const int u = threadIdx.x;
+    if (u < NF) {
+        const float z = (x[u] - norm[u]) * norm[NF + u];
+        S.z[u] = fminf(fmaxf(z, -Z_CLIP), Z_CLIP);
+    }
+    __syncthreads();
+    if (u < NH) {
+        float acc = bf(R.b1);
+#pragma unroll
+        for (int k = 0; k < NF / 4; k++) {
+            acc = fmaf(bf_lo(R.w1[k].x), S.z[4 * k + 0], acc);
+            acc = fmaf(bf_hi(R.w1[k].x), S.z[4 * k + 1], acc);
+            acc = fmaf(bf_lo(R.w1[k].y), S.z[4 * k + 2], acc);
+            acc = fmaf(bf_hi(R.w1[k].y), S.z[4 * k + 3], acc);
+        }
+        S.h1[u] = gelu(acc);
+    }
+    __syncthreads();
+    if (u < NH) {
+        float acc = bf(R.b2);
+#pragma unroll
+        for (int k = 0; k < NH / 8; k++) {
+            const uint4 q = R.w2[k];
+            acc = fmaf(bf_lo(q.x), S.h1[8 * k + 0], acc);
+            acc = fmaf(bf_hi(q.x), S.h1[8 * k + 1], acc);
+            acc = fmaf(bf_lo(q.y), S.h1[8 * k + 2], acc);
+            acc = fmaf(bf_hi(q.y), S.h1[8 * k + 3], acc);
+            acc = fmaf(bf_lo(q.z), S.h1[8 * k + 4], acc);
+            acc = fmaf(bf_hi(q.z), S.h1[8 * k + 5], acc);
+            acc = fmaf(bf_lo(q.w), S.h1[8 * k + 6], acc);
+            acc = fmaf(bf_hi(q.w), S.h1[8 * k + 7], acc);
+        }
+        S.h2[u] = gelu(acc);
+        S.w3[u] = bf(R.w3);
+    }
+    __syncthreads();
+    if (u == 0) {
+        float acc = bf(p[OFF_B3]);
+        for (int k = 0; k < NH; k++) acc = fmaf(S.w3[k], S.h2[k], acc);
+        S.out = acc;
+    }
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_mlp_predict(const unsigned short* __restrict__ p, const float* __restrict__ norm,
                                                      const float* __restrict__ X, float* __restrict__ Y, int n)
 {
@@ -260,18 +342,23 @@ constexpr int FEAT_SRC = 64;    // sources per work item of the pair pass
 // from FP32 star-relative positions (the star subtracted in FP64) and the hardware square
 // root and reciprocal: a few 1e-4 relative in d at 1e-3 AU separations 5 AU out, ample for
 // a logarithmic feature.  Per body: |a| and |a - a_kep| / |a_kep|, a_kep = -m_star (r - r_star)/|r - r_star|^3.
-// Each block stores its extrema in its own slot; the last block to finish (counter) reduces
-// the slots, completes the feature vector, runs the network and applies the proposal.
+// Each block stores its extrema in its own slot.  The kernel runs in two launches: the pair
+// items [it0, it1) need only the state, so they run on a stream of their own while the step's
+// first force evaluation is computed (tail = 0: slots [0, blocks)); the body items need that
+// evaluation and run after it (tail = 1: slots from slot0), and the last of their blocks to
+// finish (counter) reduces all final_slots slots, completes the feature vector, runs the
+// network and applies the proposal.
 __global__ __launch_bounds__(256) void k_learn_step(const red::d4* __restrict__ r, const red::d4* __restrict__ a,
                                                     const int* __restrict__ id, int n_si, int n_massive, int n,
-                                                    long n0, int ch0, long n1, int ch1, long n_items,
+                                                    long n0, int ch0, long n1, int ch1, long it0, long it1,
+                                                    int slot0, int final_slots, int tail,
                                                     float* __restrict__ part, unsigned* __restrict__ counter,
                                                     float* __restrict__ out, const unsigned short* __restrict__ wb,
                                                     const float* __restrict__ norm, red::ProposeArgs A,
                                                     red::Ctl* __restrict__ ctl, int apply, double eta_n,
                                                     double safety, double max_log)
 {
-    __shared__ Lds L;
+    __shared__ Lds1 S;
     __shared__ float red3[3][4];
     __shared__ float x[NF];
     __shared__ int last;
@@ -281,7 +368,7 @@ __global__ __launch_bounds__(256) void k_learn_step(const red::d4* __restrict__ 
     const red::d4 star = r[0];
     const double ms = star.w;
     float best = __uint_as_float(0x7f800000u), amax = 0.0f, pmax = 0.0f;
-    for (long w = (long)blockIdx.x * (blockDim.x >> 6) + wid; w < n_items; w += nwaves) {
+    for (long w = it0 + (long)blockIdx.x * (blockDim.x >> 6) + wid; w < it1; w += nwaves) {
         if (w < n0 + n1) {
             int i, i1, src0, src1;
             long c;
@@ -308,7 +395,7 @@ __global__ __launch_bounds__(256) void k_learn_step(const red::d4* __restrict__ 
             const float hi = sqrtf(xi * xi + yi * yi + zi * zi);
             const float mi = (float)ri.w;
             const float three_ms = (float)(3.0 * ms);
-            for (int k = 0; k < nj; ++k) {
+            auto pair = [&](int k) {
                 const float qx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sx), k));
                 const float qy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sy), k));
                 const float qz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sz), k));
@@ -321,6 +408,13 @@ __global__ __launch_bounds__(256) void k_learn_step(const red::d4* __restrict__ 
                 const float q3 = d2 * __builtin_amdgcn_sqrtf(d2) * three_ms * __builtin_amdgcn_rcpf(mm * hm * hm * hm);
                 const bool ok = live && qm >= 0.0f && j0 + k != i && mm > 0.0f;
                 best = ok ? fminf(best, q3) : best;
+            };
+            if (nj == FEAT_SRC) {
+                // full chunk: unrolled, so that several pairs' sqrt / rcp chains overlap
+#pragma unroll 8
+                for (int k = 0; k < FEAT_SRC; ++k) pair(k);
+            } else {
+                for (int k = 0; k < nj; ++k) pair(k);
             }
         } else {
             const int i = 1 + (int)(w - n0 - n1) * 64 + lane;
@@ -351,21 +445,28 @@ __global__ __launch_bounds__(256) void k_learn_step(const red::d4* __restrict__ 
             amax = fmaxf(amax, red3[1][k]);
             pmax = fmaxf(pmax, red3[2][k]);
         }
-        part[3 * blockIdx.x + 0] = best;
-        part[3 * blockIdx.x + 1] = amax;
-        part[3 * blockIdx.x + 2] = pmax;
-        __threadfence();
-        const unsigned done = atomicAdd(counter, 1u);
-        last = done == gridDim.x - 1;
+        part[3 * (slot0 + blockIdx.x) + 0] = best;
+        part[3 * (slot0 + blockIdx.x) + 1] = amax;
+        part[3 * (slot0 + blockIdx.x) + 2] = pmax;
+        if (tail) {
+            __threadfence();
+            const unsigned done = atomicAdd(counter, 1u);
+            last = done == gridDim.x - 1;
+        } else {
+            last = 0;
+        }
     }
     __syncthreads();
     if (!last) return;
     // last block: every other block's slot is visible (release fence before its increment);
-    // all its threads read the slots and the block reduces them
+    // all its threads read the slots and the block reduces them, while the network's weight
+    // rows are on their way into registers
     __threadfence();
+    RowW R;
+    if (wb && threadIdx.x < NH) load_row(R, wb, threadIdx.x);
     {
         float q3 = __uint_as_float(0x7f800000u), ma = 0.0f, mp = 0.0f;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+        for (int b = threadIdx.x; b < final_slots; b += blockDim.x) {
             q3 = fminf(q3, __hip_atomic_load(&part[3 * b + 0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
             ma = fmaxf(ma, __hip_atomic_load(&part[3 * b + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
             mp = fmaxf(mp, __hip_atomic_load(&part[3 * b + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
@@ -399,12 +500,11 @@ __global__ __launch_bounds__(256) void k_learn_step(const red::d4* __restrict__ 
     }
     __syncthreads();
     if (!wb) return;
-    load_block(L, wb, norm, x, 0, 1);
-    forward(L, 1);
+    forward_one(S, R, wb, norm, x);
     if (threadIdx.x == 0) {
-        out[NF] = L.out[0];
+        out[NF] = S.out;
         if (apply) {
-            const double y = fmin(max_log, fmax(-max_log, (double)L.out[0]));
+            const double y = fmin(max_log, fmax(-max_log, (double)S.out));
             ctl->dt = safety * eta_n * exp(y);
         }
     }
@@ -442,21 +542,41 @@ namespace red {
 
 int mlp_device(const red_mlp* M) { return M->device; }
 
-hipError_t launch_learn_step(const red_mlp* net, const ProposeArgs& A, const d4* r, const d4* a, const int* id,
-                             int n_si, int n_massive, int n, const LearnBuffers& B, Ctl* ctl, int apply,
-                             double eta_n, double safety, double max_log, hipStream_t s)
+LearnPlan learn_plan(int n_si, int n_massive, int n)
 {
+    LearnPlan L;
     const long c0 = n_si > 1 ? (n_si - 1 + 63) / 64 : 0;                // SI sink groups (no star)
     const long c1 = n_si > 1 ? (n_massive - n_si + 63) / 64 : 0;        // NSI sink groups
-    const int ch0 = n_massive > 1 ? (n_massive - 1 + FEAT_SRC - 1) / FEAT_SRC : 0;
-    const int ch1 = n_si > 1 ? (n_si - 1 + FEAT_SRC - 1) / FEAT_SRC : 0;
-    const long n0 = c0 * ch0, n1 = c1 * ch1;
-    const long nb2 = n > 1 ? (n - 1 + 63) / 64 : 0;
-    const long items = n0 + n1 + nb2;
-    const long blocks = std::max<long>(1, std::min<long>((items + 3) / 4, LEARN_MAX_BLOCKS));
-    hipLaunchKernelGGL(k_learn_step, dim3((unsigned)blocks), dim3(256), 0, s, r, a, id, n_si, n_massive, n, n0, ch0,
-                       n1, ch1, items, B.part, B.counter, B.out, net ? net->wb : nullptr, net ? net->norm : nullptr, A,
-                       ctl, apply, eta_n, safety, max_log);
+    L.ch0 = n_massive > 1 ? (n_massive - 1 + FEAT_SRC - 1) / FEAT_SRC : 0;
+    L.ch1 = n_si > 1 ? (n_si - 1 + FEAT_SRC - 1) / FEAT_SRC : 0;
+    L.n0 = c0 * L.ch0;
+    L.n1 = c1 * L.ch1;
+    L.nb = n > 1 ? (n - 1 + 63) / 64 : 0;
+    L.b0 = (int)std::min<long>((L.n0 + L.n1 + 3) / 4, LEARN_PAIR_BLOCKS);
+    L.b1 = (int)std::max<long>(1, std::min<long>((L.nb + 3) / 4, LEARN_BODY_BLOCKS));
+    return L;
+}
+
+hipError_t launch_learn_pairs(const LearnPlan& L, const d4* r, const int* id, int n_si, int n_massive, int n,
+                              const LearnBuffers& B, hipStream_t s)
+{
+    if (L.b0 <= 0) return hipSuccess;
+    ProposeArgs A;
+    memset(&A, 0, sizeof(A));
+    hipLaunchKernelGGL(k_learn_step, dim3((unsigned)L.b0), dim3(256), 0, s, r, nullptr, id, n_si, n_massive, n, L.n0,
+                       L.ch0, L.n1, L.ch1, 0L, L.n0 + L.n1, 0, 0, 0, B.part, B.counter, B.out, nullptr, nullptr, A,
+                       nullptr, 0, 0.0, 0.0, 0.0);
+    return hipGetLastError();
+}
+
+hipError_t launch_learn_tail(const red_mlp* net, const ProposeArgs& A, const LearnPlan& L, const d4* r, const d4* a,
+                             const int* id, int n_si, int n_massive, int n, const LearnBuffers& B, Ctl* ctl,
+                             int apply, double eta_n, double safety, double max_log, hipStream_t s)
+{
+    const long it0 = L.n0 + L.n1;
+    hipLaunchKernelGGL(k_learn_step, dim3((unsigned)L.b1), dim3(256), 0, s, r, a, id, n_si, n_massive, n, L.n0, L.ch0,
+                       L.n1, L.ch1, it0, it0 + L.nb, L.b0, L.b0 + L.b1, 1, B.part, B.counter, B.out,
+                       net ? net->wb : nullptr, net ? net->norm : nullptr, A, ctl, apply, eta_n, safety, max_log);
     return hipGetLastError();
 }
 

@@ -202,19 +202,32 @@ struct ProposeArgs {
     int have_prev;
 };
 // Device buffers of the fused step-start kernel.
-constexpr int LEARN_MAX_BLOCKS = 512;
+constexpr int LEARN_PAIR_BLOCKS = 2048;   // blocks of the pair launch (slots 0 ..)
+constexpr int LEARN_BODY_BLOCKS = 512;    // blocks of the body launch (slots after those)
+constexpr int LEARN_MAX_BLOCKS = LEARN_PAIR_BLOCKS + LEARN_BODY_BLOCKS;
 struct LearnBuffers {
     float* part;                // 3 floats per block: min (d/R_H)^3, max |a|, max |a - a_kep|/|a_kep|
     unsigned* counter;          // blocks finished; the last block resets it to 0
     float* out;                 // host-mapped: features [0..11], network output [12]
 };
-// One launch at the start of a step: reduces the state features of (r, a) -- pairs SI x
-// massive and NSI x SI without the star (index 0), bodies 1..n-1 with id > 0 -- completes the
-// feature vector from A, runs the network (if net) and, with apply, sets
+// Work of the step-start kernel: pair items SI x massive and NSI x SI without the star
+// (index 0), body items 1..n-1 with id > 0, and the block counts of its two launches.
+struct LearnPlan {
+    long n0, n1, nb;
+    int ch0, ch1;
+    int b0, b1;
+};
+LearnPlan learn_plan(int n_si, int n_massive, int n);
+// First launch (state only, may run beside the step's first force evaluation): the pair
+// extrema of r into slots [0, b0).
+hipError_t launch_learn_pairs(const LearnPlan& L, const d4* r, const int* id, int n_si, int n_massive, int n,
+                              const LearnBuffers& B, hipStream_t s);
+// Second launch, after the first and after the evaluation of a: the body extrema, the
+// reduction of all slots, the feature vector from A, the network (if net) and, with apply,
 // ctl->dt = safety * eta_n * exp(clamp(y, +-max_log)).
-hipError_t launch_learn_step(const red_mlp* net, const ProposeArgs& A, const d4* r, const d4* a, const int* id,
-                             int n_si, int n_massive, int n, const LearnBuffers& B, Ctl* ctl, int apply,
-                             double eta_n, double safety, double max_log, hipStream_t s);
+hipError_t launch_learn_tail(const red_mlp* net, const ProposeArgs& A, const LearnPlan& L, const d4* r, const d4* a,
+                             const int* id, int n_si, int n_massive, int n, const LearnBuffers& B, Ctl* ctl,
+                             int apply, double eta_n, double safety, double max_log, hipStream_t s);
 int mlp_device(const red_mlp* M);
 
 
