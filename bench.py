@@ -125,6 +125,63 @@ def proposer_section(native, years, use_graph=True):
     return out
 
 
+def drift_section(native, years, report_years=100.0):
+    """Energy and angular-momentum drift of the 16384-body disk over `years` simulated years
+    (FP64 with the reference's sequential sums, RKF7(8) at tol 1e-12, graphs), driven like
+    the reference's main loop: ejection check, step, collision check (merges at 1 x (R_i +
+    R_j), close encounters at 3 mutual Hill radii reported), compaction after 8 events.  A
+    merge changes the total energy by itself; the energy is measured on both sides of every
+    collision pass, so those jumps are kept apart from the integration drift."""
+    import numpy as np
+    from rednative import generate
+    b = generate.to_internal(generate.disk_powerlaw(n=16383, seed=202))
+    d = native.Disk(b, integrator=4, adaptive=True, tolerance=1e-12, dt_day=0.01, use_graph=True,
+                    radii_enhance_factor=1.0, encounter_hill_factor=3.0, ejection=100.0, event_capacity=65536)
+    E0, L0 = d.energy()
+    t0 = d.t
+    t_stop = t0 + years * 365.25 * GAUSS
+    jumps = 0.0
+    checkpoints = []
+    next_report = report_years
+    w0 = time.perf_counter()
+    while d.t <= t_stop:
+        if d.check_for_ejection_hit_centrum():
+            d.clear_event_counter()
+        d.step()
+        if d.n_event() > 0:
+            Ea, _ = d.energy()
+            if d.check_for_collision():
+                d.clear_event_counter()
+            Eb, _ = d.energy()
+            jumps += Eb - Ea
+        d.check_for_rebuild_vectors(8)
+        yrs = (d.t - t0) / GAUSS / 365.25
+        if yrs >= next_report:
+            E, L = d.energy()
+            drift = abs((E - E0 - jumps) / E0)
+            checkpoints.append([round(yrs, 3), drift])
+            print(f"drift: {yrs:.1f} yr, integration dE/E {drift:.3e}, merges dE/E {jumps / E0:.3e}, "
+                  f"{time.perf_counter() - w0:.0f} s", file=sys.stderr, flush=True)
+            next_report += report_years
+    d.sync()
+    sec = time.perf_counter() - w0
+    E1, L1 = d.energy()
+    c = d.counters()
+    sim = (d.t - t0) / GAUSS / 365.25
+    out = {"workload": "1 star + 16383 protoplanets/planetesimals (power-law masses, seed 202), RKF7(8) tol 1e-12, "
+                       "FP64 sequential sums, collisions 1x(R_i+R_j) merged, close encounters at 3 mutual Hill radii, "
+                       "ejection 100 AU",
+           "sim_years": sim, "seconds": sec, "sim_yr_per_hr": sim / (sec / 3600.0),
+           "dE_over_E_integration": abs((E1 - E0 - jumps) / E0), "dE_over_E_merges": jumps / E0,
+           "dE_over_E_total": abs((E1 - E0) / E0),
+           "dL_over_L": float(np.linalg.norm(L1 - L0) / np.linalg.norm(L0)),
+           "steps": c["passed"], "attempts": c["tried"], "collisions": c["n_collision"],
+           "close_encounters": c["n_close_encounter"], "ejections": c["n_ejection"], "bodies_left": d.n,
+           "checkpoints": checkpoints}
+    d.close()
+    return out
+
+
 def ensemble_section(native, days, sample):
     """512 disks (N log-uniform in [64, 4096], seeded) integrated together against the same
     disks stepped one context at a time (a sample of them, scaled): disk-steps per second."""
@@ -201,8 +258,10 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-days", type=float, default=20.0)
     ap.add_argument("--no-graph", action="store_true", help="proposer section without HIP graphs")
-    ap.add_argument("--only", choices=["proposer", "ensemble"], default=None,
-                    help="run one auxiliary section alone and print its JSON (for profiling)")
+    ap.add_argument("--only", choices=["proposer", "ensemble", "drift"], default=None,
+                    help="run one auxiliary section alone and print its JSON (drift: the 16384-body "
+                         "disk's energy drift over --drift-years, not part of the default run)")
+    ap.add_argument("--drift-years", type=float, default=1000.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -220,6 +279,9 @@ def main():
         return
     if args.only == "ensemble":
         print(json.dumps({"ensemble": ensemble_section(native, args.ensemble_days, 16)}))
+        return
+    if args.only == "drift":
+        print(json.dumps({"drift": drift_section(native, args.drift_years)}))
         return
 
     bodies = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
