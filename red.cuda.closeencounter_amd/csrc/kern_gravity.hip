@@ -113,6 +113,46 @@ __device__ __forceinline__ void pair_test(const PairTest& P, const EventSink& E,
 }
 
 
+// IEEE square root and quotient for operands well inside the normal range.  For such
+// operands the compiler's correctly rounded sequences reduce to these cores: the square
+// root's range scaling is by 2^0, its special-value select keeps the core result, the
+// quotient's v_div_scale returns its operands unchanged with no scale flag (exponent gap
+// < 768, no denormal numerator, reciprocal or quotient), v_div_fmas is then a plain fused
+// multiply-add and v_div_fixup passes a normal quotient through.  So the results are
+// bit-identical, with about ten fewer instructions per pair.  Callers take this path only
+// when every lane's operands lie in [2^-300, 2^300] (in_core_range); otherwise they use
+// sqrt() and the division operator.
+__device__ __forceinline__ bool in_core_range(double x)
+{
+    return (unsigned)(__double2hiint(x) - 0x2D300000) < 0x25800000u;   // 2^-300 <= x < 2^300
+}
+
+__device__ __forceinline__ double sqrt_core(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double e = fma(-g, g, x);
+    g = fma(e, h, g);
+    e = fma(-g, g, x);
+    return fma(e, h, g);
+}
+
+__device__ __forceinline__ double div_core(double num, double den)
+{
+    double rc = __builtin_amdgcn_rcp(den);
+    double e = fma(-den, rc, 1.0);
+    rc = fma(rc, e, rc);
+    e = fma(-den, rc, 1.0);
+    rc = fma(rc, e, rc);
+    const double q = num * rc;
+    const double rem = fma(-den, q, num);
+    return fma(rem, rc, q);
+}
+
 // Distance below which pair_test can report an event for sink i, so that farther pairs skip
 // it without changing any result: the collision reach with the largest body radius, and the
 // Hill reach with the largest non-star mass, using |r_j| <= |r_i| + d (which turns
@@ -317,9 +357,8 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
         const bool live = sink_in && id[i] > 0;
         const double Ri = EV ? radius[i] : 0.0;
         const double scr = EV ? pair_screen(P, Ri, mass[i], p) : 0.0;
-        d4 qa[M];
-        int ida[M];            // raw ids: comparing here would wait for the loads
-        auto fetch = [&](int c) {
+        // raw ids are kept: comparing them here would wait for the loads
+        auto fetch = [&](int c, d4* qd, int* idd) {
             const int j0 = sw.src0 + c * SQ_J;
             const int nj = min(SQ_J, sw.src1 - j0);
 #pragma unroll
@@ -328,8 +367,8 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
                 // for the data to merge it; slots past the chunk are never used
                 const int jj = slot + m * SLOTS;
                 const int j = j0 + (jj < nj ? jj : 0);
-                qa[m] = r[j];
-                ida[m] = id[j];
+                qd[m] = r[j];
+                idd[m] = id[j];
             }
         };
         // one term: formed for every slot (the quotient is masked afterwards, so the M
@@ -352,12 +391,46 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
             dd = d;
             hit = EV && in && use && live && d < scr;
         };
+        // full chunk with every operand of the wave in the core range: the same terms through
+        // sqrt_core / div_core
+        auto compute_core = [&](int j0, int bsel, const d4* q, const int* qid, double* dd, bool* hit) {
+            double dx[M], dy[M], dz[M], r2[M];
+            bool use[M];
+            bool ok = true;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const int j = j0 + slot + m * SLOTS;
+                dx[m] = q[m].x - p.x;
+                dy[m] = q[m].y - p.y;
+                dz[m] = q[m].z - p.z;
+                r2[m] = (dx[m] * dx[m] + dy[m] * dy[m]) + dz[m] * dz[m];
+                const bool act = qid[m] >= 0;
+                use[m] = sink_in && (OVERLAP ? (act && j != i) : act);
+                ok = ok && (!use[m] || (in_core_range(r2[m]) && in_core_range(q[m].w)));
+            }
+            if (!__all(ok)) return false;
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const int jj = slot + m * SLOTS;
+                const double d = sqrt_core(r2[m]);
+                const double qt = div_core(q[m].w, d * r2[m]);
+                const double sc = use[m] ? qt : 0.0;
+                buf[bsel][jj][3 * sk + 0] = sc * dx[m];
+                buf[bsel][jj][3 * sk + 1] = sc * dy[m];
+                buf[bsel][jj][3 * sk + 2] = sc * dz[m];
+                dd[m] = d;
+                hit[m] = EV && use[m] && live && d < scr;
+            }
+            return true;
+        };
         auto compute = [&](int c, int bsel, const d4* q, const int* qid) {
             const int j0 = sw.src0 + c * SQ_J;
             const int nj = min(SQ_J, sw.src1 - j0);
             double dd[M];
             bool hit[M];
-            if (nj == SQ_J) {   // full chunk (uniform): straight-line code
+            if (nj == SQ_J && compute_core(j0, bsel, q, qid, dd, hit)) {
+                // done on the core path
+            } else if (nj == SQ_J) {   // full chunk (uniform): straight-line code
 #pragma unroll
                 for (int m = 0; m < M; ++m)
                     term(bsel, slot + m * SLOTS, j0 + slot + m * SLOTS, q[m], qid[m], true, dd[m], hit[m]);
@@ -377,24 +450,31 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
                 }
             }
         };
-        d4 q[M];
-        int qid[M];
+        // two register sets: even chunks in A, odd chunks in B, each refilled with the chunk
+        // two ahead as soon as it has been computed (no copies between the sets); the
+        // producers pass the same 1 + nch barriers as the consumer
+        d4 qA[M], qB[M];
+        int iA[M], iB[M];
         if (nch > 0) {
-            fetch(0);
-#pragma unroll
-            for (int m = 0; m < M; ++m) { q[m] = qa[m]; qid[m] = ida[m]; }
-            if (nch > 1) fetch(1);
-            compute(0, 0, q, qid);
+            fetch(0, qA, iA);
+            if (nch > 1) fetch(1, qB, iB);
+            compute(0, 0, qA, iA);
+            if (nch > 2) fetch(2, qA, iA);
         }
         __syncthreads();
-        for (int c = 0; c < nch; ++c) {
+        for (int c = 0; c < nch; c += 2) {
             if (c + 1 < nch) {
-#pragma unroll
-                for (int m = 0; m < M; ++m) { q[m] = qa[m]; qid[m] = ida[m]; }
-                if (c + 2 < nch) fetch(c + 2);
-                compute(c + 1, (c + 1) & 1, q, qid);
+                compute(c + 1, 1, qB, iB);
+                if (c + 3 < nch) fetch(c + 3, qB, iB);
             }
             __syncthreads();
+            if (c + 1 < nch) {
+                if (c + 2 < nch) {
+                    compute(c + 2, 0, qA, iA);
+                    if (c + 4 < nch) fetch(c + 4, qA, iA);
+                }
+                __syncthreads();
+            }
         }
         return;
     }

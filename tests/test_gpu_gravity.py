@@ -143,3 +143,37 @@ def test_mixed_accuracy_and_events(oracle_mod, native_mod, kahan):
     assert len(ev_o) > 0
     _events_equal(ev_o, ev_g)
 
+
+
+def test_fp64_extreme_scales_bit_exact(oracle_mod, native_mod):
+    """Separations and masses across and around 2^-300 .. 2^300, where the producer/consumer
+    sweep switches between its core square-root / quotient sequences and the full IEEE ones
+    (per wave): every result must still equal the oracle's bit for bit."""
+    from rednative import files
+    rng = np.random.default_rng(77)
+    groups = [
+        (1e-45, 40),    # mutual r^2 ~ 1e-90, around 2^-300
+        (1e-30, 40),    # r^2 ~ 1e-60: inside
+        (1.0, 40),      # ordinary
+        (1.5e45, 40),   # r^2 ~ 1e90 .. 1e91, around 2^300
+    ]
+    r = [np.array([3.0, 0.0, 0.0])]
+    for scale, k in groups:
+        r += list(rng.uniform(-1.0, 1.0, (k, 3)) * scale)
+    r = np.array(r)
+    n = len(r)
+    m = np.exp(rng.uniform(np.log(1e-95), np.log(1e20), n))   # some below 2^-300
+    m[0] = 1.0
+    types = np.full(n, 3, dtype=np.int32); types[0] = 0
+    counts = [1, 0, 0, n - 1, 0, 0, 0]
+    b = files.BodySet(counts, np.arange(1, n + 1, dtype=np.int32), [f"b{k}" for k in range(n)], types,
+                      np.zeros(n), m, np.full(n, 1e-300), np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int32),
+                      np.zeros(n), r, np.zeros((n, 3)))
+    print(f"case extreme scales n={n}", flush=True)
+    o = oracle_mod.Oracle(b)
+    a_o = o.accel(0.0, b.r, b.v)
+    assert np.all(np.isfinite(a_o))
+    for spt, block in ((4, 512), (8, 512), (16, 1024)):
+        d = native_mod.Disk(b, sinks_per_thread=spt, block=block, strategy=3)
+        assert_bitwise(d.accel(0.0, b.r, b.v), a_o, f"accel sg={spt}")
+        d.close()
