@@ -125,18 +125,34 @@ def proposer_section(native, years, use_graph=True):
     return out
 
 
-def drift_section(native, years, report_years=100.0):
-    """Energy and angular-momentum drift of the 16384-body disk over `years` simulated years
-    (FP64 with the reference's sequential sums, RKF7(8) at tol 1e-12, graphs), driven like
-    the reference's main loop: ejection check, step, collision check (merges at 1 x (R_i +
-    R_j), close encounters at 3 mutual Hill radii reported), compaction after 8 events.  A
-    merge changes the total energy by itself; the energy is measured on both sides of every
+def drift_section(native, years, disk="powerlaw", report_years=None):
+    """Energy and angular-momentum drift over `years` simulated years, driven like the
+    reference's main loop: ejection check, step, collision check (merges at 1 x (R_i + R_j)),
+    compaction after 8 events.  disk = "powerlaw": the 16384-body disk, FP64 with the
+    reference's sequential sums, RKF7(8) at tol 1e-12, close encounters at 3 mutual Hill
+    radii reported; disk = "mixed": the 131072-body disk in mixed precision, Cash-Karp
+    RK5(4) at tol 1e-10 (no Hill test: with RK5 any pair event accepts the attempt).  A merge
+    changes the total energy by itself; the energy is measured on both sides of every
     collision pass, so those jumps are kept apart from the integration drift."""
     import numpy as np
     from rednative import generate
-    b = generate.to_internal(generate.disk_powerlaw(n=16383, seed=202))
-    d = native.Disk(b, integrator=4, adaptive=True, tolerance=1e-12, dt_day=0.01, use_graph=True,
-                    radii_enhance_factor=1.0, encounter_hill_factor=3.0, ejection=100.0, event_capacity=65536)
+    if disk in ("mixed", "mixed-disk-fp64"):
+        prec = native.PREC_MIXED if disk == "mixed" else native.PREC_FP64
+        b = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
+        d = native.Disk(b, integrator=3, adaptive=True, tolerance=1e-10, dt_day=1.0, precision=prec,
+                        radii_enhance_factor=1.0, ejection=100.0, event_capacity=65536)
+        label = ("1 star + 2 giants + 65536 planetesimals + 65533 test particles (seed 303), "
+                 + ("mixed precision" if disk == "mixed" else "FP64 sequential sums")
+                 + ", Cash-Karp RK5(4) tol 1e-10, collisions 1x(R_i+R_j) merged, ejection 100 AU")
+    else:
+        b = generate.to_internal(generate.disk_powerlaw(n=16383, seed=202))
+        d = native.Disk(b, integrator=4, adaptive=True, tolerance=1e-12, dt_day=0.01, use_graph=True,
+                        radii_enhance_factor=1.0, encounter_hill_factor=3.0, ejection=100.0, event_capacity=65536)
+        label = ("1 star + 16383 protoplanets/planetesimals (power-law masses, seed 202), RKF7(8) tol 1e-12, "
+                 "FP64 sequential sums, collisions 1x(R_i+R_j) merged, close encounters at 3 mutual Hill radii, "
+                 "ejection 100 AU")
+    if report_years is None:
+        report_years = years / 10.0
     E0, L0 = d.energy()
     t0 = d.t
     t_stop = t0 + years * 365.25 * GAUSS
@@ -168,9 +184,7 @@ def drift_section(native, years, report_years=100.0):
     E1, L1 = d.energy()
     c = d.counters()
     sim = (d.t - t0) / GAUSS / 365.25
-    out = {"workload": "1 star + 16383 protoplanets/planetesimals (power-law masses, seed 202), RKF7(8) tol 1e-12, "
-                       "FP64 sequential sums, collisions 1x(R_i+R_j) merged, close encounters at 3 mutual Hill radii, "
-                       "ejection 100 AU",
+    out = {"workload": label,
            "sim_years": sim, "seconds": sec, "sim_yr_per_hr": sim / (sec / 3600.0),
            "dE_over_E_integration": abs((E1 - E0 - jumps) / E0), "dE_over_E_merges": jumps / E0,
            "dE_over_E_total": abs((E1 - E0) / E0),
@@ -256,12 +270,13 @@ def main():
     ap.add_argument("--no-proposer", action="store_true")
     ap.add_argument("--proposer-years", type=float, default=5.0)
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--ensemble-days", type=float, default=20.0)
+    ap.add_argument("--ensemble-days", type=float, default=100.0)
     ap.add_argument("--no-graph", action="store_true", help="proposer section without HIP graphs")
     ap.add_argument("--only", choices=["proposer", "ensemble", "drift"], default=None,
                     help="run one auxiliary section alone and print its JSON (drift: the 16384-body "
                          "disk's energy drift over --drift-years, not part of the default run)")
     ap.add_argument("--drift-years", type=float, default=1000.0)
+    ap.add_argument("--drift-disk", choices=["powerlaw", "mixed", "mixed-disk-fp64"], default="powerlaw")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -281,7 +296,7 @@ def main():
         print(json.dumps({"ensemble": ensemble_section(native, args.ensemble_days, 16)}))
         return
     if args.only == "drift":
-        print(json.dumps({"drift": drift_section(native, args.drift_years)}))
+        print(json.dumps({"drift": drift_section(native, args.drift_years, args.drift_disk)}))
         return
 
     bodies = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
