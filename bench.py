@@ -69,6 +69,30 @@ def run_path(native, bodies, precision, steps, warmup, kahan=False):
                 counts=d.counts(), t_end=d.t)
 
 
+def accuracy_section(native, bodies, steps=10, dt_day=1.0):
+    """Mixed against FP64 on the same disk: the relative acceleration error per body of one
+    force evaluation, and the position / velocity difference after `steps` fixed Cash-Karp
+    steps of dt_day from the same state (fixed steps, so both paths take the same times)."""
+    import numpy as np
+    kw = dict(radii_enhance_factor=1.0, ejection=100.0)
+    d64 = native.Disk(bodies, integrator=3, adaptive=False, dt_day=dt_day, **kw)
+    dmx = native.Disk(bodies, integrator=3, adaptive=False, dt_day=dt_day, precision=native.PREC_MIXED, **kw)
+    a64 = d64.accel(0.0, bodies.r, bodies.v)
+    amx = dmx.accel(0.0, bodies.r, bodies.v)
+    rel = np.linalg.norm(amx - a64, axis=1) / np.maximum(np.linalg.norm(a64, axis=1), 1e-300)
+    for _ in range(steps):
+        d64.step()
+        dmx.step()
+    s64, smx = d64.state(), dmx.state()
+    dr = np.linalg.norm(smx["r"] - s64["r"], axis=1) / np.maximum(np.linalg.norm(s64["r"], axis=1), 1e-300)
+    dv = np.linalg.norm(smx["v"] - s64["v"], axis=1) / np.maximum(np.linalg.norm(s64["v"], axis=1), 1e-300)
+    d64.close(); dmx.close()
+    return {"accel_rel_err_max": float(rel.max()), "accel_rel_err_median": float(np.median(rel)),
+            "steps": steps, "dt_day": dt_day,
+            "pos_rel_err_max": float(dr.max()), "pos_rel_err_median": float(np.median(dr)),
+            "vel_rel_err_max": float(dv.max()), "vel_rel_err_median": float(np.median(dv))}
+
+
 def cpu_baseline(bodies_full, target_s=10.0):
     """CPU oracle force evaluations on a bounded sample of the same disk."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -326,6 +350,7 @@ def main():
         achieved64 = FLOP_PER_PAIR * ni_pairs / ((ni64_ms / max(ni64_n, 1)) * 1e-3) / 1e12
     traffic, traffic_src = load_traffic(args.pmc_summary)
 
+    acc = None if args.no_fp64 else accuracy_section(native, bodies)
     cpu = None if args.no_cpu else cpu_baseline(bodies)
     prop = None if args.no_proposer else proposer_section(native, args.proposer_years)
     ens = None if args.no_ensemble else ensemble_section(native, args.ensemble_days, 16)
@@ -355,6 +380,7 @@ def main():
                  "ms_per_step": s64["ms_per_step"], "dE_over_E_timed": s64["dE_over_E"],
                  "steps": args.fp64_steps, "ni_sweep_tflops_20flop": achieved64,
                  "ni_sweep_frac_fp64_peak": achieved64 / PEAK_FP64_TFLOPS},
+        "mixed_vs_fp64": acc,
         "speedup_sim_yr_per_hr_vs_fp64": sm["sim_yr_per_hr"] / s64["sim_yr_per_hr"] if s64 else None,
         "speedup_pairs_per_s_vs_fp64": sm["pairs_per_s"] / s64["pairs_per_s"] if s64 else None,
         "roofline": {"bound": "valu", "kernel": "k_grav_mixed_bcast (test-particle sweep)",
