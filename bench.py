@@ -69,6 +69,59 @@ def run_path(native, bodies, precision, steps, warmup, kahan=False):
                 counts=d.counts(), t_end=d.t)
 
 
+def reference_workload_section(native, reps=20):
+    """The workload of the reference's only published timings (nvprof traces of
+    calc_grav_accel on a GeForce GT 630, red.cuda/nvprofile.results.xlsx, from the harness
+    red.code_snippet.cuda.test/main.cu:1049): 1 star + 5119 protoplanets + 5120 test
+    particles, FP64, 5120 sinks x 5120 sources per sweep.  Here: the FP64 path with the
+    reference's sequential sums, the SI and NI sweeps timed with HIP events."""
+    import math
+    import numpy as np
+    from rednative import generate
+    rng = generate._rng(1049)
+    B = generate._Builder()
+    generate._add_star(B)
+    rho = 2.0 * generate.GRAM_PER_CM3_TO_SOLAR_PER_AU3
+    m = np.exp(rng.uniform(math.log(1e-12), math.log(1e-9), 5119))
+    generate._add_bodies(B, 3, "pp", m, rho, generate._elements(rng, 5119, 0.4, 4.0), 1.0)
+    generate._add_bodies(B, 6, "tp", np.zeros(5120), 0.0, generate._elements(rng, 5120, 0.4, 4.0), 1.0)
+    b = generate.to_internal(B.build())
+    d = native.Disk(b, radii_enhance_factor=1.0)
+    n = b.n
+    rr = np.zeros((n, 4)); vv = np.zeros((n, 4))
+    rr[:, :3] = b.r; rr[:, 3] = b.mass; vv[:, :3] = b.v
+    bufs = [d.device_alloc(n * 32) for _ in range(3)]
+    d.to_device(bufs[0], rr)
+    d.to_device(bufs[1], vv)
+    d.accel_device(0.0, 0, bufs[0], bufs[1], bufs[2])
+    d.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d.accel_device(0.0, 0, bufs[0], bufs[1], bufs[2])
+    d.sync()
+    eval_us = (time.perf_counter() - t0) / reps * 1e6
+    d.set_profiling(True)
+    d.profile(reset=True)
+    for _ in range(reps):
+        d.accel_device(0.0, 0, bufs[0], bufs[1], bufs[2])
+    d.sync()
+    prof = d.profile(reset=True)
+    for p in bufs:
+        d.device_free(p)
+    d.close()
+    si_us = prof["si"][0] / max(prof["si"][1], 1) * 1e3
+    ni_us = prof["ni"][0] / max(prof["ni"][1], 1) * 1e3
+    pairs = 5120 * 5120
+    return {"workload": "1 star + 5119 protoplanets + 5120 test particles, FP64 sequential sums, collision test on",
+            "eval_us": eval_us, "reference_published_eval_ms": 135.45 + 166.45,
+            "speedup_vs_published_eval": (135.45 + 166.45) * 1e3 / eval_us,
+            "si_sweep_us": si_us, "ni_sweep_us": ni_us,
+            "pairs_per_s_si": (pairs - 5120) / (si_us * 1e-6), "pairs_per_s_ni": pairs / (ni_us * 1e-6),
+            "reference_published_ms": {"si": [134.9, 136.0], "ni": [165.7, 167.2], "padded": [72.6, 73.2],
+                                       "hardware": "GeForce GT 630"},
+            "speedup_vs_published_si": 135.45e3 / si_us, "speedup_vs_published_ni": 166.45e3 / ni_us}
+
+
 def accuracy_section(native, bodies, steps=10, dt_day=1.0):
     """Mixed against FP64 on the same disk: the relative acceleration error per body of one
     force evaluation, and the position / velocity difference after `steps` fixed Cash-Karp
@@ -351,6 +404,7 @@ def main():
     traffic, traffic_src = load_traffic(args.pmc_summary)
 
     acc = None if args.no_fp64 else accuracy_section(native, bodies)
+    refw = None if args.no_fp64 else reference_workload_section(native)
     cpu = None if args.no_cpu else cpu_baseline(bodies)
     prop = None if args.no_proposer else proposer_section(native, args.proposer_years)
     ens = None if args.no_ensemble else ensemble_section(native, args.ensemble_days, 16)
@@ -381,6 +435,7 @@ def main():
                  "steps": args.fp64_steps, "ni_sweep_tflops_20flop": achieved64,
                  "ni_sweep_frac_fp64_peak": achieved64 / PEAK_FP64_TFLOPS},
         "mixed_vs_fp64": acc,
+        "reference_workload": refw,
         "speedup_sim_yr_per_hr_vs_fp64": sm["sim_yr_per_hr"] / s64["sim_yr_per_hr"] if s64 else None,
         "speedup_pairs_per_s_vs_fp64": sm["pairs_per_s"] / s64["pairs_per_s"] if s64 else None,
         "roofline": {"bound": "valu", "kernel": "k_grav_mixed_bcast (test-particle sweep)",
