@@ -533,6 +533,8 @@ int red_ctx_destroy(red_ctx* c)
     SetupLock lk;
     destroy_graphs(c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);               // work forked off and
+    if (c->learn_side) (void)hipStreamSynchronize(c->learn_side);   // not yet joined
     for (int k = 0; k < 2; k++) { (void)hipFree(c->d_r[k]); (void)hipFree(c->d_v[k]); }
     for (auto p : c->d_sr) (void)hipFree(p);
     for (auto p : c->d_sv) (void)hipFree(p);
@@ -545,11 +547,9 @@ int red_ctx_destroy(red_ctx* c)
     for (void* p : c->user_buffers) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->h_prop_out) (void)hipHostFree(c->h_prop_out);
-    if (c->learn_side) (void)hipStreamSynchronize(c->learn_side);
     if (c->ev_learn_fork) (void)hipEventDestroy(c->ev_learn_fork);
     if (c->ev_learn_join) (void)hipEventDestroy(c->ev_learn_join);
     if (c->learn_side) (void)hipStreamDestroy(c->learn_side);
-    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->side) (void)hipStreamDestroy(c->side);
