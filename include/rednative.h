@@ -237,7 +237,8 @@ int red_copy_to_host(red_ctx* ctx, void* host, const void* dev, unsigned long lo
  * the attempt is captured once in a HIP graph (params->use_graph) and replayed, and each
  * disk runs the reference's step control (integrator.cu:211-263, int_rungekutta8.cu:704-797)
  * on its own time and trial step, so it takes exactly the steps it would take alone.
- * FP64 only; pair tests are not applied inside an ensemble. */
+ * FP64, barycentric frame; pair tests, ejection and hit-centrum checks do not run inside
+ * an ensemble, and parameters asking for them are refused with RED_ERR_UNSUPPORTED. */
 typedef struct red_ensemble red_ensemble;
 typedef struct red_ensemble_info {
     int    supersteps;            /* attempts launched (one graph replay each)      */

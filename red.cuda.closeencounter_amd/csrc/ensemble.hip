@@ -333,6 +333,11 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     if (p->integrator < RED_EULER || p->integrator > RED_RKF78 || p->precision != RED_PREC_FP64 ||
         p->frame_center != RED_FRAME_BARY)
         return RED_ERR_UNSUPPORTED;
+    // no pair tests, ejection or hit-centrum checks run inside an ensemble: asking for them
+    // is refused rather than ignored
+    if (p->radii_enhance_factor > 0.0 || p->encounter_hill_factor > 0.0 || p->close_encounter ||
+        p->ejection > 0.0 || p->hit_centrum > 0.0)
+        return RED_ERR_UNSUPPORTED;
     red_ensemble* E = new red_ensemble();
     E->prm = *p;
     E->S = make_scheme(p->integrator, p->adaptive != 0);

@@ -471,14 +471,23 @@ class Disk:
 
 
 def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_check_for_tp=False,
-            use_graph=True, device=0, block=0, **_):
+            use_graph=True, device=0, block=0, frame_center=1, precision=PREC_FP64, radii_enhance_factor=0.0,
+            encounter_hill_factor=0.0, close_encounter=False, ejection=0.0, hit_centrum=0.0):
+    """Ensemble parameters; every option is passed on, so that the library can refuse the ones
+    an ensemble does not support instead of their being dropped here."""
     prm = RedParams()
     prm.integrator = integrator
     prm.adaptive = int(adaptive)
     prm.tolerance = tolerance
     prm.error_check_for_tp = int(error_check_for_tp)
     prm.inner_steps = 1
-    prm.frame_center = 1
+    prm.frame_center = frame_center
+    prm.precision = precision
+    prm.radii_enhance_factor = radii_enhance_factor
+    prm.encounter_hill_factor = encounter_hill_factor
+    prm.close_encounter = int(close_encounter)
+    prm.ejection = ejection
+    prm.hit_centrum = hit_centrum
     prm.dt_day = dt_day
     prm.use_graph = int(use_graph)
     prm.device = device
