@@ -573,6 +573,8 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     }
     if (p->integrator < RED_EULER || p->integrator > RED_RKF78) { delete c; return RED_ERR_UNSUPPORTED; }
     if (p->frame_center != RED_FRAME_BARY && p->frame_center != RED_FRAME_ASTRO) { delete c; return RED_ERR_ARG; }
+    if (p->precision != RED_PREC_FP64 && p->precision != RED_PREC_MIXED) { delete c; return RED_ERR_ARG; }
+    if (p->strategy < 0 || p->strategy > 3) { delete c; return RED_ERR_ARG; }
     if (!p->inner_steps && (p->radii_enhance_factor > 0.0 || p->encounter_hill_factor > 0.0)) { delete c; return RED_ERR_UNSUPPORTED; }
     if (n > 0 && (!b->id || !b->type || !b->mass || !b->radius || !b->density || !b->cd || !b->r || !b->v)) {
         delete c;
