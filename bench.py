@@ -202,7 +202,7 @@ def proposer_section(native, years, use_graph=True):
     return out
 
 
-def drift_section(native, years, disk="powerlaw", report_years=None):
+def drift_section(native, years, disk="powerlaw", report_years=None, tol=None):
     """Energy and angular-momentum drift over `years` simulated years, driven like the
     reference's main loop: ejection check, step, collision check (merges at 1 x (R_i + R_j)),
     compaction after 8 events.  disk = "powerlaw": the 16384-body disk, FP64 with the
@@ -210,22 +210,25 @@ def drift_section(native, years, disk="powerlaw", report_years=None):
     radii reported; disk = "mixed": the 131072-body disk in mixed precision, Cash-Karp
     RK5(4) at tol 1e-10 (no Hill test: with RK5 any pair event accepts the attempt).  A merge
     changes the total energy by itself; the energy is measured on both sides of every
-    collision pass, so those jumps are kept apart from the integration drift."""
+    collision pass, so those jumps are kept apart from the integration drift.  tol replaces
+    the disk's step tolerance."""
     import numpy as np
     from rednative import generate
     if disk in ("mixed", "mixed-disk-fp64"):
         prec = native.PREC_MIXED if disk == "mixed" else native.PREC_FP64
         b = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
-        d = native.Disk(b, integrator=3, adaptive=True, tolerance=1e-10, dt_day=1.0, precision=prec,
+        tol = 1e-10 if tol is None else tol
+        d = native.Disk(b, integrator=3, adaptive=True, tolerance=tol, dt_day=1.0, precision=prec,
                         radii_enhance_factor=1.0, ejection=100.0, event_capacity=65536)
         label = ("1 star + 2 giants + 65536 planetesimals + 65533 test particles (seed 303), "
                  + ("mixed precision" if disk == "mixed" else "FP64 sequential sums")
-                 + ", Cash-Karp RK5(4) tol 1e-10, collisions 1x(R_i+R_j) merged, ejection 100 AU")
+                 + f", Cash-Karp RK5(4) tol {tol:g}, collisions 1x(R_i+R_j) merged, ejection 100 AU")
     else:
         b = generate.to_internal(generate.disk_powerlaw(n=16383, seed=202))
-        d = native.Disk(b, integrator=4, adaptive=True, tolerance=1e-12, dt_day=0.01, use_graph=True,
+        tol = 1e-12 if tol is None else tol
+        d = native.Disk(b, integrator=4, adaptive=True, tolerance=tol, dt_day=0.01, use_graph=True,
                         radii_enhance_factor=1.0, encounter_hill_factor=3.0, ejection=100.0, event_capacity=65536)
-        label = ("1 star + 16383 protoplanets/planetesimals (power-law masses, seed 202), RKF7(8) tol 1e-12, "
+        label = (f"1 star + 16383 protoplanets/planetesimals (power-law masses, seed 202), RKF7(8) tol {tol:g}, "
                  "FP64 sequential sums, collisions 1x(R_i+R_j) merged, close encounters at 3 mutual Hill radii, "
                  "ejection 100 AU")
     if report_years is None:
@@ -354,6 +357,8 @@ def main():
                          "disk's energy drift over --drift-years, not part of the default run)")
     ap.add_argument("--drift-years", type=float, default=1000.0)
     ap.add_argument("--drift-disk", choices=["powerlaw", "mixed", "mixed-disk-fp64"], default="powerlaw")
+    ap.add_argument("--drift-tol", type=float, default=None,
+                    help="step tolerance of the drift run (default 1e-12 for powerlaw, 1e-10 otherwise)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -373,7 +378,7 @@ def main():
         print(json.dumps({"ensemble": ensemble_section(native, args.ensemble_days, 16)}))
         return
     if args.only == "drift":
-        print(json.dumps({"drift": drift_section(native, args.drift_years, args.drift_disk)}))
+        print(json.dumps({"drift": drift_section(native, args.drift_years, args.drift_disk, tol=args.drift_tol)}))
         return
 
     bodies = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))

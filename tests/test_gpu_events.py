@@ -114,6 +114,31 @@ def test_driver_loop_mixed_events_match_fp64(native_mod):
     assert [key(e) for e in d64.raw_events()] == [key(e) for e in dmx.raw_events()]
 
 
+def test_mixed_close_encounter_mode_matches_fp64(oracle_mod, native_mod):
+    """Close-encounter mode in mixed precision: on one state the CLOSE_ENCOUNTER records
+    (pair order as found, stage, orbital elements) equal the FP64 path's bit for bit, and
+    after the check both equal the oracle's (12 pairs within 2500 (R_i + R_j) on this
+    state).  The raw records are compared GPU against GPU only: the reference's CPU sweep
+    stores the pair's parameters at detection (pp_disk.cu:515-516), its GPU kernel leaves
+    them to the check (pp_disk.cu:33-66), and the oracle follows the CPU sweep."""
+    b = planted_disk(seed=47)
+    kw = dict(integrator=2, dt_day=0.5, close_encounter=True, radii_enhance_factor=2500.0)
+    o = oracle_mod.Oracle(b, **kw)
+    d64 = native_mod.Disk(b, event_capacity=20000, **kw)
+    dmx = native_mod.Disk(b, precision=native_mod.PREC_MIXED, event_capacity=20000, **kw)
+    print(f"case mixed close-encounter mode n={b.n}", flush=True)
+    for d in (o, d64, dmx):
+        d.accel(0.0, b.r, b.v, stage=0)
+    e64 = d64.raw_events()
+    assert len(e64) == len(o.raw_events()) == 12 and all(e["name"] == 3 for e in e64)
+    events_equal(e64, dmx.raw_events(), "mixed same state")
+    assert o.check_for_close_encounter() and d64.check_for_close_encounter() and dmx.check_for_close_encounter()
+    sp = o.sp_events()
+    assert len(sp) == 12 and all(e["oe1"][0] != 0.0 for e in sp)
+    events_equal(sp, d64.sp_events(), "fp64 sp events")
+    events_equal(sp, dmx.sp_events(), "mixed sp events")
+
+
 def test_hill_close_encounters_reported(oracle_mod, native_mod):
     # pairs within 10 mutual Hill radii in the ring (pairs with the star are never tested)
     b = planted_disk(seed=44)
