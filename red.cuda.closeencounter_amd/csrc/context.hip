@@ -208,6 +208,15 @@ void refresh_pair_test(red_ctx* c)
     }
     P.rmax_all = rall;
     P.mmax_ns = mns;
+    // ids only turn negative on the device between refreshes, so the host's active set
+    // holds the device's and the flag stays conservative
+    P.mass_core = 1;
+    for (int i = 0; i < c->nMassive(); i++) {
+        if (c->h_id[i] < 0) continue;
+        uint64_t bits;
+        memcpy(&bits, &c->h_mass[i], sizeof(bits));
+        if ((uint32_t)((uint32_t)(bits >> 32) - 0x2D300000u) >= 0x25800000u) P.mass_core = 0;
+    }
     P.close_encounter = c->prm.close_encounter;
     P.n_star = c->cnt[0];
     P.enabled = c->prm.inner_steps && (P.radii_f > 0.0 || P.hill_f > 0.0);
@@ -258,8 +267,11 @@ hipError_t timed(red_ctx* c, int cls, F fn, hipStream_t s = nullptr)
 
 // Force evaluation of stage k on state (r, v) into a: the SI, NSI and NI sweeps
 // (pp_disk.cu:609-681).
-hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const d4* v, d4* a)
+hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const d4* v, d4* a, bool own_state = true)
 {
+    // caller-owned positions carry masses the context has not seen
+    PairTest P = c->ptest;
+    if (!own_state) P.mass_core = 0;
     GravBuffers B;
     B.r = r; B.v = v; B.a = a;
     B.mass = c->d_mass; B.radius = c->d_radius; B.id = c->d_id; B.ctl = c->d_ctl;
@@ -294,7 +306,7 @@ hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const
     for (int k = 0; k < 3; k++) {
         if (sw[k].sink1 <= sw[k].sink0) continue;
         hipStream_t st = (fork && k == 0) ? c->side : c->stream;
-        RED_HIP_TRY(timed(c, RED_PROF_SI + k, [&] { return launch_sweep(sw[k], B, c->gcfg, c->ptest, E, st); }, st));
+        RED_HIP_TRY(timed(c, RED_PROF_SI + k, [&] { return launch_sweep(sw[k], B, c->gcfg, P, E, st); }, st));
     }
     if (fork) {
         RED_HIP_TRY(hipEventRecord(c->ev_join, c->side));
@@ -784,7 +796,7 @@ int red_accel(red_ctx* c, double t, int stage, const double* r_dev, const double
     hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, c->stream, c->d_ctl, t, 0.0, 0, 1, 0, 0);
     CTX_HIP(c, hipGetLastError());
     c->raw_valid = false;
-    CTX_HIP(c, eval_forces(c, stage, 0.0, (const d4*)r_dev, (const d4*)v_dev, (d4*)a_dev));
+    CTX_HIP(c, eval_forces(c, stage, 0.0, (const d4*)r_dev, (const d4*)v_dev, (d4*)a_dev, false));
     return RED_OK;
 }
 

@@ -323,6 +323,8 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
 // chunks (fewer sinks per block) spread the per-chunk barrier and producer latency over
 // more sources; the tile stays within 2 x 1024 x 3 doubles.
 constexpr int SQ_TILE = 1024;         // sinks x sources per chunk, at most
+constexpr int BUF_RSRC_DW3 = 0x00020000;   // raw buffer descriptor word 3 on CDNA (32-bit data format)
+constexpr long SEQ_MAX_SOURCES = (1L << 31) / 32;   // buffer byte offsets stay below 2^31
 
 template <bool OVERLAP, bool EV, int NP, int SQ_SG>
 __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __restrict__ r, const d4* __restrict__ v,
@@ -357,18 +359,31 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
         const bool live = sink_in && id[i] > 0;
         const double Ri = EV ? radius[i] : 0.0;
         const double scr = EV ? pair_screen(P, Ri, mass[i], p) : 0.0;
+        // Sources reach the lanes through buffer loads: a wave-uniform chunk offset plus the
+        // lane's fixed byte offset, with no address arithmetic per chunk.  The buffers end at
+        // the sweep's last source, so slots past it in the last chunk read zeros (they are
+        // never used); the host keeps nsrc * 32 bytes below 2^31.
+        const __amdgpu_buffer_rsrc_t rs_r =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(r + sw.src0), 0, nsrc * (int)sizeof(d4), BUF_RSRC_DW3);
+        const __amdgpu_buffer_rsrc_t rs_i =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(id + sw.src0), 0, nsrc * (int)sizeof(int), BUF_RSRC_DW3);
+        unsigned off_r[M], off_i[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+            off_r[m] = (unsigned)(slot + m * SLOTS) * (unsigned)sizeof(d4);
+            off_i[m] = (unsigned)(slot + m * SLOTS) * (unsigned)sizeof(int);
+        }
         // raw ids are kept: comparing them here would wait for the loads
         auto fetch = [&](int c, d4* qd, int* idd) {
-            const int j0 = sw.src0 + c * SQ_J;
-            const int nj = min(SQ_J, sw.src1 - j0);
+            const int sb = c * SQ_J;
 #pragma unroll
             for (int m = 0; m < M; ++m) {
-                // unconditional (clamped) loads: a masked load would make the compiler wait
-                // for the data to merge it; slots past the chunk are never used
-                const int jj = slot + m * SLOTS;
-                const int j = j0 + (jj < nj ? jj : 0);
-                qd[m] = r[j];
-                idd[m] = id[j];
+                const double2 lo = __builtin_bit_cast(
+                    double2, __builtin_amdgcn_raw_buffer_load_b128(rs_r, off_r[m], sb * (int)sizeof(d4), 0));
+                const double2 hi = __builtin_bit_cast(
+                    double2, __builtin_amdgcn_raw_buffer_load_b128(rs_r, off_r[m] + 16u, sb * (int)sizeof(d4), 0));
+                qd[m] = d4{lo.x, lo.y, hi.x, hi.y};
+                idd[m] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_i, off_i[m], sb * (int)sizeof(int), 0);
             }
         };
         // one term: formed for every slot (the quotient is masked afterwards, so the M
@@ -406,7 +421,14 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
                 r2[m] = (dx[m] * dx[m] + dy[m] * dy[m]) + dz[m] * dz[m];
                 const bool act = qid[m] >= 0;
                 use[m] = sink_in && (OVERLAP ? (act && j != i) : act);
-                ok = ok && (!use[m] || (in_core_range(r2[m]) && in_core_range(q[m].w)));
+            }
+            // source masses need no test per pair when the host has found them all in range
+            if (P.mass_core) {
+#pragma unroll
+                for (int m = 0; m < M; ++m) ok = ok && (!use[m] || in_core_range(r2[m]));
+            } else {
+#pragma unroll
+                for (int m = 0; m < M; ++m) ok = ok && (!use[m] || (in_core_range(r2[m]) && in_core_range(q[m].w)));
             }
             if (!__all(ok)) return false;
 #pragma unroll
@@ -478,7 +500,10 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
         }
         return;
     }
-    // consumer wave
+    // consumer wave: its additions are the block's serial chain, each waiting on the last,
+    // so it issues ahead of the producer waves on its SIMD whenever its next addition is
+    // ready (the producers fill the cycles in between)
+    __builtin_amdgcn_s_setprio(3);
     __syncthreads();
     for (int c = 0; c < nch; ++c) {
         const int b = c & 1;
@@ -1021,13 +1046,14 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
         return hipGetLastError();
     }
     const bool use_split = sweep_wants_split(sw, C) && split_scratch_bytes(sw) <= B.scratch_bytes;
-    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0 && !use_split) {
+    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0 && nsrc < SEQ_MAX_SOURCES &&
+        !use_split) {
         const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
         const bool ev = P.enabled != 0;
         // sinks per block (chunk length 1024 / sg) and producer waves per block: as asked for,
         // else long chunks for few sinks (the serial chain per sink is the critical path) and
         // many producers when there are few blocks to spread over the chip
-        const int sg = C.seq_sinks > 0 ? C.seq_sinks : (nsink <= 4096 ? 4 : 8);
+        const int sg = C.seq_sinks > 0 ? C.seq_sinks : (nsink <= 4096 ? 4 : 16);
         const int nblk = (nsink + sg - 1) / sg;
         const int np = C.block_seq > 0 ? C.block_seq / 64 - 1 : (nblk >= 128 ? 7 : 15);
         if (ov) return ev ? go_fp64_seq<true, true>(np, sg, sw, B, P, E, s) : go_fp64_seq<true, false>(np, sg, sw, B, P, E, s);

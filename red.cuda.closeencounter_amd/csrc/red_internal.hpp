@@ -55,6 +55,8 @@ struct PairTest {
     double rmax_all;               // largest radius of any body (FP64 screen)
     double mmax_ns;                // largest mass of any non-star body (FP64 screen)
     int n_star;                    // bodies [0, n_star) are stars: no Hill test with them
+    int mass_core;                 // every active massive body's mass (the w lane of the
+                                   // swept positions) lies in [2^-300, 2^300]
     int close_encounter;           // 1: report CLOSE_ENCOUNTER, keep (i, j) order
     int enabled;                   // any pair test active
 };
