@@ -352,6 +352,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-days", type=float, default=100.0)
     ap.add_argument("--no-graph", action="store_true", help="proposer section without HIP graphs")
+    ap.add_argument("--no-aux", action="store_true",
+                    help="skip the mixed-vs-FP64 accuracy and reference-workload sections (for kernel traces "
+                         "of the main sections alone)")
     ap.add_argument("--only", choices=["proposer", "ensemble", "drift"], default=None,
                     help="run one auxiliary section alone and print its JSON (drift: the 16384-body "
                          "disk's energy drift over --drift-years, not part of the default run)")
@@ -408,8 +411,8 @@ def main():
         achieved64 = FLOP_PER_PAIR * ni_pairs / ((ni64_ms / max(ni64_n, 1)) * 1e-3) / 1e12
     traffic, traffic_src = load_traffic(args.pmc_summary)
 
-    acc = None if args.no_fp64 else accuracy_section(native, bodies)
-    refw = None if args.no_fp64 else reference_workload_section(native)
+    acc = None if args.no_fp64 or args.no_aux else accuracy_section(native, bodies)
+    refw = None if args.no_fp64 or args.no_aux else reference_workload_section(native)
     cpu = None if args.no_cpu else cpu_baseline(bodies)
     prop = None if args.no_proposer else proposer_section(native, args.proposer_years)
     ens = None if args.no_ensemble else ensemble_section(native, args.ensemble_days, 16)

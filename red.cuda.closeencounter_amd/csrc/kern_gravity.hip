@@ -1051,9 +1051,10 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
         const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
         const bool ev = P.enabled != 0;
         // sinks per block (chunk length 1024 / sg) and producer waves per block: as asked for,
-        // else long chunks for few sinks (the serial chain per sink is the critical path) and
+        // else long chunks for few sinks (the serial chain per sink is the critical path), and
+        // 16 sinks (fewer consumer additions per term) once that still gives 3 blocks per CU;
         // many producers when there are few blocks to spread over the chip
-        const int sg = C.seq_sinks > 0 ? C.seq_sinks : (nsink <= 4096 ? 4 : 16);
+        const int sg = C.seq_sinks > 0 ? C.seq_sinks : (nsink <= 4096 ? 4 : (nsink >= 12288 ? 16 : 8));
         const int nblk = (nsink + sg - 1) / sg;
         const int np = C.block_seq > 0 ? C.block_seq / 64 - 1 : (nblk >= 128 ? 7 : 15);
         if (ov) return ev ? go_fp64_seq<true, true>(np, sg, sw, B, P, E, s) : go_fp64_seq<true, false>(np, sg, sw, B, P, E, s);

@@ -145,6 +145,61 @@ def test_mixed_accuracy_and_events(oracle_mod, native_mod, kahan):
 
 
 
+def _extreme_disk(m_lo, m_hi, mirrored=False, seed=77):
+    """Star + four groups of 40 bodies at separations of 1e-45, 1e-30, 1 and 1.5e45 AU with
+    log-uniform masses in [m_lo, m_hi].  mirrored: every body has a twin at -r with the
+    same mass next to it, so the barycentre is exactly the origin and the barycentric
+    transform leaves the positions unchanged."""
+    from rednative import files
+    rng = np.random.default_rng(seed)
+    groups = [
+        (1e-45, 40),    # mutual r^2 ~ 1e-90, around 2^-300
+        (1e-30, 40),    # r^2 ~ 1e-60: inside
+        (1.0, 40),      # ordinary
+        (1.5e45, 40),   # r^2 ~ 1e90 .. 1e91, around 2^300
+    ]
+    r = [np.array([0.0, 0.0, 0.0]) if mirrored else np.array([3.0, 0.0, 0.0])]
+    for scale, k in groups:
+        if mirrored:
+            for x in rng.uniform(-1.0, 1.0, (k // 2, 3)) * scale:
+                r += [x, -x]
+        else:
+            r += list(rng.uniform(-1.0, 1.0, (k, 3)) * scale)
+    r = np.array(r)
+    n = len(r)
+    m = np.exp(rng.uniform(np.log(m_lo), np.log(m_hi), n))
+    if mirrored:
+        m[2::2] = m[1::2]
+    m[0] = 1.0
+    types = np.full(n, 3, dtype=np.int32); types[0] = 0
+    counts = [1, 0, 0, n - 1, 0, 0, 0]
+    return files.BodySet(counts, np.arange(1, n + 1, dtype=np.int32), [f"b{k}" for k in range(n)], types,
+                         np.zeros(n), m, np.full(n, 1e-300), np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int32),
+                         np.zeros(n), r, np.zeros((n, 3)))
+
+
+@pytest.mark.parametrize("m_lo", [1e-95, 1e-20])
+def test_fp64_extreme_scales_step_bit_exact(oracle_mod, native_mod, m_lo):
+    """An Euler step sweeps the context's own state, whose source masses the host checks
+    once: with masses down to 1e-95 (below 2^-300) every pair tests its mass, with masses
+    in [1e-20, 1e20] only the separations are tested.  Either way the step equals the
+    oracle's bit for bit."""
+    b = _extreme_disk(m_lo, 1e20, mirrored=True)
+    print(f"case extreme scales step m_lo={m_lo} n={b.n}", flush=True)
+    o = oracle_mod.Oracle(b, integrator=0, dt_day=0.01)
+    so0 = o.state()
+    assert_bitwise(so0["r"], b.r, "barycentric transform is the identity")
+    for spt, block in ((4, 512), (16, 1024)):
+        o = oracle_mod.Oracle(b, integrator=0, dt_day=0.01)
+        d = native_mod.Disk(b, integrator=0, dt_day=0.01, strategy=3, sinks_per_thread=spt, block=block)
+        assert o.step() == d.step()
+        so, sg = o.state(), d.state()
+        assert np.all(np.isfinite(so["v"])) and np.any(so["v"] != 0.0)
+        assert_bitwise(sg["v"], so["v"], f"v sg={spt}")
+        assert_bitwise(sg["r"], so["r"], f"r sg={spt}")
+        d.close()
+
+
 def test_fp64_extreme_scales_bit_exact(oracle_mod, native_mod):
     """Separations and masses across and around 2^-300 .. 2^300, where the producer/consumer
     sweep switches between its core square-root / quotient sequences and the full IEEE ones
