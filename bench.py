@@ -276,6 +276,57 @@ def drift_section(native, years, disk="powerlaw", report_years=None, tol=None):
     return out
 
 
+def small_disk_cpu_baseline(bodies, kw, prm):
+    """CPU leg of the small-disk section: the oracle (one thread, like the reference's CPU
+    path) through the same main loop; returns the wall time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    from rednative import mainloop
+    o = pyoracle.Oracle(bodies, **kw)
+    t0 = time.perf_counter()
+    mainloop.run_main_loop(o, prm, lambda e: None)
+    return time.perf_counter() - t0
+
+
+def small_disk_section(native, with_cpu=True):
+    """The 256-body workload at its full length: star + 2 giants + 253 planetesimals (seed
+    101), RK4 at dt = 1 day, 10^4 steps of the reference's main loop (ejection at 100 AU,
+    collisions at 1 x (R_i + R_j), a snapshot read back every 365.25 days).  The GPU runs it
+    with the fused fixed-step loop (red_run_fixed) and one step at a time (HIP graphs); the
+    final states must be identical."""
+    import numpy as np
+    from rednative import generate, mainloop
+    b = generate.to_internal(generate.disk_small(seed=101))
+    kw = dict(integrator=2, adaptive=False, dt_day=1.0, ejection=100.0, radii_enhance_factor=1.0)
+    prm = dict(stop_time=9999.5 * GAUSS, output_interval=365.25 * GAUSS, inner_steps=True, radii_enhance_factor=1.0)
+    out = {"workload": "1 star + 2 giants + 253 planetesimals (seed 101), RK4 dt = 1 day, 10^4 steps, "
+                       "ejection 100 AU, collisions 1x(R_i+R_j), snapshot every 365.25 days"}
+    states = {}
+    for mode in ("fused", "stepwise"):
+        d = native.Disk(b, use_graph=True, **kw)
+        mainloop.run_main_loop(d, dict(prm, stop_time=20.5 * GAUSS), lambda e: None, fused=(mode == "fused"))
+        d = native.Disk(b, use_graph=True, **kw)
+        d.sync()
+        snaps = []
+        t0 = time.perf_counter()
+        info = mainloop.run_main_loop(d, prm, lambda e: snaps.append(e.t), fused=(mode == "fused"))
+        d.sync()
+        sec = time.perf_counter() - t0
+        s = d.state()
+        states[mode] = (s["r"].copy(), s["v"].copy())
+        out[mode] = {"seconds": sec, "us_per_step": sec / info["steps"] * 1e6, "steps": info["steps"],
+                     "snapshots": len(snaps), "sim_yr_per_hr": (d.t / GAUSS / 365.25) / (sec / 3600.0)}
+        d.close()
+    out["identical"] = bool(all(np.array_equal(states["fused"][k].view(np.uint64), states["stepwise"][k].view(np.uint64))
+                                for k in range(2)))
+    if with_cpu:
+        sec = small_disk_cpu_baseline(b, kw, prm)
+        out["cpu_baseline"] = {"seconds": sec, "us_per_step": sec / 1e4 * 1e6, "cores": 1, "kind": "port"}
+        out["fused_vs_cpu"] = sec / out["fused"]["seconds"]
+    out["fused_vs_stepwise"] = out["stepwise"]["seconds"] / out["fused"]["seconds"]
+    return out
+
+
 def ensemble_section(native, days, sample):
     """512 disks (N log-uniform in [64, 4096], seeded) integrated together against the same
     disks stepped one context at a time (a sample of them, scaled): disk-steps per second."""
@@ -355,7 +406,7 @@ def main():
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the mixed-vs-FP64 accuracy and reference-workload sections (for kernel traces "
                          "of the main sections alone)")
-    ap.add_argument("--only", choices=["proposer", "ensemble", "drift"], default=None,
+    ap.add_argument("--only", choices=["proposer", "ensemble", "drift", "smalldisk"], default=None,
                     help="run one auxiliary section alone and print its JSON (drift: the 16384-body "
                          "disk's energy drift over --drift-years, not part of the default run)")
     ap.add_argument("--drift-years", type=float, default=1000.0)
@@ -379,6 +430,9 @@ def main():
         return
     if args.only == "ensemble":
         print(json.dumps({"ensemble": ensemble_section(native, args.ensemble_days, 16)}))
+        return
+    if args.only == "smalldisk":
+        print(json.dumps({"small_disk": small_disk_section(native, with_cpu=not args.no_cpu)}))
         return
     if args.only == "drift":
         print(json.dumps({"drift": drift_section(native, args.drift_years, args.drift_disk, tol=args.drift_tol)}))
@@ -416,6 +470,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(bodies)
     prop = None if args.no_proposer else proposer_section(native, args.proposer_years)
     ens = None if args.no_ensemble else ensemble_section(native, args.ensemble_days, 16)
+    small = None if args.no_aux else small_disk_section(native, with_cpu=not args.no_cpu)
 
     line = {
         "metric": "pair-interactions/s & sim-yr/hr at N=131072, mixed vs FP64, ΔE/E≤1e-9",
@@ -456,6 +511,7 @@ def main():
         "cpu_baseline": cpu,
         "proposer": prop,
         "ensemble": ens,
+        "small_disk": small,
     }
     print(json.dumps(line))
 

@@ -175,6 +175,28 @@ int red_sync(red_ctx* ctx);
  * int_rungekutta5.cu:202-282, int_rungekutta8.cu:704-797). */
 int red_step(red_ctx* ctx, red_step_info* info);
 
+/* Fixed-step main-loop iterations on the device.  No single counterpart in the reference:
+ * it runs, for fixed-step integrators (Euler, RK2, RK4, and RK5 / RKF7(8) without step
+ * control), the iterations of the driver loop main.cu:159-281 in which nothing is left for
+ * the host -- the ejection / hit-centrum check finds no body, the step, *ps += |dt|, and no
+ * pair event was recorded -- in one work-group of one kernel, with the same operations in
+ * the same order as red_step, so the state stays bit-identical to stepping one at a time.
+ * Before each iteration it stops (reason) when t > t_stop (RED_FUSED_STOP_TIME),
+ * output_interval <= |*ps| (RED_FUSED_STOP_OUTPUT), max_steps steps are done
+ * (RED_FUSED_STOP_STEPS) or the ejection / hit-centrum check would find a body
+ * (RED_FUSED_STOP_EJECTION: the caller runs that iteration itself); after a step that
+ * recorded pair events it stops with RED_FUSED_STOP_EVENTS (the step is done; the caller
+ * runs the collision / close-encounter and rebuild checks).  *ps is the driver's
+ * accumulated step (in and out).  For disks of at most 4096 bodies with FP64 sums in the
+ * reference order and the barycentric frame; RED_ERR_UNSUPPORTED otherwise. */
+#define RED_FUSED_STOP_STEPS    1
+#define RED_FUSED_STOP_TIME     2
+#define RED_FUSED_STOP_OUTPUT   3
+#define RED_FUSED_STOP_EJECTION 4
+#define RED_FUSED_STOP_EVENTS   5
+int red_run_fixed(red_ctx* ctx, int max_steps, double t_stop, double output_interval, double* ps,
+                  int* steps_done, int* stop_reason);
+
 /* Event checks of the driver loop (main.cu:201-255).  *handled = 1 when events were
  * found and handled.  Replace pp_disk::check_for_ejection_hit_centrum (pp_disk.cu:683-715),
  * check_for_collision (:717-738), check_for_close_encounter(true) (:740-770) and

@@ -97,6 +97,8 @@ struct red_ctx {
     int updates_cap = 0;
     double* d_energy = nullptr;
     double* d_energy_part = nullptr;      // potential partials, n_si x chunks
+    StageArgs* d_fused_rows = nullptr;    // fused fixed-step loop: stage arguments, both parities
+    FusedOut* d_fused_out = nullptr;
     size_t energy_part_cap = 0;
     // integrator state (integrator.h:359-383)
     Scheme S;
@@ -556,6 +558,7 @@ int red_ctx_destroy(red_ctx* c)
     (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy); (void)hipFree(c->d_energy_part);
     (void)hipFree(c->d_partial); (void)hipFree(c->d_seg);
     (void)hipFree(c->d_learn_part); (void)hipFree(c->d_learn_counter);
+    (void)hipFree(c->d_fused_rows); (void)hipFree(c->d_fused_out);
     for (void* p : c->user_buffers) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->h_prop_out) (void)hipHostFree(c->h_prop_out);
@@ -1035,6 +1038,81 @@ int red_step(red_ctx* c, red_step_info* info)
         info->err = max_err;
         info->tries = iter;
         info->n_events = adaptive ? c->h_ctl->n_events : -1;
+    }
+    return RED_OK;
+}
+
+int red_run_fixed(red_ctx* c, int max_steps, double t_stop, double output_interval, double* ps, int* steps_done,
+                  int* stop_reason)
+{
+    if (!c || !ps || !steps_done || !stop_reason) return RED_ERR_ARG;
+    *steps_done = 0;
+    *stop_reason = 0;
+    const Scheme& S = c->S;
+    if (c->prm.adaptive && S.err_form >= 0)
+        return c->fail(RED_ERR_UNSUPPORTED, "red_run_fixed: the integrator controls its step size");
+    if (c->gcfg.precision != RED_PREC_FP64 || c->gcfg.fp64_split || c->gcfg.astro)
+        return c->fail(RED_ERR_UNSUPPORTED,
+                       "red_run_fixed: needs FP64 sums in the reference order and the barycentric frame");
+    if (c->n > FUSED_MAX_BODIES || S.n_main > FUSED_MAX_ROWS)
+        return c->fail(RED_ERR_UNSUPPORTED, "red_run_fixed: more than 4096 bodies");
+    if (!c->d_fused_rows || !c->d_fused_out) {
+        SetupLock lk;
+        if (!c->d_fused_rows) CTX_HIP(c, dmalloc(&c->d_fused_rows, 2 * FUSED_MAX_ROWS));
+        if (!c->d_fused_out) CTX_HIP(c, dmalloc(&c->d_fused_out, 1));
+    }
+    // stage arguments of both state-buffer parities (stage 0 reads y_n of that parity):
+    // rows 1 .. n_main-1 into S_k, then y_(n+1) into the other buffer, as enqueue_attempt
+    FusedArgs A;
+    memset(&A, 0, sizeof(A));
+    std::vector<StageArgs> rows(2 * S.n_main);
+    const int cur0 = c->cur;
+    for (int p = 0; p < 2; p++) {
+        c->cur = p;
+        for (int r = 1; r < S.n_main; r++)
+            rows[p * S.n_main + r - 1] = make_args(c, S.rows[r], c->d_r[p], c->d_v[p], c->d_sr[r], c->d_sv[r], c->n);
+        rows[p * S.n_main + S.n_main - 1] = make_args(c, S.np1, c->d_r[p], c->d_v[p], c->d_r[1 - p], c->d_v[1 - p], c->n);
+    }
+    c->cur = cur0;
+    for (int r = 1; r < S.n_main; r++) A.form[r - 1] = S.rows[r].form;
+    A.form[S.n_main - 1] = S.np1.form;
+    CTX_HIP(c, hipMemcpyAsync(c->d_fused_rows, rows.data(), sizeof(StageArgs) * rows.size(), hipMemcpyHostToDevice,
+                              c->stream));
+    for (int k = 0; k < 2; k++) { A.r[k] = c->d_r[k]; A.v[k] = c->d_v[k]; }
+    for (int k = 0; k < S.n_main; k++) { A.a[k] = c->d_a[k]; A.c_stage[k] = S.c[k]; }
+    A.rows = c->d_fused_rows;
+    A.n_main = S.n_main;
+    A.clear_events_after_f0 = S.clear_events_each_attempt ? 1 : 0;
+    sweeps_of(c, A.sw);
+    A.n = c->n;
+    A.mass = c->d_mass; A.radius = c->d_radius; A.id = c->d_id; A.type = c->d_type;
+    A.ej = c->prm.ejection; A.hc = c->prm.hit_centrum;
+    A.ev_buf = c->d_events; A.ev_cap = c->ev_cap;
+    A.ctl = c->d_ctl;
+    A.dt = c->dt_try;
+    A.t_stop = t_stop;
+    A.interval = output_interval;
+    A.out = c->d_fused_out;
+    c->raw_valid = false;
+    // bounded launches: each runs at most FUSED_STEPS_PER_LAUNCH steps
+    while (true) {
+        A.t = c->t;
+        A.ps = *ps;
+        A.cur = c->cur;
+        A.max_steps = std::min(FUSED_STEPS_PER_LAUNCH, max_steps - *steps_done);
+        CTX_HIP(c, launch_fused_fixed(A, c->ptest, c->stream));
+        FusedOut o;
+        CTX_HIP(c, stream_copy(c->stream, &o, c->d_fused_out, sizeof(o), hipMemcpyDeviceToHost));
+        // integrator.cu:258-263 and pp_disk::swap for every step done
+        c->t = o.t;
+        c->cur = o.cur;
+        c->dt_did = c->dt_try;
+        c->n_tried += o.steps;
+        c->n_passed += o.steps;
+        *ps = o.ps;
+        *steps_done += o.steps;
+        *stop_reason = o.reason;
+        if (o.reason != RED_FUSED_STOP_STEPS || *steps_done >= max_steps) break;
     }
     return RED_OK;
 }

@@ -28,11 +28,9 @@ __global__ __launch_bounds__(256) void k_ejection(const d4* __restrict__ r, cons
     if (i >= n) return;
     if (!(id[i] > 0 && type[i] != 0)) return;
     const d4 q = r[i];
-    const double r2 = (q.x * q.x) + (q.y * q.y) + (q.z * q.z);
-    int name = RED_EV_NONE;
-    if (0.0 < ej && ej * ej < r2) name = RED_EV_EJECTION;
-    else if (0.0 < hc && hc * hc > r2) name = RED_EV_HIT_CENTRUM;
+    const int name = ejection_name(q, ej, hc);
     if (name == RED_EV_NONE) return;
+    const double r2 = (q.x * q.x) + (q.y * q.y) + (q.z * q.z);
     const int k = atomicAdd(&ctl->n_events_hc, 1);
     if (k < cap) {
         red_event* e = buf + k;

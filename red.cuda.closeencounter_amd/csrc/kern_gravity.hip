@@ -33,6 +33,7 @@
 #include <algorithm>
 
 #include "red_internal.hpp"
+#include "stage_forms.hpp"
 
 namespace red {
 namespace {
@@ -1103,6 +1104,376 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
         return hipGetLastError();
     }
     return hipSuccess;
+}
+
+}  // namespace red
+
+// ---------------------------------------------------------------------------------------
+// Fused fixed-step loop.  One work-group runs main-loop iterations of one small disk:
+// the ejection / hit-centrum predicate (detection only), f0, every stage state and its
+// forces, y_(n+1), then t += dt and ps += |dt|, with barriers where the stream of separate
+// launches has its kernel boundaries.  The stage states come from combine<FORM> and the
+// forces from the same pair terms as the sweep kernels, with every sink's sum in ascending
+// j, so the state equals that of red_step bit for bit.
+namespace red {
+namespace {
+
+constexpr int FUSED_BLOCK = 256;
+constexpr int FUSED_TILE = 1536;   // pair terms per LDS tile (3 doubles each: 36 KiB)
+constexpr int FUSED_TILE_SINKS = 16;
+
+// A source as a pair needs it, staged in LDS: position, mass, radius, activity.
+struct FusedSrc {
+    double x, y, z, m, R;
+    int act, pad;
+};
+// A sink of the few-sinks mode, staged in LDS.
+struct FusedSink {
+    double x, y, z, R;
+    int live, pad;
+};
+
+struct FusedLds {
+    double terms[3 * FUSED_TILE];
+    FusedSrc src[FUSED_TILE];
+    FusedSink sink[FUSED_TILE_SINKS];
+    int flag;
+};
+
+// U pair terms of one work-item in the reference's operation order (as fp64_pair), from
+// staged records, their dependency chains interleaved: the square roots and quotients go
+// through the core sequences when every active lane's operands are in range (the wave
+// votes), else through sqrt() and '/', with identical results.  Returns d and whether the
+// pair counts; the caller runs the pair tests afterwards (few values live across them).
+template <int U, bool OV>
+__device__ __forceinline__ void fused_terms(const FusedSrc* q, const int* j, const int* i, const double* px,
+                                            const double* py, const double* pz, const bool* val, double* tx,
+                                            double* ty, double* tz, double* d, bool* use)
+{
+    double dx[U], dy[U], dz[U], r2[U], qt[U];
+    bool ok = true;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        dx[u] = q[u].x - px[u];
+        dy[u] = q[u].y - py[u];
+        dz[u] = q[u].z - pz[u];
+        r2[u] = (dx[u] * dx[u] + dy[u] * dy[u]) + dz[u] * dz[u];
+        use[u] = val[u] && (OV ? (q[u].act && j[u] != i[u]) : q[u].act != 0);
+        ok = ok && (!use[u] || (in_core_range(r2[u]) && in_core_range(q[u].m)));
+    }
+    if (__all(ok)) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            d[u] = sqrt_core(r2[u]);
+            qt[u] = div_core(q[u].m, d[u] * r2[u]);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            d[u] = sqrt(r2[u]);
+            qt[u] = q[u].m / (d[u] * r2[u]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const double sc = use[u] ? qt[u] : 0.0;
+        tx[u] = sc * dx[u];
+        ty[u] = sc * dy[u];
+        tz[u] = sc * dz[u];
+    }
+}
+
+// Stages sources [j0, j0 + nj) of a sweep in LDS (one global round trip for the tile).
+__device__ __forceinline__ void fused_stage_sources(FusedLds& L, int j0, int nj, const d4* r, const double* radius,
+                                                    const int* id)
+{
+    for (int k = threadIdx.x; k < nj; k += FUSED_BLOCK) {
+        const d4 q = r[j0 + k];
+        L.src[k] = FusedSrc{q.x, q.y, q.z, q.w, radius[j0 + k], id[j0 + k] >= 0 ? 1 : 0, 0};
+    }
+}
+
+// One sweep by the whole work-group, every pair's operands staged in LDS.  Few sinks
+// against many sources: all work-items form the pair terms of a tile (up to 16 sinks x
+// FUSED_TILE / 16 sources) in LDS, then one lane per (sink, component) row adds them in
+// ascending j; otherwise one sink per work-item adds its terms as they are formed, over
+// source tiles.
+template <bool OV, bool EV>
+__device__ void fused_sweep(const Sweep& sw, const d4* r, const d4* v, d4* a, const double* mass,
+                            const double* radius, const int* id, const PairTest& P, const EventSink& E, Ctl* ctl,
+                            FusedLds& L)
+{
+    const int tid = threadIdx.x;
+    const int nsink = sw.sink1 - sw.sink0;
+    const int nsrc = sw.src1 - sw.src0;
+    if (nsrc > 64 && nsink * 4 <= FUSED_BLOCK) {
+        for (int g0 = 0; g0 < nsink; g0 += FUSED_TILE_SINKS) {
+            const int S = min(FUSED_TILE_SINKS, nsink - g0);
+            const int J = FUSED_TILE / S;
+            if (tid < S) {
+                const int i = sw.sink0 + g0 + tid;
+                const d4 p = r[i];
+                L.sink[tid] = FusedSink{p.x, p.y, p.z, EV ? radius[i] : 0.0, id[i] > 0 ? 1 : 0, 0};
+            }
+            double sum = 0.0;   // lane tid < 3 S: row (sink g0 + tid / 3, component tid % 3)
+            for (int j0 = 0; j0 < nsrc; j0 += J) {
+                const int nj = min(J, nsrc - j0);
+                fused_stage_sources(L, sw.src0 + j0, nj, r, radius, id);
+                __syncthreads();
+                constexpr int TU = 2;   // terms per work-item and pass
+                for (int k0 = 0; k0 < S * nj; k0 += TU * FUSED_BLOCK) {
+                    FusedSrc q[TU];
+                    int jg[TU], ig[TU], kk[TU], sk[TU];
+                    double px[TU], py[TU], pz[TU], tx[TU], ty[TU], tz[TU], d[TU];
+                    bool val[TU], use[TU];
+#pragma unroll
+                    for (int u = 0; u < TU; ++u) {
+                        kk[u] = k0 + u * FUSED_BLOCK + tid;
+                        val[u] = kk[u] < S * nj;
+                        const int kc = val[u] ? kk[u] : 0;
+                        sk[u] = kc / nj;
+                        const int jj = kc - sk[u] * nj;
+                        q[u] = L.src[jj];
+                        jg[u] = sw.src0 + j0 + jj;
+                        ig[u] = sw.sink0 + g0 + sk[u];
+                        px[u] = L.sink[sk[u]].x; py[u] = L.sink[sk[u]].y; pz[u] = L.sink[sk[u]].z;
+                    }
+                    fused_terms<TU, OV>(q, jg, ig, px, py, pz, val, tx, ty, tz, d, use);
+#pragma unroll
+                    for (int u = 0; u < TU; ++u) {
+                        if (val[u]) {
+                            double* t3 = L.terms + 3 * kk[u];
+                            t3[0] = tx[u]; t3[1] = ty[u]; t3[2] = tz[u];
+                        }
+                    }
+                    if (EV) {
+#pragma unroll
+                        for (int u = 0; u < TU; ++u)
+                            if (use[u] && L.sink[sk[u]].live)
+                                pair_test(P, E, ctl, ig[u], jg[u], d[u], L.sink[sk[u]].R, q[u].R, r, v, mass, id);
+                    }
+                }
+                __syncthreads();
+                if (tid < 3 * S) {
+                    // the row in ascending j; the next 16 terms are read while the current
+                    // 16 are added (two register sets), so the LDS latency stays off the
+                    // serial chain
+                    const int s = tid / 3;
+                    const double* t3 = L.terms + 3 * (s * nj) + (tid - 3 * s);
+                    constexpr int GQ = 16;
+                    const int ng = nj / GQ;
+                    double x[GQ], y[GQ];
+                    if (ng > 0) {
+#pragma unroll
+                        for (int k = 0; k < GQ; ++k) x[k] = t3[3 * k];
+                    }
+                    for (int g = 0; g < ng; g += 2) {
+                        if (g + 1 < ng) {
+#pragma unroll
+                            for (int k = 0; k < GQ; ++k) y[k] = t3[3 * ((g + 1) * GQ + k)];
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int k = 0; k < GQ; ++k) sum += x[k];
+                        __builtin_amdgcn_sched_barrier(0);
+                        if (g + 1 < ng) {
+                            if (g + 2 < ng) {
+#pragma unroll
+                                for (int k = 0; k < GQ; ++k) x[k] = t3[3 * ((g + 2) * GQ + k)];
+                            }
+                            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int k = 0; k < GQ; ++k) sum += y[k];
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
+                    }
+                    for (int jj = ng * GQ; jj < nj; ++jj) sum += t3[3 * jj];
+                }
+                __syncthreads();
+            }
+            if (tid < 3 * S) {
+                const int i = sw.sink0 + g0 + tid / 3;
+                const int comp = tid % 3;
+                double* o = reinterpret_cast<double*>(a + i);
+                o[comp] = L.sink[tid / 3].live ? sum : 0.0;
+                if (comp == 0) o[3] = 0.0;
+            }
+            __syncthreads();   // the sink records are rewritten by the next group
+        }
+        return;
+    }
+    for (int i0 = sw.sink0; i0 < sw.sink1; i0 += FUSED_BLOCK) {
+        const int i = i0 + tid;
+        const bool in = i < sw.sink1;
+        const int ic = in ? i : sw.sink0;
+        const d4 p = r[ic];
+        const bool live = in && id[ic] > 0;
+        const double Ri = EV ? radius[ic] : 0.0;
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        for (int j0 = sw.src0; j0 < sw.src1; j0 += FUSED_TILE) {
+            const int nj = min(FUSED_TILE, sw.src1 - j0);
+            fused_stage_sources(L, j0, nj, r, radius, id);
+            __syncthreads();
+            if (in) {
+                // four sources at a time, their terms added in ascending j
+                constexpr int TU = 4;
+                for (int k0 = 0; k0 < nj; k0 += TU) {
+                    FusedSrc q[TU];
+                    int jg[TU], ig[TU];
+                    double px[TU], py[TU], pz[TU], tx[TU], ty[TU], tz[TU], d[TU];
+                    bool val[TU], use[TU];
+#pragma unroll
+                    for (int u = 0; u < TU; ++u) {
+                        val[u] = k0 + u < nj;
+                        q[u] = L.src[val[u] ? k0 + u : k0];
+                        jg[u] = j0 + k0 + u;
+                        ig[u] = i;
+                        px[u] = p.x; py[u] = p.y; pz[u] = p.z;
+                    }
+                    fused_terms<TU, OV>(q, jg, ig, px, py, pz, val, tx, ty, tz, d, use);
+#pragma unroll
+                    for (int u = 0; u < TU; ++u) {
+                        if (val[u]) {
+                            ax += tx[u];
+                            ay += ty[u];
+                            az += tz[u];
+                        }
+                    }
+                    if (EV && live) {
+#pragma unroll
+                        for (int u = 0; u < TU; ++u)
+                            if (use[u]) pair_test(P, E, ctl, i, jg[u], d[u], Ri, q[u].R, r, v, mass, id);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (in) a[i] = d4{live ? ax : 0.0, live ? ay : 0.0, live ? az : 0.0, 0.0};
+    }
+}
+
+// The three sweeps of one force evaluation (as eval_forces launches them).
+__device__ void fused_forces(const FusedArgs& A, const PairTest& P, int stage, const d4* r, const d4* v, d4* a,
+                             FusedLds& L)
+{
+    EventSink E;
+    E.buf = A.ev_buf;
+    E.cap = A.ev_cap;
+    E.stage = stage;
+    E.c_stage = A.c_stage[stage];
+    for (int k = 0; k < 3; ++k) {
+        const Sweep sw = A.sw[k];
+        if (sw.sink1 <= sw.sink0) continue;
+        const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
+        if (P.enabled) {
+            if (ov) fused_sweep<true, true>(sw, r, v, a, A.mass, A.radius, A.id, P, E, A.ctl, L);
+            else fused_sweep<false, true>(sw, r, v, a, A.mass, A.radius, A.id, P, E, A.ctl, L);
+        } else {
+            if (ov) fused_sweep<true, false>(sw, r, v, a, A.mass, A.radius, A.id, P, E, A.ctl, L);
+            else fused_sweep<false, false>(sw, r, v, a, A.mass, A.radius, A.id, P, E, A.ctl, L);
+        }
+    }
+}
+
+// One body of a stage combination (k_stage's work for body i).
+template <int FORM>
+__device__ __forceinline__ void fused_stage_body(const StageArgs& S, int i, double dt)
+{
+    const d4 yr = S.yr[i];
+    const d4 yv = S.yv[i];
+    d4 orr = combine<FORM>(S, yr, i, false, dt);
+    d4 ovv = combine<FORM>(S, yv, i, true, dt);
+    orr.w = yr.w;
+    ovv.w = 0.0;
+    S.outr[i] = orr;
+    S.outv[i] = ovv;
+}
+
+__device__ void fused_stage(int form, const StageArgs& S, int n, double dt)
+{
+    for (int i = threadIdx.x; i < n; i += FUSED_BLOCK) {
+        switch (form) {
+        case SF_AXPY: fused_stage_body<SF_AXPY>(S, i, dt); break;
+        case SF_DTSUM: fused_stage_body<SF_DTSUM>(S, i, dt); break;
+        case SF_ACCUM: fused_stage_body<SF_ACCUM>(S, i, dt); break;
+        case SF_RK4NP1: fused_stage_body<SF_RK4NP1>(S, i, dt); break;
+        default: fused_stage_body<SF_RKF78NP1>(S, i, dt); break;
+        }
+    }
+}
+
+// Every loop decision is taken by work-item 0 and read by all through LDS between barriers,
+// so the whole work-group leaves the loop together; the loop ends after at most
+// A.max_steps steps.
+__global__ __launch_bounds__(FUSED_BLOCK) void k_fused_fixed(FusedArgs A, PairTest P)
+{
+    __shared__ FusedLds L;
+    __shared__ StageArgs rows_l[2 * FUSED_MAX_ROWS];
+    int& flag = L.flag;
+    {
+        // the stage arguments of both parities, read once from global memory
+        const int words = 2 * A.n_main * (int)(sizeof(StageArgs) / sizeof(int));
+        const int* src = reinterpret_cast<const int*>(A.rows);
+        int* dst = reinterpret_cast<int*>(rows_l);
+        for (int w = threadIdx.x; w < words; w += FUSED_BLOCK) dst[w] = src[w];
+        __syncthreads();
+    }
+    const int tid = threadIdx.x;
+    double t = A.t, ps = A.ps;
+    int cur = A.cur, steps = 0, reason = 0;
+    const bool scan = 0.0 < A.ej || 0.0 < A.hc;
+    for (;;) {
+        if (!(t <= A.t_stop)) { reason = RED_FUSED_STOP_TIME; break; }
+        if (steps >= A.max_steps) { reason = RED_FUSED_STOP_STEPS; break; }
+        if (A.interval <= fabs(ps)) { reason = RED_FUSED_STOP_OUTPUT; break; }
+        if (tid == 0) flag = 0;
+        __syncthreads();
+        if (scan) {
+            for (int i = tid; i < A.n; i += FUSED_BLOCK)
+                if (A.id[i] > 0 && A.type[i] != 0 && ejection_name(A.r[cur][i], A.ej, A.hc) != RED_EV_NONE) flag = 1;
+        }
+        if (tid == 0) {
+            A.ctl->t = t;
+            A.ctl->dt = A.dt;
+            A.ctl->attempt = 0;
+            A.ctl->err_bits = 0ull;
+            A.ctl->nan_flag = 0;
+        }
+        __syncthreads();
+        if (flag != 0) { reason = RED_FUSED_STOP_EJECTION; break; }
+        fused_forces(A, P, 0, A.r[cur], A.v[cur], A.a[0], L);
+        __syncthreads();
+        if (A.clear_events_after_f0 && tid == 0)
+            __hip_atomic_store(&A.ctl->n_events, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const StageArgs* rows = rows_l + cur * A.n_main;
+        for (int k = 1; k <= A.n_main; ++k) {
+            const StageArgs& S = rows[k - 1];
+            fused_stage(A.form[k - 1], S, A.n, A.dt);
+            __syncthreads();
+            if (k < A.n_main) {
+                fused_forces(A, P, k, S.outr, S.outv, A.a[k], L);
+                __syncthreads();
+            }
+        }
+        t = t + A.dt;
+        ps = ps + fabs(A.dt);
+        ++steps;
+        cur = 1 - cur;
+        if (tid == 0) flag = __hip_atomic_load(&A.ctl->n_events, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const bool events = flag > 0;
+        __syncthreads();
+        if (events) { reason = RED_FUSED_STOP_EVENTS; break; }
+    }
+    if (tid == 0) *A.out = FusedOut{t, ps, steps, cur, reason, 0};
+}
+
+}  // namespace
+
+hipError_t launch_fused_fixed(const FusedArgs& A, const PairTest& P, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fused_fixed, dim3(1), dim3(FUSED_BLOCK), 0, s, A, P);
+    return hipGetLastError();
 }
 
 }  // namespace red

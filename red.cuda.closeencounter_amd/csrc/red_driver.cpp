@@ -76,7 +76,7 @@ std::string combine_path(const std::string& dir, const std::string& file)  // fi
 // ---- options (options.cpp:59-195) -------------------------------------------------------
 struct Options {
     int id_dev = 0;
-    bool gpu = true, verbose = false, ups = false, ef = false, mixed = false;
+    bool gpu = true, verbose = false, ups = false, ef = false, mixed = false, stepwise = false;
     int n_tpb = 64;
     std::string input_dir, printout_dir, param_file, gasdisk_file, body_file, restart;
     std::string info = "info", event = "event", log = "log", result = "result";
@@ -99,6 +99,7 @@ void usage()
                  "     -v      | --verbose                      : verbose mode\n"
                  "     -ef                                      : prefix output names with the run configuration\n"
                  "     -mixed                                   : mixed-precision force sweeps\n"
+                 "     -stepwise                                : fixed-step runs step from the host one at a time\n"
                  "     -restart <filename>                      : resume from the last snapshot of a text result file\n"
                  "     -gpu    | --gpu                          : execute on the GPU (the only path of this build)\n"
                  "     -h      | --help                         : print this help\n";
@@ -146,6 +147,8 @@ Options parse_options(int argc, const char** argv)
             o.ef = true;
         } else if (p == "-mixed") {
             o.mixed = true;
+        } else if (p == "-stepwise") {
+            o.stepwise = true;
         } else if (p == "-restart") {
             o.restart = next(i, p);
         } else if (p == "--cpu" || p == "-cpu") {
@@ -554,25 +557,10 @@ int main(int argc, const char** argv, const char** env)
         clock_t sum_steps = 0, one_step = 0, info_start = clock();
         print_result(result_f, S, P.binary);
         double t = P.start_time;
-        while (t <= P.stop_time) {
-            if (P.output_interval <= std::fabs(ps)) {
-                ps = 0.0;
-                print_result(result_f, S, P.binary);
-            }
+        const bool tests = 0.0 < P.radii_enhance_factor || 0.0 < P.hill;
+        // the checks after a step (main.cu:225-255)
+        auto after_step = [&]() {
             int handled = 0;
-            check(S.ctx, red_check_ejection_hit_centrum(S.ctx, &handled));
-            if (handled) {
-                print_event_data(event_f, log_f, S);
-                check(S.ctx, red_clear_events(S.ctx));
-            }
-            clock_t t0 = clock();
-            red_step_info info;
-            check(S.ctx, red_step(S.ctx, &info));
-            one_step = clock() - t0;
-            sum_steps += one_step;
-            dt = info.dt_did;
-            ps += std::fabs(dt);
-            const bool tests = 0.0 < P.radii_enhance_factor || 0.0 < P.hill;
             if (P.close_encounter && P.inner_steps) {
                 if (tests) {
                     check(S.ctx, red_check_close_encounter(S.ctx, &handled));
@@ -606,6 +594,54 @@ int main(int argc, const char** argv, const char** env)
                           << std::setprecision(16) << t << std::endl;
                 }
             }
+        };
+        // fixed-step integrators run the iterations that need nothing from the host on the
+        // device (red_run_fixed), in calls of at most 4096 steps so the info file keeps its
+        // cadence; adaptive runs and unsupported configurations step one at a time
+        bool fused = !opt.stepwise;
+        while (t <= P.stop_time) {
+            if (P.output_interval <= std::fabs(ps)) {
+                ps = 0.0;
+                print_result(result_f, S, P.binary);
+            }
+            if (fused) {
+                int k = 0, why = 0;
+                clock_t t0 = clock();
+                const int rc = red_run_fixed(S.ctx, 4096, P.stop_time, P.output_interval, &ps, &k, &why);
+                if (rc == RED_ERR_UNSUPPORTED) {
+                    fused = false;
+                } else {
+                    check(S.ctx, rc);
+                    if (k > 0) {
+                        sum_steps += clock() - t0;
+                        one_step = (clock() - t0) / k;
+                        check(S.ctx, red_get_dt(S.ctx, &dt));
+                    }
+                    if (why == RED_FUSED_STOP_EVENTS) after_step();
+                    // an output stop with no step done (output_interval <= 0) leaves the
+                    // iteration to the host as an ejection stop does
+                    if (why != RED_FUSED_STOP_EJECTION && !(why == RED_FUSED_STOP_OUTPUT && k == 0)) {
+                        if (5.0 < (clock() - info_start) / (double)CLOCKS_PER_SEC)
+                            print_info(info_f, S, dt, sum_steps, one_step, &info_start);
+                        check(S.ctx, red_get_time(S.ctx, &t));
+                        continue;
+                    }
+                }
+            }
+            int handled = 0;
+            check(S.ctx, red_check_ejection_hit_centrum(S.ctx, &handled));
+            if (handled) {
+                print_event_data(event_f, log_f, S);
+                check(S.ctx, red_clear_events(S.ctx));
+            }
+            clock_t t0 = clock();
+            red_step_info info;
+            check(S.ctx, red_step(S.ctx, &info));
+            one_step = clock() - t0;
+            sum_steps += one_step;
+            dt = info.dt_did;
+            ps += std::fabs(dt);
+            after_step();
             if (5.0 < (clock() - info_start) / (double)CLOCKS_PER_SEC)
                 print_info(info_f, S, dt, sum_steps, one_step, &info_start);
             check(S.ctx, red_get_time(S.ctx, &t));

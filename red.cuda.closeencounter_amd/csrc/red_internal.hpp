@@ -164,6 +164,16 @@ hipError_t launch_stage(int form, const StageArgs& A, hipStream_t s);
 hipError_t launch_err(int form, const StageArgs& A, hipStream_t s);
 hipError_t launch_reset_ctl(Ctl* ctl, int clear_events, hipStream_t s);
 
+// Ejection / hit-centrum predicate of one active non-star body (pp_disk.cu:72-142):
+// |r|^2 > R_ej^2 (R_ej > 0) is an ejection, otherwise |r|^2 < R_hc^2 (R_hc > 0) a hit.
+__device__ __forceinline__ int ejection_name(const d4& q, double ej, double hc)
+{
+    const double r2 = (q.x * q.x) + (q.y * q.y) + (q.z * q.z);
+    if (0.0 < ej && ej * ej < r2) return RED_EV_EJECTION;
+    if (0.0 < hc && hc * hc > r2) return RED_EV_HIT_CENTRUM;
+    return RED_EV_NONE;
+}
+
 // Ejection / hit-centrum scan (pp_disk.cu:72-142): appends events and negates ids.
 hipError_t launch_ejection_scan(const d4* r, const d4* v, int* id, const int* type, int n, double ej,
                                 double hc, Ctl* ctl, red_event* buf, int cap, hipStream_t s);
@@ -185,6 +195,39 @@ hipError_t launch_apply_updates(const BodyUpdate* u, int nu, d4* r, d4* v, doubl
 int energy_chunks(int n_massive);
 hipError_t launch_energy(const d4* r, const d4* v, const double* mass, const int* id, int n_massive, int n_si,
                          double* out6, double* part, hipStream_t s);
+
+// ---- fused fixed-step loop: main-loop iterations of one small disk in one work-group ----
+constexpr int FUSED_MAX_ROWS = MAX_TERMS;     // stage rows after f0, plus y_(n+1)
+constexpr int FUSED_MAX_BODIES = 4096;
+constexpr int FUSED_STEPS_PER_LAUNCH = 4096;  // the host relaunches for longer runs
+struct FusedOut {
+    double t, ps;                  // time and accumulated output step after the last step
+    int steps, cur, reason, pad;   // steps done, state buffer parity, RED_FUSED_STOP_*
+};
+struct FusedArgs {
+    d4* r[2];                      // positions by buffer parity (y_n in r[cur])
+    d4* v[2];
+    d4* a[FUSED_MAX_ROWS];         // accelerations per stage
+    const StageArgs* rows;         // [2][n_main]: rows 1 .. n_main-1 into S_k, then y_(n+1)
+    int form[FUSED_MAX_ROWS];      // stage form of each entry of a parity's rows
+    int n_main;
+    int clear_events_after_f0;     // RKF7(8) clears the event list at the attempt start
+    double c_stage[FUSED_MAX_ROWS];
+    Sweep sw[3];
+    int n;
+    const double* mass;
+    const double* radius;
+    const int* id;
+    const int* type;
+    double ej, hc;
+    red_event* ev_buf;
+    int ev_cap;
+    Ctl* ctl;
+    double t, dt, t_stop, ps, interval;
+    int cur, max_steps;
+    FusedOut* out;
+};
+hipError_t launch_fused_fixed(const FusedArgs& A, const PairTest& P, hipStream_t s);
 
 // ---- learned step-size proposer ----
 constexpr int MLP_NF = RED_MLP_FEATURES, MLP_NH = 64;

@@ -26,6 +26,9 @@ RED_OK = 0
 ERR_NAMES = {1: "RED_ERR_ARG", 2: "RED_ERR_HIP", 3: "RED_ERR_EVENT_OVERFLOW", 4: "RED_ERR_NAN",
              5: "RED_ERR_UNSUPPORTED", 6: "RED_ERR_NO_STAR", 7: "RED_ERR_IO", 8: "RED_ERR_STEP_LIMIT"}
 PREC_FP64, PREC_MIXED = 0, 1
+ERR_UNSUPPORTED = 5
+# stop reasons of red_run_fixed
+FUSED_STOP_STEPS, FUSED_STOP_TIME, FUSED_STOP_OUTPUT, FUSED_STOP_EJECTION, FUSED_STOP_EVENTS = 1, 2, 3, 4, 5
 
 
 class RedError(RuntimeError):
@@ -91,6 +94,8 @@ SIGNATURES = {
     "red_state_device": (_I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "red_sync": (_I, [_P]),
     "red_step": (_I, [_P, ctypes.POINTER(RedStepInfo)]),
+    "red_run_fixed": (_I, [_P, _I, _D, _D, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                           ctypes.POINTER(ctypes.c_int)]),
     "red_check_ejection_hit_centrum": (_I, [_P, _IP]),
     "red_check_collision": (_I, [_P, _IP]),
     "red_check_close_encounter": (_I, [_P, _IP]),
@@ -371,6 +376,14 @@ class Disk:
         self._chk(self.L.red_step(self.ctx, ctypes.byref(info)))
         self.last_step = info
         return info.dt_did
+
+    def run_fixed(self, max_steps: int, t_stop: float, output_interval: float, ps: float = 0.0):
+        """Fixed-step main-loop iterations on the device (red_run_fixed): runs until one
+        needs the host and returns (steps done, accumulated output step ps, stop reason)."""
+        p = ctypes.c_double(ps); k = ctypes.c_int(0); why = ctypes.c_int(0)
+        self._chk(self.L.red_run_fixed(self.ctx, int(max_steps), float(t_stop), float(output_interval),
+                                       ctypes.byref(p), ctypes.byref(k), ctypes.byref(why)))
+        return k.value, p.value, why.value
 
     def _flag(self, fn, *args):
         h = ctypes.c_int(0)

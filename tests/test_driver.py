@@ -116,6 +116,28 @@ def test_driver_events_match_python_loop(tmp_path, native_mod):
 
 
 @pytest.mark.gpu
+def test_driver_fused_matches_stepwise(tmp_path, native_mod):
+    """A fixed-step run with collisions and an ejection: the driver's fused device loop
+    (red_run_fixed between the iterations that need the host) and -stepwise (one red_step
+    per iteration) write identical result and event files."""
+    b = generate.disk_allsi(150, seed=17, a_range=(1.0, 1.05), m_range=(1e-10, 1e-8))
+    print("case driver fused vs stepwise", flush=True)
+    out = {}
+    for mode in ("fused", "stepwise"):
+        d = tmp_path / mode
+        d.mkdir()
+        files.write_body_file(str(d / "disk.txt"), b)
+        (d / "p.txt").write_text("integrator = rk4\nlength = 0.2\noutput_interval = 0.01\n"
+                                 "collision_factor = 300\nejection = 50\ndt = 0.5\n")
+        extra = ["-stepwise"] if mode == "stepwise" else []
+        r = run_red(["-iDir", str(d), "-p", "p.txt", "-ic", "disk.txt"] + extra)
+        assert r.returncode == 0, r.stderr
+        out[mode] = ((d / "result.txt").read_bytes(), (d / "event.txt").read_bytes())
+    assert out["fused"][0] == out["stepwise"][0]
+    assert out["fused"][1] == out["stepwise"][1] and len(out["fused"][1]) > 0
+
+
+@pytest.mark.gpu
 def test_driver_astro_frame_matches_python_loop(tmp_path, native_mod):
     """frame_center = astro: the driver shifts a body list whose star is off the origin into
     the star's frame (tools.cpp:209-236) and integrates there; the Python loop over the

@@ -15,7 +15,7 @@ from conftest import GOLDEN
 from rednative import files, mainloop
 
 
-def run_twobody(engine_factory):
+def run_twobody(engine_factory, fused=True):
     b = files.read_body_file(os.path.join(GOLDEN, "TwoBody2.txt"))
     prm = files.read_parameters(os.path.join(GOLDEN, "parameters2.txt"))
     eng, bc = engine_factory(b, prm)
@@ -26,7 +26,7 @@ def run_twobody(engine_factory):
         s = e.state()
         snaps.append((e.t, s["r"].copy(), s["v"].copy()))
 
-    info = mainloop.run_main_loop(eng, prm, snap)
+    info = mainloop.run_main_loop(eng, prm, snap, fused=fused)
     return snaps, info
 
 
