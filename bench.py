@@ -11,7 +11,11 @@ Reported: pair-interactions/s of the mixed-precision path (FP64 differences, FP3
 r^-3, compensated FP64 sums), simulated years per hour, the relative energy change
 over the timed steps, the same figures for the plain FP64 path on the same state,
 the roofline position of the dominant kernel (the test-particle force sweep) from
-HIP events around its launches, and the CPU oracle's pair rate on the host.
+HIP events around its launches, and the CPU oracle's pair rate on the host.  Further
+sections (each also alone with --only): the learned step-size proposer on the 16384-body
+disk, the 512-disk ensemble, the 256-body disk's 10^4 fixed RK4 steps through the fused
+device loop and step by step (small_disk), and, only with --only drift, the energy drift
+over a long run.
 """
 import argparse
 import json
