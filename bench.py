@@ -292,12 +292,13 @@ def small_disk_cpu_baseline(bodies, kw, prm):
     return time.perf_counter() - t0
 
 
-def small_disk_section(native, with_cpu=True):
+def small_disk_section(native, with_cpu=True, use_graph=True):
     """The 256-body workload at its full length: star + 2 giants + 253 planetesimals (seed
     101), RK4 at dt = 1 day, 10^4 steps of the reference's main loop (ejection at 100 AU,
     collisions at 1 x (R_i + R_j), a snapshot read back every 365.25 days).  The GPU runs it
     with the fused fixed-step loop (red_run_fixed) and one step at a time (HIP graphs); the
-    final states must be identical."""
+    final states must be identical.  use_graph=False: the step-by-step leg without HIP graphs
+    (for kernel traces)."""
     import numpy as np
     from rednative import generate, mainloop
     b = generate.to_internal(generate.disk_small(seed=101))
@@ -307,9 +308,9 @@ def small_disk_section(native, with_cpu=True):
                        "ejection 100 AU, collisions 1x(R_i+R_j), snapshot every 365.25 days"}
     states = {}
     for mode in ("fused", "stepwise"):
-        d = native.Disk(b, use_graph=True, **kw)
+        d = native.Disk(b, use_graph=use_graph, **kw)
         mainloop.run_main_loop(d, dict(prm, stop_time=20.5 * GAUSS), lambda e: None, fused=(mode == "fused"))
-        d = native.Disk(b, use_graph=True, **kw)
+        d = native.Disk(b, use_graph=use_graph, **kw)
         d.sync()
         snaps = []
         t0 = time.perf_counter()
@@ -406,7 +407,8 @@ def main():
     ap.add_argument("--proposer-years", type=float, default=5.0)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-days", type=float, default=100.0)
-    ap.add_argument("--no-graph", action="store_true", help="proposer section without HIP graphs")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="proposer and small-disk sections without HIP graphs (for kernel traces)")
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the mixed-vs-FP64 accuracy and reference-workload sections (for kernel traces "
                          "of the main sections alone)")
@@ -436,7 +438,8 @@ def main():
         print(json.dumps({"ensemble": ensemble_section(native, args.ensemble_days, 16)}))
         return
     if args.only == "smalldisk":
-        print(json.dumps({"small_disk": small_disk_section(native, with_cpu=not args.no_cpu)}))
+        print(json.dumps({"small_disk": small_disk_section(native, with_cpu=not args.no_cpu,
+                                                           use_graph=not args.no_graph)}))
         return
     if args.only == "drift":
         print(json.dumps({"drift": drift_section(native, args.drift_years, args.drift_disk, tol=args.drift_tol)}))
