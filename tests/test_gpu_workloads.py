@@ -135,3 +135,23 @@ def test_fused_loop_stops_and_fallbacks(native_mod):
         assert ex.value.code == native_mod.ERR_UNSUPPORTED
         info = mainloop.run_main_loop(e, dict(stop_time=e.t + 4.5 * dt, output_interval=1e9), lambda eng: None)
         assert info["steps"] >= 1 and e.t > 4.5 * dt
+
+
+def test_fused_loop_hill_events_and_every_step_output(oracle_mod, native_mod):
+    """Pair events from the mutual-Hill-radius test (10 R_H, collisions at 1 x (R_i + R_j))
+    handed back after the step that found them, and a snapshot after every step (the fused
+    loop stops at each output time after one step)."""
+    b = planted_disk(seed=44)
+    kw = dict(integrator=2, adaptive=False, dt_day=0.5, hit_centrum=0.0, ejection=0.0, radii_enhance_factor=1.0,
+              encounter_hill_factor=10.0)
+    print("case fused loop hill events", flush=True)
+    for every in (7, 1):
+        prm = loop_prm(0.5, 30, every, radii_enhance_factor=1.0, encounter_hill_factor=10.0)
+        ref = run_loop(oracle_mod.Oracle(b, **kw), prm)
+        d = native_mod.Disk(b, event_capacity=20000, **kw)
+        got = run_loop(d, prm)
+        compare_runs(ref, got)
+        assert d.counters()["n_close_encounter"] > 0
+        assert got[2]["fused_steps"] > 0
+        if every == 1:
+            assert len(got[0]) >= got[2]["steps"]
