@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
                                                    int jtile)
 {
     __shared__ d4 src[MAX_JTILE];
-    __shared__ double term[3][MAX_JTILE];
+    __shared__ double term[2][3][MAX_JTILE];
     const EnsTile T = tiles[blockIdx.x];
     if (!(flags[T.disk] & need_bit)) return;
     if (T.kind == 0) {
@@ -82,23 +82,84 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
         }
         if (in) a[i] = d4{ax, ay, az, 0.0};
     } else {
+        // One sink against many sources (a disk's star): its three sums are serial chains in
+        // ascending j, the kernel's critical path.  With more than one wave, waves 1.. form
+        // the terms of chunk c + 1 into one half of a double-buffered LDS tile while lanes
+        // 0-2 of wave 0 add chunk c, the next 16 terms read while the current 16 are added;
+        // with one wave it forms, then adds, each chunk.
         const int i = T.sink0;
         const d4 p = r[i];
-        double acc = 0.0;
         const int c = threadIdx.x;
-        for (int j0 = T.src0; j0 < T.src1; j0 += jtile) {
+        const int nsrc = T.src1 - T.src0;
+        const int nch = (nsrc + jtile - 1) / jtile;
+        const bool split = blockDim.x > 64;
+        const int p0 = split ? 64 : 0;                       // first producer work-item
+        const bool producer = (int)threadIdx.x >= p0;
+        auto produce = [&](int ch) {
+            const int j0 = T.src0 + ch * jtile;
             const int nj = min(jtile, T.src1 - j0);
-            __syncthreads();
-            for (int k = threadIdx.x; k < nj; k += blockDim.x) {
+            double* t0 = term[ch & 1][0];
+            double* t1 = term[ch & 1][1];
+            double* t2 = term[ch & 1][2];
+            for (int k = (int)threadIdx.x - p0; k < nj; k += (int)blockDim.x - p0) {
                 double tx, ty, tz;
                 fp64_term(r[j0 + k], p.x, p.y, p.z, (j0 + k) != i, tx, ty, tz);
-                term[0][k] = tx;
-                term[1][k] = ty;
-                term[2][k] = tz;
+                t0[k] = tx;
+                t1[k] = ty;
+                t2[k] = tz;
             }
+        };
+        double acc = 0.0;
+        auto consume = [&](int ch) {
+            const int nj = min(jtile, T.src1 - (T.src0 + ch * jtile));
+            const double* t = term[ch & 1][c];
+            constexpr int GQ = 16;
+            const int ng = nj / GQ;
+            double x[GQ], y[GQ];
+            if (ng > 0) {
+#pragma unroll
+                for (int k = 0; k < GQ; ++k) x[k] = t[k];
+            }
+            for (int g = 0; g < ng; g += 2) {
+                if (g + 1 < ng) {
+#pragma unroll
+                    for (int k = 0; k < GQ; ++k) y[k] = t[(g + 1) * GQ + k];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int k = 0; k < GQ; ++k) acc += x[k];
+                __builtin_amdgcn_sched_barrier(0);
+                if (g + 1 < ng) {
+                    if (g + 2 < ng) {
+#pragma unroll
+                        for (int k = 0; k < GQ; ++k) x[k] = t[(g + 2) * GQ + k];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int k = 0; k < GQ; ++k) acc += y[k];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            for (int k = ng * GQ; k < nj; ++k) acc += t[k];
+        };
+        if (split) {
+            if (producer && nch > 0) produce(0);
             __syncthreads();
-            if (c < 3)
-                for (int k = 0; k < nj; ++k) acc += term[c][k];
+            for (int ch = 0; ch < nch; ++ch) {
+                if (producer) {
+                    if (ch + 1 < nch) produce(ch + 1);
+                } else if (c < 3) {
+                    consume(ch);
+                }
+                __syncthreads();
+            }
+        } else {
+            for (int ch = 0; ch < nch; ++ch) {
+                produce(ch);
+                __syncthreads();
+                if (c < 3) consume(ch);
+                __syncthreads();
+            }
         }
         if (c < 3) reinterpret_cast<double*>(a + i)[c] = acc;
         if (c == 3) reinterpret_cast<double*>(a + i)[3] = 0.0;
@@ -420,6 +481,10 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
     const bool adaptive = E->prm.adaptive && S.err_form >= 0;
     int attempts = 0, active = 0;
     const auto t_begin = std::chrono::steady_clock::now();
+    // a disk done against an earlier t_stop continues (a run to t1, then to t2, steps as one
+    // run to t2); a disk left in a retry by max_attempts resumes it
+    for (int d = 0; d < E->nd; d++)
+        if (E->phase[d] == 2) E->phase[d] = 0;
     for (;;) {
         active = 0;
         for (int d = 0; d < E->nd; d++) {

@@ -74,3 +74,28 @@ def test_ensemble_fixed_step_rk4(native_mod):
     for k, b in enumerate(disks):
         alone = run_alone(native_mod, b, t_stop, **kw)
         assert_bitwise(E.state(k)[0], alone.state()["r"], f"disk {k}")
+
+
+def test_ensemble_advance_in_parts(native_mod):
+    """A run to t1 and then to t2 steps every member as one run to t2 does (the loop
+    condition only decides whether a step is taken), also when the first call is cut short
+    by max_attempts; every member still equals the single-disk path."""
+    disks = small_ensemble(n=6, seed=407)
+    kw = dict(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001)
+    print("case ensemble advance in parts", flush=True)
+    t1, t2 = 1.5 * 0.01720209895, 4.0 * 0.01720209895
+    whole = native_mod.Ensemble(disks, **kw)
+    whole.advance(t2)
+    parts = native_mod.Ensemble(disks, **kw)
+    info = parts.advance(t1)
+    assert info["supersteps"] > 0 and info["active_disks"] == 0
+    cut = parts.advance(t2, max_attempts=1)
+    assert cut["supersteps"] == 1 and cut["active_disks"] > 0
+    rest = parts.advance(t2)
+    assert rest["supersteps"] > 0 and rest["active_disks"] == 0
+    for k, b in enumerate(disks):
+        assert_bitwise(parts.state(k)[0], whole.state(k)[0], f"disk {k} r")
+        assert_bitwise(parts.state(k)[1], whole.state(k)[1], f"disk {k} v")
+        assert parts.disk_info(k) == whole.disk_info(k)
+        alone = run_alone(native_mod, b, t2, **kw)
+        assert_bitwise(parts.state(k)[0], alone.state()["r"], f"disk {k} alone")
