@@ -332,14 +332,15 @@ def small_disk_section(native, with_cpu=True, use_graph=True):
     return out
 
 
-def ensemble_section(native, days, sample):
+def ensemble_section(native, days, sample, use_graph=True):
     """512 disks (N log-uniform in [64, 4096], seeded) integrated together against the same
-    disks stepped one context at a time (a sample of them, scaled): disk-steps per second."""
+    disks stepped one context at a time (a sample of them, scaled): disk-steps per second.
+    use_graph=False: both without HIP graphs (for kernel traces)."""
     from rednative import generate
     disks = [generate.to_internal(d) for d in generate.ensemble(n_disks=512, seed=404)]
     kw = dict(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001)
     t_stop = days * GAUSS
-    E = native.Ensemble(disks, use_graph=True, **kw)
+    E = native.Ensemble(disks, use_graph=use_graph, **kw)
     block, jtile, _ = E.autotune(reps=3)
     t0 = time.perf_counter()
     info = E.advance(t_stop)
@@ -349,7 +350,7 @@ def ensemble_section(native, days, sample):
     t1 = time.perf_counter()
     steps_one = 0
     for k in range(sample):
-        d = native.Disk(disks[k], use_graph=True, **kw)
+        d = native.Disk(disks[k], use_graph=use_graph, **kw)
         while d.t <= t_stop:
             d.step()
         d.sync()
@@ -408,7 +409,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-days", type=float, default=100.0)
     ap.add_argument("--no-graph", action="store_true",
-                    help="proposer and small-disk sections without HIP graphs (for kernel traces)")
+                    help="proposer, ensemble and small-disk sections without HIP graphs (for kernel traces)")
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the mixed-vs-FP64 accuracy and reference-workload sections (for kernel traces "
                          "of the main sections alone)")
@@ -435,7 +436,7 @@ def main():
         print(json.dumps({"proposer": proposer_section(native, args.proposer_years, not args.no_graph)}))
         return
     if args.only == "ensemble":
-        print(json.dumps({"ensemble": ensemble_section(native, args.ensemble_days, 16)}))
+        print(json.dumps({"ensemble": ensemble_section(native, args.ensemble_days, 16, not args.no_graph)}))
         return
     if args.only == "smalldisk":
         print(json.dumps({"small_disk": small_disk_section(native, with_cpu=not args.no_cpu,
