@@ -143,11 +143,50 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
             for (int k = ng * GQ; k < nj; ++k) acc += t[k];
         };
         if (split) {
-            if (producer && nch > 0) produce(0);
+            // the adding wave issues ahead of the producers on its SIMD whenever its next
+            // addition is ready; the producers hold the sources of the chunk they form next
+            // in registers, requested one chunk period ahead
+            if (!producer) __builtin_amdgcn_s_setprio(3);
+            constexpr int MS = MAX_JTILE / 64;   // source slots per producer (at least 64 of them)
+            const int tp = (int)threadIdx.x - p0;
+            const int np = (int)blockDim.x - p0;
+            d4 q[MS];
+            auto fetch = [&](int ch) {
+                const int j0 = T.src0 + ch * jtile;
+                const int nj = min(jtile, T.src1 - j0);
+#pragma unroll
+                for (int m = 0; m < MS; ++m) {
+                    const int k = tp + m * np;
+                    if (k < nj) q[m] = r[j0 + k];
+                }
+            };
+            auto form = [&](int ch) {
+                const int j0 = T.src0 + ch * jtile;
+                const int nj = min(jtile, T.src1 - j0);
+#pragma unroll
+                for (int m = 0; m < MS; ++m) {
+                    const int k = tp + m * np;
+                    if (k < nj) {
+                        double tx, ty, tz;
+                        fp64_term(q[m], p.x, p.y, p.z, (j0 + k) != i, tx, ty, tz);
+                        term[ch & 1][0][k] = tx;
+                        term[ch & 1][1][k] = ty;
+                        term[ch & 1][2][k] = tz;
+                    }
+                }
+            };
+            if (producer && nch > 0) {
+                fetch(0);
+                form(0);
+                if (nch > 1) fetch(1);
+            }
             __syncthreads();
             for (int ch = 0; ch < nch; ++ch) {
                 if (producer) {
-                    if (ch + 1 < nch) produce(ch + 1);
+                    if (ch + 1 < nch) {
+                        form(ch + 1);
+                        if (ch + 2 < nch) fetch(ch + 2);
+                    }
                 } else if (c < 3) {
                     consume(ch);
                 }
