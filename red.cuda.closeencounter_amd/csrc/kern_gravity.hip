@@ -313,7 +313,7 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
 // source terms of a sink one after another in ascending j (pp_disk.cu:144-215); only those
 // additions are a serial chain -- the pair terms themselves (differences, IEEE sqrt and
 // divide) are independent.  A block of 1 + NP waves takes SG sinks: NP producer waves
-// compute the terms of the next J = 1024 / SG sources into one half of a double-buffered
+// compute the terms of the next J = TILE / SG sources into one half of a double-buffered
 // LDS tile while the consumer wave adds the previous chunk in ascending j, one lane per
 // (sink, component) row, starting from 0.  Every term is formed and added exactly as in the
 // fused loop, so the sums are bit-identical to the reference order, while the long FP64
@@ -322,12 +322,14 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
 // holds exactly M x SLOTS sources, so no producer lane idles on a full chunk, and the
 // source records of the chunk after next are requested before a chunk is computed.  Longer
 // chunks (fewer sinks per block) spread the per-chunk barrier and producer latency over
-// more sources; the tile stays within 2 x 1024 x 3 doubles.
+// more sources; the tile stays within 2 x TILE x 3 doubles, TILE = 1024, or 2048 for sweeps
+// of at most 256 blocks of 4 sinks (one block per CU, so LDS does not limit occupancy).
 constexpr int SQ_TILE = 1024;         // sinks x sources per chunk, at most
+constexpr int SQ_TILE_FEW = 2048;     // the same for sweeps of at most one block per CU
 constexpr int BUF_RSRC_DW3 = 0x00020000;   // raw buffer descriptor word 3 on CDNA (32-bit data format)
 constexpr long SEQ_MAX_SOURCES = (1L << 31) / 32;   // buffer byte offsets stay below 2^31
 
-template <bool OVERLAP, bool EV, int NP, int SQ_SG>
+template <bool OVERLAP, bool EV, int NP, int SQ_SG, int TILE_SQ = SQ_TILE>
 __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __restrict__ r, const d4* __restrict__ v,
                                                                  d4* __restrict__ a, const double* __restrict__ mass,
                                                                  const double* __restrict__ radius,
@@ -336,9 +338,9 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
 {
     constexpr int SQ_ROWS = 3 * SQ_SG;                // consumer lanes in use
     constexpr int SLOTS = NP * 64 / SQ_SG;            // source slots per pass over the chunk
-    constexpr int M = SQ_TILE / SQ_SG / SLOTS > 0 ? SQ_TILE / SQ_SG / SLOTS : 1;   // sources per lane and chunk
+    constexpr int M = TILE_SQ / SQ_SG / SLOTS > 0 ? TILE_SQ / SQ_SG / SLOTS : 1;   // sources per lane and chunk
     constexpr int SQ_J = M * SLOTS;                   // sources per chunk
-    static_assert(SQ_J * SQ_SG <= SQ_TILE, "tile");
+    static_assert(SQ_J * SQ_SG <= TILE_SQ, "tile");
     static_assert(SQ_ROWS <= 64, "one consumer wave");
     __shared__ double buf[2][SQ_J][SQ_ROWS];
     const int wv = threadIdx.x >> 6;
@@ -552,6 +554,20 @@ hipError_t go_fp64_seq_sg(int np, const Sweep& sw, const GravBuffers& B, const P
                           hipStream_t s)
 {
     dim3 g((sw.sink1 - sw.sink0 + SG - 1) / SG);
+    // at most one block per CU (the SI sweep of a few hundred sinks): LDS is no constraint,
+    // and longer chunks mean fewer barriers on the serial chain
+    if constexpr (SG == 4) {
+        if (g.x <= 256 && np > 7) {
+            hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 15, SG, SQ_TILE_FEW>), g, dim3(1024), 0, s, B.r, B.v, B.a,
+                               B.mass, B.radius, B.id, sw, P, E, B.ctl);
+            return hipGetLastError();
+        }
+        if (g.x <= 256 && np > 3) {
+            hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 7, SG, SQ_TILE_FEW>), g, dim3(512), 0, s, B.r, B.v, B.a,
+                               B.mass, B.radius, B.id, sw, P, E, B.ctl);
+            return hipGetLastError();
+        }
+    }
     if (np <= 3)
         hipLaunchKernelGGL((k_grav_fp64_seq<OV, EV, 3, SG>), g, dim3(256), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
                            sw, P, E, B.ctl);
