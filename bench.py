@@ -292,13 +292,14 @@ def small_disk_cpu_baseline(bodies, kw, prm):
     return time.perf_counter() - t0
 
 
-def small_disk_section(native, with_cpu=True, use_graph=True):
+def small_disk_section(native, with_cpu=True, use_graph=True, fused_only=False):
     """The 256-body workload at its full length: star + 2 giants + 253 planetesimals (seed
     101), RK4 at dt = 1 day, 10^4 steps of the reference's main loop (ejection at 100 AU,
     collisions at 1 x (R_i + R_j), a snapshot read back every 365.25 days).  The GPU runs it
     with the fused fixed-step loop (red_run_fixed) and one step at a time (HIP graphs); the
     final states must be identical.  use_graph=False: the step-by-step leg without HIP graphs
-    (for kernel traces)."""
+    (for kernel traces); fused_only: the fused leg alone (for counter runs, which sample
+    every launch)."""
     import numpy as np
     from rednative import generate, mainloop
     b = generate.to_internal(generate.disk_small(seed=101))
@@ -307,7 +308,7 @@ def small_disk_section(native, with_cpu=True, use_graph=True):
     out = {"workload": "1 star + 2 giants + 253 planetesimals (seed 101), RK4 dt = 1 day, 10^4 steps, "
                        "ejection 100 AU, collisions 1x(R_i+R_j), snapshot every 365.25 days"}
     states = {}
-    for mode in ("fused", "stepwise"):
+    for mode in ("fused",) if fused_only else ("fused", "stepwise"):
         d = native.Disk(b, use_graph=use_graph, **kw)
         mainloop.run_main_loop(d, dict(prm, stop_time=20.5 * GAUSS), lambda e: None, fused=(mode == "fused"))
         d = native.Disk(b, use_graph=use_graph, **kw)
@@ -322,6 +323,8 @@ def small_disk_section(native, with_cpu=True, use_graph=True):
         out[mode] = {"seconds": sec, "us_per_step": sec / info["steps"] * 1e6, "steps": info["steps"],
                      "snapshots": len(snaps), "sim_yr_per_hr": (d.t / GAUSS / 365.25) / (sec / 3600.0)}
         d.close()
+    if fused_only:
+        return out
     out["identical"] = bool(all(np.array_equal(states["fused"][k].view(np.uint64), states["stepwise"][k].view(np.uint64))
                                 for k in range(2)))
     if with_cpu:
@@ -416,6 +419,8 @@ def main():
     ap.add_argument("--only", choices=["proposer", "ensemble", "drift", "smalldisk"], default=None,
                     help="run one auxiliary section alone and print its JSON (drift: the 16384-body "
                          "disk's energy drift over --drift-years, not part of the default run)")
+    ap.add_argument("--fused-only", action="store_true",
+                    help="with --only smalldisk: the fused leg alone (no step-by-step leg, no CPU leg)")
     ap.add_argument("--drift-years", type=float, default=1000.0)
     ap.add_argument("--drift-disk", choices=["powerlaw", "mixed", "mixed-disk-fp64"], default="powerlaw")
     ap.add_argument("--drift-tol", type=float, default=None,
@@ -440,7 +445,8 @@ def main():
         return
     if args.only == "smalldisk":
         print(json.dumps({"small_disk": small_disk_section(native, with_cpu=not args.no_cpu,
-                                                           use_graph=not args.no_graph)}))
+                                                           use_graph=not args.no_graph,
+                                                           fused_only=args.fused_only)}))
         return
     if args.only == "drift":
         print(json.dumps({"drift": drift_section(native, args.drift_years, args.drift_disk, tol=args.drift_tol)}))
