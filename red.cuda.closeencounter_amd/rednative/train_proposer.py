@@ -1,7 +1,7 @@
 """Records step histories and trains the step-size proposer on the GPU.
 
-    python -m rednative.train_proposer --out proposer.npz [--c1-disks 64 --c1-steps 15000
-        --c3-steps 3000 --c2-disks 8 --c2-steps 12000 --epochs 30]
+    python -m rednative.train_proposer --out proposer.npz [--small-disks 64 --small-steps 15000
+        --mixed-steps 3000 --powerlaw-disks 8 --powerlaw-steps 12000 --epochs 30]
 
 Records come from the library's own step log (red_history_*):
   * disks drawn as the 256-body planetary disk (1 star, 2 giants, 253 planetesimals, seeds
@@ -39,23 +39,23 @@ def _log(msg):
     print(msg, flush=True)
 
 
-def gather(c1_disks: int, c1_steps: int, c3_steps: int, c2_disks: int, c2_steps: int, threads: int = 8):
+def gather(small_disks: int, small_steps: int, mixed_steps: int, powerlaw_disks: int, powerlaw_steps: int, threads: int = 8):
     parts = {}
     t0 = time.time()
-    if c1_disks > 0:
-        disks = [generate.to_internal(generate.disk_small(seed=1000 + k)) for k in range(c1_disks)]
-        parts["c1"] = proposer.collect_many(
-            disks, c1_steps, threads=threads, explore=EXPLORE,
-            progress=lambda k, H: _log(f"  c1 disk {k}: {len(H)} records ({time.time() - t0:.0f}s)"), **RKF78_TOL)
-    if c3_steps > 0:
+    if small_disks > 0:
+        disks = [generate.to_internal(generate.disk_small(seed=1000 + k)) for k in range(small_disks)]
+        parts["small"] = proposer.collect_many(
+            disks, small_steps, threads=threads, explore=EXPLORE,
+            progress=lambda k, H: _log(f"  small disk {k}: {len(H)} records ({time.time() - t0:.0f}s)"), **RKF78_TOL)
+    if mixed_steps > 0:
         b = generate.to_internal(generate.disk_mixed())
-        parts["c3"] = proposer.collect(b, c3_steps, explore=EXPLORE, **RK5_MIXED)
-        _log(f"  c3: {len(parts['c3'])} records ({time.time() - t0:.0f}s)")
-    if c2_disks > 0:
-        disks = [generate.to_internal(generate.disk_powerlaw(n=4095, seed=2020 + k)) for k in range(c2_disks)]
-        parts["c2family"] = proposer.collect_many(
-            disks, c2_steps, threads=threads, explore=EXPLORE,
-            progress=lambda k, H: _log(f"  c2-family disk {k}: {len(H)} records ({time.time() - t0:.0f}s)"),
+        parts["mixed"] = proposer.collect(b, mixed_steps, explore=EXPLORE, **RK5_MIXED)
+        _log(f"  mixed disk: {len(parts['mixed'])} records ({time.time() - t0:.0f}s)")
+    if powerlaw_disks > 0:
+        disks = [generate.to_internal(generate.disk_powerlaw(n=4095, seed=2020 + k)) for k in range(powerlaw_disks)]
+        parts["powerlaw"] = proposer.collect_many(
+            disks, powerlaw_steps, threads=threads, explore=EXPLORE,
+            progress=lambda k, H: _log(f"  power-law disk {k}: {len(H)} records ({time.time() - t0:.0f}s)"),
             **RKF78_TOL)
     return parts
 
@@ -63,11 +63,11 @@ def gather(c1_disks: int, c1_steps: int, c3_steps: int, c2_disks: int, c2_steps:
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--out", required=True)
-    ap.add_argument("--c1-disks", type=int, default=64)
-    ap.add_argument("--c1-steps", type=int, default=15000)
-    ap.add_argument("--c3-steps", type=int, default=3000)
-    ap.add_argument("--c2-disks", type=int, default=0)
-    ap.add_argument("--c2-steps", type=int, default=12000)
+    ap.add_argument("--small-disks", type=int, default=64)
+    ap.add_argument("--small-steps", type=int, default=15000)
+    ap.add_argument("--mixed-steps", type=int, default=3000)
+    ap.add_argument("--powerlaw-disks", type=int, default=0)
+    ap.add_argument("--powerlaw-steps", type=int, default=12000)
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=4096)
@@ -76,7 +76,7 @@ def main(argv=None):
     ap.add_argument("--records", default=None, help="also save the records (.npz)")
     a = ap.parse_args(argv)
     _log("recording step histories")
-    parts = gather(a.c1_disks, a.c1_steps, a.c3_steps, a.c2_disks, a.c2_steps, a.threads)
+    parts = gather(a.small_disks, a.small_steps, a.mixed_steps, a.powerlaw_disks, a.powerlaw_steps, a.threads)
     R = np.concatenate(list(parts.values()))
     counts = {k: int(len(v)) for k, v in parts.items()}
     _log(f"records: {counts} total {len(R)}")
