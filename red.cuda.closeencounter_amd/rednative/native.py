@@ -157,7 +157,7 @@ def load_library(path: Optional[str] = None):
         return _lib
     p = path or LIB_PATH
     if not os.path.exists(p):
-        raise FileNotFoundError(f"{p} is not built; run __graft_entry__.build() or make -C {PKG}")
+        raise FileNotFoundError(f"{p} is not built; run make -C {PKG}")
     L = ctypes.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(L, name)
