@@ -97,6 +97,12 @@ def lib():
         L.orc_calculate_phase.argtypes = [D, dp, dp, dp]
         L.orc_calculate_orbital_element.restype = I
         L.orc_calculate_orbital_element.argtypes = [D, dp, dp, dp]
+        L.orc_create_sp_events.restype = I
+        L.orc_create_sp_events.argtypes = [ctypes.POINTER(OrcEvent), I, ctypes.POINTER(OrcEvent)]
+        L.orc_max_error.restype = D
+        L.orc_max_error.argtypes = [dp, dp, I, D, D]
+        L.orc_handle_collision_pair.restype = I
+        L.orc_handle_collision_pair.argtypes = [P, ctypes.POINTER(OrcEvent), I]
         _lib = L
     return _lib
 
@@ -244,6 +250,11 @@ class Oracle:
         self.L.orc_accel(self.h, t, stage, _dp(r), _dp(v), _dp(a))
         return a
 
+    def handle_collision_pair(self, event: OrcEvent, kernel_negated: bool = False) -> OrcEvent:
+        """pp_disk.cu:1151-1202 on a caller-built record; returns the completed record."""
+        self._check(self.L.orc_handle_collision_pair(self.h, ctypes.byref(event), int(kernel_negated)))
+        return event
+
     def counters(self):
         c = np.zeros(8, dtype=np.int32)
         self.L.orc_counters(self.h, _ip(c))
@@ -255,3 +266,20 @@ class Oracle:
         E = np.zeros(1); L = np.zeros(3)
         self.L.orc_energy(self.h, _dp(E), _dp(L))
         return float(E[0]), L
+
+
+def create_sp_events(records):
+    """pp_disk.cu:1111-1149 on a list of OrcEvent records; returns the kept records."""
+    L = lib()
+    n = len(records)
+    src = (OrcEvent * max(n, 1))(*records)
+    out = (OrcEvent * max(n, 1))()
+    k = L.orc_create_sp_events(src, n, out)
+    return [out[i] for i in range(k)]
+
+
+def max_error(err_r, err_v, n_var, dt, lam):
+    """integrator.cu:211-256 (CPU branch) on given error vectors."""
+    er = np.ascontiguousarray(err_r, dtype=np.float64)
+    ev = np.ascontiguousarray(err_v, dtype=np.float64)
+    return lib().orc_max_error(_dp(er), _dp(ev), int(n_var), float(dt), float(lam))

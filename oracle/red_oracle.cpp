@@ -259,6 +259,41 @@ static int calculate_orbital_element(const double mu, oe_t* oe, const vec_t rVec
 struct Bound { int sink0, sink1, src0, src1; };
 
 // ---- pp_disk (CPU path) ---------------------------------------------------------------
+// pp_disk.cu:1111-1149: one record per (id1, id2) pair in first-occurrence order, the
+// record with the earliest t kept (the first of equal times).
+static void create_sp_events_list(const std::vector<orc_event>& events, std::vector<orc_event>& sp_events)
+{
+    int ec = (int)events.size();
+    sp_events.resize(ec);
+    std::vector<bool> processed(ec, false);
+    int n = 0;
+    for (int k = 0; k < ec; k++) {
+        if (!processed[k]) { processed[k] = true; sp_events[n] = events[k]; }
+        else continue;
+        for (int i = k + 1; i < ec; i++) {
+            if (sp_events[n].id1 == events[i].id1 && sp_events[n].id2 == events[i].id2) {
+                processed[i] = true;
+                if (sp_events[n].t > events[i].t) sp_events[n] = events[i];
+            }
+        }
+        n++;
+    }
+    sp_events.resize(n);
+}
+
+// integrator.cu:211-256, CPU branch: the largest signed component of each error vector,
+// starting from 0 (a NaN component never compares larger and is passed over), scaled by
+// |dt * lambda|.
+static double max_error(const double* err_r, const double* err_v, int n_var, double dt, double lambda)
+{
+    double max_err_r = 0.0, max_err_v = 0.0;
+    for (int i = 0; i < n_var; i++) {
+        if (max_err_r < err_r[i]) max_err_r = err_r[i];
+        if (max_err_v < err_v[i]) max_err_v = err_v[i];
+    }
+    return fabs(dt * lambda * std::max(max_err_r, max_err_v));
+}
+
 struct Disk {
     int n_s, n_gp, n_rp, n_pp, n_spl, n_pl, n_tp;
     int n_i[7];
@@ -520,24 +555,7 @@ struct Disk {
     }
 
     // pp_disk.cu:1111-1149
-    void create_sp_events() {
-        int ec = (int)events.size();
-        sp_events.resize(ec);
-        std::vector<bool> processed(ec, false);
-        int n = 0;
-        for (int k = 0; k < ec; k++) {
-            if (!processed[k]) { processed[k] = true; sp_events[n] = events[k]; }
-            else continue;
-            for (int i = k + 1; i < ec; i++) {
-                if (sp_events[n].id1 == events[i].id1 && sp_events[n].id2 == events[i].id2) {
-                    processed[i] = true;
-                    if (sp_events[n].t > events[i].t) sp_events[n] = events[i];
-                }
-            }
-            n++;
-        }
-        sp_events.resize(n);
-    }
+    void create_sp_events() { create_sp_events_list(events, sp_events); }
 
     // pp_disk.cu:683-715 + 1091-1109
     bool check_for_ejection_hit_centrum() {
@@ -686,14 +704,7 @@ struct Integrator {
         }
     }
     // integrator.cu:211-256 (CPU branch)
-    double get_max_error(int n_var, double lambda) {
-        double max_err_r = 0.0, max_err_v = 0.0;
-        for (int i = 0; i < n_var; i++) {
-            if (max_err_r < err[0][i]) max_err_r = err[0][i];
-            if (max_err_v < err[1][i]) max_err_v = err[1][i];
-        }
-        return fabs(dt_try * lambda * std::max(max_err_r, max_err_v));
-    }
+    double get_max_error(int n_var, double lambda) { return max_error(err[0].data(), err[1].data(), n_var, dt_try, lambda); }
     void update_counters(int iter) { n_tried_step += iter; n_failed_step += (iter - 1); n_passed_step++; }
     double* Y(int i) { return (double*)ppd->y[i].data(); }
     double* YO(int i) { return (double*)ppd->yout[i].data(); }
@@ -1200,6 +1211,31 @@ int orc_calculate_orbital_element(double mu, const double* r3, const double* v3,
     int rc = calculate_orbital_element(mu, &o, get3(r3), get3(v3));
     set6(oe6, o);
     return rc;
+}
+
+// pp_disk.cu:1111-1149 on a caller-built raw list (exposed for the hand-built tests).
+int orc_create_sp_events(const orc_event* in, int n, orc_event* out)
+{
+    std::vector<orc_event> ev(in, in + n), sp;
+    create_sp_events_list(ev, sp);
+    for (size_t k = 0; k < sp.size(); k++) out[k] = sp[k];
+    return (int)sp.size();
+}
+
+// integrator.cu:211-256 on caller-given error vectors (n_var components each).
+double orc_max_error(const double* err_r, const double* err_v, int n_var, double dt, double lambda)
+{
+    return max_error(err_r, err_v, n_var, dt, lambda);
+}
+
+// pp_disk.cu:1151-1202: merges the pair of a caller-built collision record into the disk
+// (idx1 survives, idx2 is removed; r1, v1, r2, v2 are the recorded stage states).  The
+// record receives p1, p2, ps, rs and vs.  Returns 0, or -1 on an exception.
+int orc_handle_collision_pair(void* h, orc_event* e, int kernel_negated)
+{
+    Oracle* o = (Oracle*)h;
+    try { o->disk.handle_collision_pair(e, kernel_negated != 0); return 0; }
+    catch (const std::string& s) { o->last_error = s; return -1; }
 }
 
 }  // extern "C"

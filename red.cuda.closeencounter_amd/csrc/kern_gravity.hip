@@ -362,14 +362,12 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
         const bool live = sink_in && id[i] > 0;
         const double Ri = EV ? radius[i] : 0.0;
         const double scr = EV ? pair_screen(P, Ri, mass[i], p) : 0.0;
-        // Sources reach the lanes through buffer loads: a wave-uniform chunk offset plus the
-        // lane's fixed byte offset, with no address arithmetic per chunk.  The buffers end at
-        // the sweep's last source, so slots past it in the last chunk read zeros (they are
-        // never used); the host keeps nsrc * 32 bytes below 2^31.
-        const __amdgpu_buffer_rsrc_t rs_r =
-            __builtin_amdgcn_make_buffer_rsrc((void*)(r + sw.src0), 0, nsrc * (int)sizeof(d4), BUF_RSRC_DW3);
-        const __amdgpu_buffer_rsrc_t rs_i =
-            __builtin_amdgcn_make_buffer_rsrc((void*)(id + sw.src0), 0, nsrc * (int)sizeof(int), BUF_RSRC_DW3);
+        // Sources reach the lanes through buffer loads: the lane's byte offset is fixed for the
+        // sweep and each chunk gets its own descriptor, based at the chunk's first source and
+        // ending at the sweep's last one (wave-uniform, so built with scalar instructions).  The
+        // hardware range check compares the lane offset with the descriptor's size, so slots
+        // past the last source in the last chunk read zeros (they are never used) and no load
+        // leaves the sweep's sources; the host keeps nsrc * 32 bytes below 2^31.
         unsigned off_r[M], off_i[M];
 #pragma unroll
         for (int m = 0; m < M; ++m) {
@@ -379,14 +377,18 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
         // raw ids are kept: comparing them here would wait for the loads
         auto fetch = [&](int c, d4* qd, int* idd) {
             const int sb = c * SQ_J;
+            const __amdgpu_buffer_rsrc_t rs_r = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(r + sw.src0 + sb), 0, (nsrc - sb) * (int)sizeof(d4), BUF_RSRC_DW3);
+            const __amdgpu_buffer_rsrc_t rs_i = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(id + sw.src0 + sb), 0, (nsrc - sb) * (int)sizeof(int), BUF_RSRC_DW3);
 #pragma unroll
             for (int m = 0; m < M; ++m) {
-                const double2 lo = __builtin_bit_cast(
-                    double2, __builtin_amdgcn_raw_buffer_load_b128(rs_r, off_r[m], sb * (int)sizeof(d4), 0));
-                const double2 hi = __builtin_bit_cast(
-                    double2, __builtin_amdgcn_raw_buffer_load_b128(rs_r, off_r[m] + 16u, sb * (int)sizeof(d4), 0));
+                const double2 lo =
+                    __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs_r, off_r[m], 0, 0));
+                const double2 hi =
+                    __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rs_r, off_r[m] + 16u, 0, 0));
                 qd[m] = d4{lo.x, lo.y, hi.x, hi.y};
-                idd[m] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_i, off_i[m], sb * (int)sizeof(int), 0);
+                idd[m] = (int)__builtin_amdgcn_raw_buffer_load_b32(rs_i, off_i[m], 0, 0);
             }
         };
         // one term: formed for every slot (the quotient is masked afterwards, so the M

@@ -62,20 +62,47 @@ def loop_prm(dt_day, steps, every, **kw):
     return prm
 
 
-def test_small_disk_full_length_bit_exact(oracle_mod, native_mod):
+@pytest.mark.parametrize("hill", [0.0, 3.0])
+def test_small_disk_full_length_bit_exact(oracle_mod, native_mod, hill):
     """Star + 2 giants + 253 planetesimals (seed 101), RK4 at dt = 1 day, ejection at 100 AU,
-    collisions at 1 x (R_i + R_j): 10^4 steps with a snapshot every 365.25 days, the GPU run
-    fused on the device."""
+    collisions at 1 x (R_i + R_j), close encounters at `hill` mutual Hill radii (0 = off):
+    10^4 steps with a snapshot every 365.25 days, the GPU run fused on the device."""
     b = generate.to_internal(generate.disk_small(seed=101))
-    kw = dict(integrator=2, adaptive=False, dt_day=1.0, ejection=100.0, radii_enhance_factor=1.0)
-    prm = dict(stop_time=9999.5 * GAUSS, output_interval=365.25 * GAUSS, inner_steps=True, radii_enhance_factor=1.0)
-    print(f"case 256-body RK4 10^4 steps n={b.n}", flush=True)
+    kw = dict(integrator=2, adaptive=False, dt_day=1.0, ejection=100.0, radii_enhance_factor=1.0,
+              encounter_hill_factor=hill)
+    prm = dict(stop_time=9999.5 * GAUSS, output_interval=365.25 * GAUSS, inner_steps=True, radii_enhance_factor=1.0,
+               encounter_hill_factor=hill)
+    print(f"case 256-body RK4 10^4 steps n={b.n} hill={hill}", flush=True)
     ref = run_loop(oracle_mod.Oracle(b, **kw), prm)
     d = native_mod.Disk(b, use_graph=True, **kw)
     got = run_loop(d, prm)
     compare_runs(ref, got)
+    # with the Hill test on, the pair tests run in every sweep; no planetesimal (non-self-
+    # interacting: it meets only the giants at 5.2 and 9.5 AU) comes within 3 mutual Hill
+    # radii in these 10^4 days, in the oracle as on the GPU, so every step stays fused
     assert got[2]["steps"] == 10000 and got[2]["fused_steps"] == 10000
+    assert len(got[1]) == len(ref[1]) == 0
     assert len(got[0]) > 20
+
+
+@pytest.mark.parametrize("maker", ["few_sinks_many_tiles", "per_sink_many_tiles"])
+def test_fused_loop_multi_tile_sweeps(oracle_mod, native_mod, maker):
+    """Sweeps that need more than one LDS tile of sources in the fused loop: 3 sinks against
+    1203 sources (few-sinks mode, 512 sources per tile) and a power-law disk whose ~68
+    self-interacting sinks see 2101 sources (one sink per work-item, 1536 sources per tile)."""
+    if maker == "few_sinks_many_tiles":
+        b = generate.to_internal(generate.disk_small(n_pl=1200, seed=17))
+    else:
+        b = generate.to_internal(generate.disk_powerlaw(n=2100, seed=18))
+    nsi = sum(b.counts[:4])
+    print(f"case fused multi-tile {maker} n={b.n} n_si={nsi}", flush=True)
+    assert (nsi <= 64 and b.n > 1024) or (nsi > 64 and b.n > 1536)
+    kw = dict(integrator=2, adaptive=False, dt_day=0.5, radii_enhance_factor=1.0)
+    prm = loop_prm(0.5, 40, 10, radii_enhance_factor=1.0)
+    ref = run_loop(oracle_mod.Oracle(b, **kw), prm)
+    got = run_loop(native_mod.Disk(b, **kw), prm)
+    compare_runs(ref, got)
+    assert got[2]["fused_steps"] > 0
 
 
 @pytest.mark.parametrize("integ", [0, 1, 2, 3, 4])
