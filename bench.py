@@ -43,8 +43,8 @@ def pair_counts(counts):
     return si + nsi + ni, dict(si=si, nsi=nsi, ni=ni)
 
 
-def run_path(native, bodies, precision, steps, warmup, kahan=False):
-    d = native.Disk(bodies, integrator=3, adaptive=True, tolerance=1e-10, dt_day=1.0, precision=precision,
+def run_path(native, bodies, precision, steps, warmup, kahan=False, tol=1e-10):
+    d = native.Disk(bodies, integrator=3, adaptive=True, tolerance=tol, dt_day=1.0, precision=precision,
                     radii_enhance_factor=1.0, ejection=100.0, kahan_pairs=kahan, use_graph=False)
     for _ in range(warmup):
         d.step()
@@ -150,14 +150,48 @@ def accuracy_section(native, bodies, steps=10, dt_day=1.0):
             "vel_rel_err_max": float(dv.max()), "vel_rel_err_median": float(np.median(dv))}
 
 
-def cpu_baseline(bodies_full, target_s=10.0):
-    """CPU oracle force evaluations on a bounded sample of the same disk."""
+def host_info():
+    """The host the CPU legs run on: lscpu's model and topology, the CPUs this process may
+    use, and the OpenMP thread count the job is given."""
+    import subprocess
+    info = {"os_cpu_count": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_threads():
+    """Threads of the CPU leg: every CPU this process may use, capped by OMP_NUM_THREADS when
+    the job sets it (the share of the host that comes with one GPU)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(bodies_full, target_s=4.0, sim_steps=3):
+    """The CPU oracle (FP64, the reference's operation order) on the same disk, timed on the
+    host: pair rate of force evaluations on all the job's threads and on one thread (each on
+    a sample of the test particles sized to about target_s), and simulated years per hour
+    from sim_steps full adaptive Cash-Karp steps on all threads after one untimed step (the
+    main loop's work per step: ejection check, step, collision check), extrapolated to one
+    thread by the pair rates."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
     from rednative import files
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
     m = sum(bodies_full.counts[:6])
 
     def sample(ntp):
@@ -168,20 +202,50 @@ def cpu_baseline(bodies_full, target_s=10.0):
                              b.mass[keep], b.radius[keep], b.density[keep], b.cd[keep], b.mig_type[keep],
                              b.mig_stop_at[keep], b.r[keep], b.v[keep])
 
-    def rate(ntp):
+    def rate(ntp, nthr):
         s = sample(ntp)
-        o = pyoracle.Oracle(s, n_threads=threads, radii_enhance_factor=1.0)
+        o = pyoracle.Oracle(s, n_threads=nthr, radii_enhance_factor=1.0)
         t0 = time.perf_counter()
         o.accel(0.0, s.r, s.v)
         dt = time.perf_counter() - t0
         return pair_counts(s.counts)[0], dt
 
-    p, dt = rate(256)
-    ntp = int(max(256, min(bodies_full.counts[6], 256 * target_s / max(dt, 1e-6))))
-    p, dt = rate(ntp)
-    return dict(value=p / dt, unit="pair-interactions/s", cores=threads, kind="port",
-                sample=f"one FP64 force evaluation (oracle, OpenMP over sinks) of the same disk with "
-                       f"{ntp} of its 65533 test particles: {p} pairs in {dt:.2f} s")
+    def measured(nthr):
+        p, dt = rate(256, nthr)
+        ntp = int(max(256, min(bodies_full.counts[6], 256 * target_s / max(dt, 1e-6))))
+        p, dt = rate(ntp, nthr)
+        return p / dt, f"{ntp} of its {bodies_full.counts[6]} test particles: {p} pairs in {dt:.2f} s"
+
+    rate_all, sample_all = measured(threads)
+    rate_one, sample_one = measured(1)
+    # full steps of the whole disk on all threads
+    o = pyoracle.Oracle(bodies_full, integrator=3, adaptive=True, tolerance=1e-10, dt_day=1.0,
+                        radii_enhance_factor=1.0, ejection=100.0, n_threads=threads)
+    def one_step():
+        o.check_for_ejection_hit_centrum()
+        o.step()
+        o.check_for_collision()
+        o.clear_event_counter()
+
+    one_step()                    # the first step's trial and rejections, untimed as on the GPU
+    t_start, tried0 = o.t, o.counters()["tried"]
+    t0 = time.perf_counter()
+    for _ in range(sim_steps):
+        one_step()
+    wall = time.perf_counter() - t0
+    yrs = (o.t - t_start) / GAUSS / 365.25
+    attempts = o.counters()["tried"] - tried0
+    sim_all = yrs / (wall / 3600.0)
+    return dict(value=rate_all, unit="pair-interactions/s", cores=threads, kind="port",
+                sample=f"one FP64 force evaluation (oracle, OpenMP over sinks, {threads} threads) of the same disk "
+                       f"with {sample_all}",
+                one_thread={"pairs_per_s": rate_one, "sample": f"the same on 1 thread with {sample_one}"},
+                sim_yr_per_hr_extrapolated={
+                    "threads": threads, "value": sim_all, "one_thread_value": sim_all * rate_one / rate_all,
+                    "basis": f"{sim_steps} full adaptive Cash-Karp RK5(4) steps (tol 1e-10, {attempts} attempts) "
+                             f"of the whole disk after one untimed step: {yrs * 365.25:.3f} simulated days in "
+                             f"{wall:.1f} s on {threads} threads; one thread scaled by the pair rates"},
+                host=host_info())
 
 
 def proposer_section(native, years, use_graph=True):
@@ -206,7 +270,19 @@ def proposer_section(native, years, use_graph=True):
     return out
 
 
-def drift_section(native, years, disk="powerlaw", report_years=None, tol=None):
+def _checkpoint_bodies(d):
+    """The disk's current state as a body set (internal units), for a checkpoint."""
+    import numpy as np
+    from rednative import files
+    s = d.state()
+    n = len(s["id"])
+    return files.BodySet(d.counts(), s["id"].astype(np.int32), [f"b{k}" for k in range(n)], s["type"].astype(np.int32),
+                         np.zeros(n), s["p"][:, 0].copy(), s["p"][:, 1].copy(), s["p"][:, 2].copy(), s["p"][:, 3].copy(),
+                         np.zeros(n, dtype=np.int32), np.zeros(n), s["r"].copy(), s["v"].copy())
+
+
+def drift_section(native, years, disk="powerlaw", report_years=None, tol=None, checkpoint_out=None, resume=None,
+                  wall_limit_s=None):
     """Energy and angular-momentum drift over `years` simulated years, driven like the
     reference's main loop: ejection check, step, collision check (merges at 1 x (R_i + R_j)),
     compaction after 8 events.  disk = "powerlaw": the 16384-body disk, FP64 with the
@@ -215,12 +291,27 @@ def drift_section(native, years, disk="powerlaw", report_years=None, tol=None):
     RK5(4) at tol 1e-10 (no Hill test: with RK5 any pair event accepts the attempt).  A merge
     changes the total energy by itself; the energy is measured on both sides of every
     collision pass, so those jumps are kept apart from the integration drift.  tol replaces
-    the disk's step tolerance."""
+    the disk's step tolerance.  checkpoint_out: write the state, time, trial step and the
+    run's totals there at the end (npz), stopping early once the run has taken wall_limit_s
+    seconds; resume: continue such a checkpoint up to `years` after the original start (the
+    state, time and trial step are the integrator's whole state, so the continued run steps
+    as an uninterrupted one would, apart from the count of events since the last compaction,
+    which restarts at 0)."""
     import numpy as np
-    from rednative import generate
+    from rednative import files, generate
+    ck = None
+    if resume:
+        ck = dict(np.load(resume, allow_pickle=False))
+        disk = str(ck["disk"])
+        tol = float(ck["tol"])
+        n = int(ck["r"].shape[0])
+        resumed = files.BodySet([int(x) for x in ck["counts"]], ck["id"].astype(np.int32),
+                                [f"b{k}" for k in range(n)], ck["type"].astype(np.int32), np.zeros(n), ck["mass"],
+                                ck["radius"], ck["density"], ck["cd"], np.zeros(n, dtype=np.int32), np.zeros(n),
+                                ck["r"], ck["v"])
     if disk in ("mixed", "mixed-disk-fp64"):
         prec = native.PREC_MIXED if disk == "mixed" else native.PREC_FP64
-        b = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
+        b = resumed if ck is not None else generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
         tol = 1e-10 if tol is None else tol
         d = native.Disk(b, integrator=3, adaptive=True, tolerance=tol, dt_day=1.0, precision=prec,
                         radii_enhance_factor=1.0, ejection=100.0, event_capacity=65536)
@@ -228,7 +319,7 @@ def drift_section(native, years, disk="powerlaw", report_years=None, tol=None):
                  + ("mixed precision" if disk == "mixed" else "FP64 sequential sums")
                  + f", Cash-Karp RK5(4) tol {tol:g}, collisions 1x(R_i+R_j) merged, ejection 100 AU")
     else:
-        b = generate.to_internal(generate.disk_powerlaw(n=16383, seed=202))
+        b = resumed if ck is not None else generate.to_internal(generate.disk_powerlaw(n=16383, seed=202))
         tol = 1e-12 if tol is None else tol
         d = native.Disk(b, integrator=4, adaptive=True, tolerance=tol, dt_day=0.01, use_graph=True,
                         radii_enhance_factor=1.0, encounter_hill_factor=3.0, ejection=100.0, event_capacity=65536)
@@ -239,10 +330,22 @@ def drift_section(native, years, disk="powerlaw", report_years=None, tol=None):
         report_years = years / 10.0
     E0, L0 = d.energy()
     t0 = d.t
-    t_stop = t0 + years * 365.25 * GAUSS
     jumps = 0.0
     checkpoints = []
-    next_report = report_years
+    wall_before = 0.0
+    done_before = dict(passed=0, tried=0, n_collision=0, n_close_encounter=0, n_ejection=0)
+    if ck is not None:
+        d.t = float(ck["t"])
+        d.dt_try = float(ck["dt_try"])
+        t0 = float(ck["t0"])
+        E0 = float(ck["E0"])
+        L0 = ck["L0"]
+        jumps = float(ck["jumps"])
+        checkpoints = [list(x) for x in ck["checkpoints"]]
+        wall_before = float(ck["wall"])
+        done_before = {k: int(ck["done_" + k]) for k in done_before}
+    t_stop = t0 + years * 365.25 * GAUSS
+    next_report = report_years * (1 + int(((d.t - t0) / GAUSS / 365.25) / report_years + 1e-9))
     w0 = time.perf_counter()
     while d.t <= t_stop:
         if d.check_for_ejection_hit_centrum():
@@ -261,13 +364,22 @@ def drift_section(native, years, disk="powerlaw", report_years=None, tol=None):
             drift = abs((E - E0 - jumps) / E0)
             checkpoints.append([round(yrs, 3), drift])
             print(f"drift: {yrs:.1f} yr, integration dE/E {drift:.3e}, merges dE/E {jumps / E0:.3e}, "
-                  f"{time.perf_counter() - w0:.0f} s", file=sys.stderr, flush=True)
+                  f"{time.perf_counter() - w0 + wall_before:.0f} s", file=sys.stderr, flush=True)
             next_report += report_years
+        if checkpoint_out and wall_limit_s and time.perf_counter() - w0 > wall_limit_s:
+            break
     d.sync()
-    sec = time.perf_counter() - w0
+    sec = time.perf_counter() - w0 + wall_before
     E1, L1 = d.energy()
-    c = d.counters()
+    c = {k: v + done_before.get(k, 0) for k, v in d.counters().items()}
     sim = (d.t - t0) / GAUSS / 365.25
+    if checkpoint_out:
+        st = _checkpoint_bodies(d)
+        np.savez(checkpoint_out, disk=disk, tol=tol, counts=np.array(st.counts), id=st.ids, type=st.types,
+                 mass=st.mass, radius=st.radius, density=st.density, cd=st.cd, r=st.r, v=st.v, t=d.t,
+                 dt_try=d.dt_try, t0=t0, E0=E0, L0=np.asarray(L0), jumps=jumps,
+                 checkpoints=np.array(checkpoints, dtype=np.float64).reshape(-1, 2), wall=sec,
+                 **{"done_" + k: c[k] for k in ("passed", "tried", "n_collision", "n_close_encounter", "n_ejection")})
     out = {"workload": label,
            "sim_years": sim, "seconds": sec, "sim_yr_per_hr": sim / (sec / 3600.0),
            "dE_over_E_integration": abs((E1 - E0 - jumps) / E0), "dE_over_E_merges": jumps / E0,
@@ -275,7 +387,12 @@ def drift_section(native, years, disk="powerlaw", report_years=None, tol=None):
            "dL_over_L": float(np.linalg.norm(L1 - L0) / np.linalg.norm(L0)),
            "steps": c["passed"], "attempts": c["tried"], "collisions": c["n_collision"],
            "close_encounters": c["n_close_encounter"], "ejections": c["n_ejection"], "bodies_left": d.n,
-           "checkpoints": checkpoints}
+           "checkpoints": checkpoints, "complete": bool(d.t > t_stop)}
+    if len(checkpoints) >= 2:
+        # linear drift rate from the checkpoints (least squares through the origin)
+        yy = np.array(checkpoints, dtype=np.float64)
+        out["dE_over_E_per_year"] = float(np.dot(yy[:, 0], yy[:, 1]) / np.dot(yy[:, 0], yy[:, 0]))
+        out["dE_over_E_1000yr_extrapolated"] = out["dE_over_E_per_year"] * 1000.0
     d.close()
     return out
 
@@ -368,6 +485,86 @@ def ensemble_section(native, days, sample, use_graph=True):
             "autotuned_block": block, "autotuned_jtile": jtile}
 
 
+PEAK_HBM_GBS = 8000.0         # MI355X HBM3E, vendor figure
+# Non-zero coefficients of the Cash-Karp stage rows 1..5 and of y_(n+1) (5th-order weights),
+# and of the error estimate (the differences of the two weight sets).
+CK_STAGE_TERMS = (1, 2, 3, 4, 5, 4)
+CK_ERR_TERMS = 5
+
+
+def hbm_rooflines(r, bodies, pmc_path):
+    """HBM roofline of the stage combinations, the error estimate and the ejection scan of a
+    timed run: algorithmic bytes (x, y, z of each vector read or written, 24 B per body and
+    vector: a stage row reads y and its k derivative records and writes the stage state, for
+    positions and velocities alike; the error estimate reads its derivative records of the
+    massive bodies; the scan reads x, y, z, id and type, 32 B per body) over the launches'
+    average duration.  These launches last a few microseconds, so the HIP events recorded
+    around each of them on the context stream add their own gaps (about 3 us): the duration
+    comes from the committed kernel trace of the same kernels when there is one
+    (duration_source), and the live HIP-event figure is reported beside it.  traffic: HBM
+    bytes per launch from the committed counter summary (FETCH_SIZE and WRITE_SIZE scaled by
+    a copy of known size)."""
+    n = bodies.n
+    nm = sum(bodies.counts[:6])
+    stage_attempt = sum(48 * n * (k + 2) for k in CK_STAGE_TERMS)
+    per_launch = {"k_stage": stage_attempt / len(CK_STAGE_TERMS), "k_err": 48.0 * nm * CK_ERR_TERMS,
+                  "k_ejection": 32.0 * n}
+    pmc = {}
+    try:
+        with open(pmc_path) as f:
+            pmc = json.load(f)
+    except (OSError, ValueError):
+        pass
+    out = {}
+    for key, cls in (("k_stage", "stage"), ("k_err", "err"), ("k_ejection", "eject")):
+        ms, cnt = r["prof"][cls]
+        if not cnt or ms <= 0.0:
+            continue
+        sec_ev = ms * 1e-3 / cnt
+        k = pmc.get(key, {})
+        sec = k["duration_us_mean"] * 1e-6 if k.get("duration_us_mean") else sec_ev
+        achieved = per_launch[key] / sec / 1e9
+        out[key] = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": achieved / PEAK_HBM_GBS, "traffic": k.get("hbm_bytes_per_launch"),
+                    "algorithmic_bytes_per_launch": per_launch[key], "launch_us": sec * 1e6,
+                    "duration_source": "kernel trace (profiles/pmc_stage_err_eject_n131072.md)"
+                    if k.get("duration_us_mean") else "HIP events",
+                    "launch_us_hip_events": sec_ev * 1e6, "achieved_hip_events": per_launch[key] / sec_ev / 1e9,
+                    "launches": cnt}
+    return out
+
+
+def load_drift(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def bound_tolerance_section(native, bodies, steps_mixed, steps_fp64, drift_path, tol=1e-11):
+    """Both paths on the same disk at the tolerance whose energy drift meets dE/E <= 1e-9
+    over 1000 yr (Cash-Karp's truncation drift is the same in both precisions; at 1e-10 it
+    extrapolates to 3.4e-9): simulated years per hour, their ratio, and the mixed path's
+    measured drift rate from the committed long run, extrapolated to 1000 yr."""
+    GAUSS_YR = GAUSS * 365.25
+    res = {"tolerance": tol}
+    for name, prec, steps in (("mixed", native.PREC_MIXED, steps_mixed), ("fp64", native.PREC_FP64, steps_fp64)):
+        r = run_path(native, bodies, prec, steps, 2, tol=tol)
+        yrs = r["sim_t"] / GAUSS_YR
+        res[name] = {"sim_yr_per_hr": yrs / (r["seconds"] / 3600.0), "ms_per_step": 1e3 * r["seconds"] / steps,
+                     "steps": steps, "attempts": r["tries"], "sim_days": r["sim_t"] / GAUSS,
+                     "dE_over_E_timed": abs((r["E1"] - r["E0"]) / r["E0"])}
+    res["speedup_sim_yr_per_hr_vs_fp64"] = res["mixed"]["sim_yr_per_hr"] / res["fp64"]["sim_yr_per_hr"]
+    dr = load_drift(drift_path)
+    if dr:
+        res["drift"] = {k: dr.get(k) for k in ("workload", "sim_years", "dE_over_E_integration",
+                                               "dE_over_E_per_year", "dE_over_E_1000yr_extrapolated",
+                                               "dL_over_L", "sim_yr_per_hr", "source")}
+        res["drift"]["label"] = "measured over sim_years, extrapolated to 1000 yr"
+    return res
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (the runtime's copy kernel): the reference
     point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -425,6 +622,15 @@ def main():
     ap.add_argument("--drift-disk", choices=["powerlaw", "mixed", "mixed-disk-fp64"], default="powerlaw")
     ap.add_argument("--drift-tol", type=float, default=None,
                     help="step tolerance of the drift run (default 1e-12 for powerlaw, 1e-10 otherwise)")
+    ap.add_argument("--drift-report-years", type=float, default=None)
+    ap.add_argument("--drift-checkpoint-out", default=None,
+                    help="write the drift run's state and totals there (npz) at its end")
+    ap.add_argument("--drift-wall-limit", type=float, default=None,
+                    help="with --drift-checkpoint-out: stop after this many seconds of the run")
+    ap.add_argument("--drift-resume", default=None, help="continue the drift run of this checkpoint")
+    ap.add_argument("--hbm-summary", default=os.path.join(ROOT, "profiles", "pmc_stage_err_eject_n131072.json"))
+    ap.add_argument("--drift-summary", default=os.path.join(ROOT, "profiles", "drift_mixed_n131072_tol1e-11.json"))
+    ap.add_argument("--no-bound", action="store_true", help="skip the section at tolerance 1e-11")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -449,7 +655,10 @@ def main():
                                                            fused_only=args.fused_only)}))
         return
     if args.only == "drift":
-        print(json.dumps({"drift": drift_section(native, args.drift_years, args.drift_disk, tol=args.drift_tol)}))
+        print(json.dumps({"drift": drift_section(native, args.drift_years, args.drift_disk, tol=args.drift_tol,
+                                                 report_years=args.drift_report_years,
+                                                 checkpoint_out=args.drift_checkpoint_out,
+                                                 resume=args.drift_resume, wall_limit_s=args.drift_wall_limit)}))
         return
 
     bodies = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
@@ -485,6 +694,8 @@ def main():
     prop = None if args.no_proposer else proposer_section(native, args.proposer_years)
     ens = None if args.no_ensemble else ensemble_section(native, args.ensemble_days, 16)
     small = None if args.no_aux else small_disk_section(native, with_cpu=not args.no_cpu)
+    bound = None if args.no_bound or args.no_fp64 else bound_tolerance_section(
+        native, bodies, max(10, args.steps // 3), max(4, args.fp64_steps // 3), args.drift_summary)
 
     line = {
         "metric": "pair-interactions/s & sim-yr/hr at N=131072, mixed vs FP64, ΔE/E≤1e-9",
@@ -521,6 +732,8 @@ def main():
                      "traffic_source": None if not traffic_src else traffic_src.get("source"),
                      "launch_ms": avg_ni_s * 1e3, "pairs_per_launch": ni_pairs,
                      "flop_per_pair": FLOP_PER_PAIR},
+        "roofline_hbm": hbm_rooflines(mixed, bodies, args.hbm_summary),
+        "at_bound_tolerance": bound,
         "profile_ms": {k: v[0] for k, v in mixed["prof"].items() if v[1]},
         "cpu_baseline": cpu,
         "proposer": prop,

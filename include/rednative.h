@@ -187,8 +187,10 @@ int red_step(red_ctx* ctx, red_step_info* info);
  * (RED_FUSED_STOP_EJECTION: the caller runs that iteration itself); after a step that
  * recorded pair events it stops with RED_FUSED_STOP_EVENTS (the step is done; the caller
  * runs the collision / close-encounter and rebuild checks).  *ps is the driver's
- * accumulated step (in and out).  For disks of at most 4096 bodies with FP64 sums in the
- * reference order and the barycentric frame; RED_ERR_UNSUPPORTED otherwise. */
+ * accumulated step (in and out).  For disks of at most 4096 bodies and 16384 interacting
+ * pairs per force evaluation (sum over the sweeps of sinks x sources; above that, launch-by-
+ * launch steps are faster), FP64 sums in the reference order and the barycentric frame;
+ * RED_ERR_UNSUPPORTED otherwise. */
 #define RED_FUSED_STOP_STEPS    1
 #define RED_FUSED_STOP_TIME     2
 #define RED_FUSED_STOP_OUTPUT   3
@@ -241,6 +243,7 @@ int red_host_transform_velocity(int n, double* v);
 #define RED_PROF_ERR    4   /* error estimate + max-reduction                        */
 #define RED_PROF_PACK   5   /* mixed-precision source packing                        */
 #define RED_PROF_LEARN  6   /* step features and learned step proposal               */
+#define RED_PROF_EJECT  7   /* ejection / hit-centrum scan                            */
 #define RED_PROF_CLASSES 8
 int red_set_profiling(red_ctx* ctx, int on);
 int red_get_profile(red_ctx* ctx, double ms[RED_PROF_CLASSES], int launches[RED_PROF_CLASSES], int reset);

@@ -1056,6 +1056,16 @@ int red_run_fixed(red_ctx* c, int max_steps, double t_stop, double output_interv
                        "red_run_fixed: needs FP64 sums in the reference order and the barycentric frame");
     if (c->n > FUSED_MAX_BODIES || S.n_main > FUSED_MAX_ROWS)
         return c->fail(RED_ERR_UNSUPPORTED, "red_run_fixed: more than 4096 bodies");
+    {
+        Sweep sw[3];
+        sweeps_of(c, sw);
+        long pairs = 0;
+        for (const Sweep& s : sw)
+            if (s.sink1 > s.sink0) pairs += (long)(s.sink1 - s.sink0) * (s.src1 - s.src0);
+        if (pairs > FUSED_MAX_PAIRS)
+            return c->fail(RED_ERR_UNSUPPORTED, "red_run_fixed: more than " + std::to_string(FUSED_MAX_PAIRS) +
+                                                    " interacting pairs per evaluation (stepwise is faster)");
+    }
     if (!c->d_fused_rows || !c->d_fused_out) {
         SetupLock lk;
         if (!c->d_fused_rows) CTX_HIP(c, dmalloc(&c->d_fused_rows, 2 * FUSED_MAX_ROWS));
@@ -1162,8 +1172,10 @@ int red_check_ejection_hit_centrum(red_ctx* c, int* handled)
     if (!(ej > 0.0) && !(hc > 0.0)) return RED_OK;
     hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(1), 0, c->stream, c->d_ctl, c->t, c->dt_try, 0, 0, 0, 1);
     CTX_HIP(c, hipGetLastError());
-    CTX_HIP(c, launch_ejection_scan(c->d_r[c->cur], c->d_v[c->cur], c->d_id, c->d_type, c->n, ej, hc, c->d_ctl,
-                                    c->d_events_hc, std::max(1, c->n), c->stream));
+    CTX_HIP(c, timed(c, RED_PROF_EJECT, [&] {
+        return launch_ejection_scan(c->d_r[c->cur], c->d_v[c->cur], c->d_id, c->d_type, c->n, ej, hc, c->d_ctl,
+                                    c->d_events_hc, std::max(1, c->n), c->stream);
+    }));
     int rc = read_ctl(c);
     if (rc) return rc;
     const int ne = c->h_ctl->n_events_hc;

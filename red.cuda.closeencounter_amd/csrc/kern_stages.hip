@@ -21,7 +21,9 @@
 // with thrust::max_element, integrator.cu:211-256):
 //   EF_RK4      e = f3 - f4                         (int_rungekutta4.cu:63-74)
 //   EF_RKF78    e = f0 + f10 - f11 - f12            (int_rungekutta8.cu:307-319)
-//   EF_CK       e = sum_k (bh_k - b_k) f_k          (int_rungekutta5.cu:79-92)
+//   EF_CK       e = sum_k (bh_k - b_k) f_k          (int_rungekutta5.cu:79-92); the zero
+//               coefficient of f1 is left out (0 f1 adds a zero, which cannot change a maximum
+//               taken from 0)
 // The maximum starts from 0 and ignores NaN like the reference CPU loop; a NaN sets a
 // flag the host turns into an error.
 #include "red_internal.hpp"
@@ -30,7 +32,8 @@
 namespace red {
 namespace {
 
-template <int FORM>
+// NT > 0: the form's term count, known at compile time (combine_nt); 0: any count.
+template <int FORM, int NT>
 __global__ __launch_bounds__(256) void k_stage(StageArgs A)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -45,18 +48,38 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A)
     }
     const d4 yr = A.yr[i];
     const d4 yv = A.yv[i];
-    d4 orr = combine<FORM>(A, yr, i, false, dt);
-    d4 ovv = combine<FORM>(A, yv, i, true, dt);
+    d4 orr, ovv;
+    if constexpr (NT > 0) {
+        orr = combine_nt<FORM, NT>(A, yr, i, false, dt);
+        ovv = combine_nt<FORM, NT>(A, yv, i, true, dt);
+    } else {
+        orr = combine<FORM>(A, yr, i, false, dt);
+        ovv = combine<FORM>(A, yv, i, true, dt);
+    }
     orr.w = yr.w;   // mass rides along unchanged
     ovv.w = 0.0;
     A.outr[i] = orr;
     A.outv[i] = ovv;
 }
 
-template <int EF>
+template <int EF, int NT = 0>
 __device__ __forceinline__ double3 err_rec(const StageArgs& A, int i, bool vel)
 {
     double3 e;
+    if constexpr (EF == EF_CK && NT > 0) {
+        // the term count known at compile time: every record requested before the first is used
+        d4 f[NT];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) f[k] = vel ? A.t[k].fa[i] : A.t[k].fv[i];
+        e.x = A.t[0].coef * f[0].x; e.y = A.t[0].coef * f[0].y; e.z = A.t[0].coef * f[0].z;
+#pragma unroll
+        for (int k = 1; k < NT; ++k) {
+            e.x += A.t[k].coef * f[k].x;
+            e.y += A.t[k].coef * f[k].y;
+            e.z += A.t[k].coef * f[k].z;
+        }
+        return e;
+    }
     if (EF == EF_RK4) {
         const d4 f3 = vel ? A.t[0].fa[i] : A.t[0].fv[i];
         const d4 f4 = vel ? A.t[1].fa[i] : A.t[1].fv[i];
@@ -124,7 +147,7 @@ __global__ __launch_bounds__(256) void k_err_disks(StageArgs A)
     }
 }
 
-template <int EF>
+template <int EF, int NT>
 __global__ __launch_bounds__(256) void k_err(StageArgs A, Ctl* ctl)
 {
     __shared__ double wmax[4];
@@ -133,8 +156,8 @@ __global__ __launch_bounds__(256) void k_err(StageArgs A, Ctl* ctl)
     double m = 0.0;
     int nan = 0;
     if (i < A.n) {
-        const double3 er = err_rec<EF>(A, i, false);
-        const double3 ev = err_rec<EF>(A, i, true);
+        const double3 er = err_rec<EF, NT>(A, i, false);
+        const double3 ev = err_rec<EF, NT>(A, i, true);
         upd_max(m, nan, er.x); upd_max(m, nan, er.y); upd_max(m, nan, er.z);
         upd_max(m, nan, ev.x); upd_max(m, nan, ev.y); upd_max(m, nan, ev.z);
     }
@@ -165,19 +188,47 @@ __global__ void k_reset_ctl(Ctl* ctl, int clear_events)
 
 }  // namespace
 
+namespace {
+
+template <int FORM, int NT>
+hipError_t go_stage(const StageArgs& A, dim3 g, dim3 b, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_stage<FORM, NT>), g, b, 0, s, A);
+    return hipGetLastError();
+}
+
+// the forms with a term loop, specialised on the row's term count
+template <int FORM>
+hipError_t go_stage_n(const StageArgs& A, dim3 g, dim3 b, hipStream_t s)
+{
+    switch (A.nterms) {
+    case 1: return go_stage<FORM, 1>(A, g, b, s);
+    case 2: return go_stage<FORM, 2>(A, g, b, s);
+    case 3: return go_stage<FORM, 3>(A, g, b, s);
+    case 4: return go_stage<FORM, 4>(A, g, b, s);
+    case 5: return go_stage<FORM, 5>(A, g, b, s);
+    case 6: return go_stage<FORM, 6>(A, g, b, s);
+    case 7: return go_stage<FORM, 7>(A, g, b, s);
+    case 8: return go_stage<FORM, 8>(A, g, b, s);
+    case 9: return go_stage<FORM, 9>(A, g, b, s);
+    default: return go_stage<FORM, 0>(A, g, b, s);
+    }
+}
+
+}  // namespace
+
 hipError_t launch_stage(int form, const StageArgs& A, hipStream_t s)
 {
     if (A.n <= 0) return hipSuccess;
     dim3 g((A.n + 255) / 256), b(256);
     switch (form) {
-    case SF_AXPY: hipLaunchKernelGGL(k_stage<SF_AXPY>, g, b, 0, s, A); break;
-    case SF_DTSUM: hipLaunchKernelGGL(k_stage<SF_DTSUM>, g, b, 0, s, A); break;
-    case SF_ACCUM: hipLaunchKernelGGL(k_stage<SF_ACCUM>, g, b, 0, s, A); break;
-    case SF_RK4NP1: hipLaunchKernelGGL(k_stage<SF_RK4NP1>, g, b, 0, s, A); break;
-    case SF_RKF78NP1: hipLaunchKernelGGL(k_stage<SF_RKF78NP1>, g, b, 0, s, A); break;
+    case SF_AXPY: return go_stage<SF_AXPY, 0>(A, g, b, s);
+    case SF_DTSUM: return go_stage_n<SF_DTSUM>(A, g, b, s);
+    case SF_ACCUM: return go_stage_n<SF_ACCUM>(A, g, b, s);
+    case SF_RK4NP1: return go_stage<SF_RK4NP1, 0>(A, g, b, s);
+    case SF_RKF78NP1: return go_stage<SF_RKF78NP1, 0>(A, g, b, s);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_err(int form, const StageArgs& A, hipStream_t s)
@@ -195,9 +246,13 @@ hipError_t launch_err(int form, const StageArgs& A, hipStream_t s)
     }
     Ctl* ctl = const_cast<Ctl*>(A.ctl);
     switch (form) {
-    case EF_RK4: hipLaunchKernelGGL(k_err<EF_RK4>, g, b, 0, s, A, ctl); break;
-    case EF_RKF78: hipLaunchKernelGGL(k_err<EF_RKF78>, g, b, 0, s, A, ctl); break;
-    case EF_CK: hipLaunchKernelGGL(k_err<EF_CK>, g, b, 0, s, A, ctl); break;
+    case EF_RK4: hipLaunchKernelGGL((k_err<EF_RK4, 0>), g, b, 0, s, A, ctl); break;
+    case EF_RKF78: hipLaunchKernelGGL((k_err<EF_RKF78, 0>), g, b, 0, s, A, ctl); break;
+    case EF_CK:
+        if (A.nterms == 5) hipLaunchKernelGGL((k_err<EF_CK, 5>), g, b, 0, s, A, ctl);
+        else if (A.nterms == 6) hipLaunchKernelGGL((k_err<EF_CK, 6>), g, b, 0, s, A, ctl);
+        else hipLaunchKernelGGL((k_err<EF_CK, 0>), g, b, 0, s, A, ctl);
+        break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -199,6 +199,14 @@ hipError_t launch_energy(const d4* r, const d4* v, const double* mass, const int
 // ---- fused fixed-step loop: main-loop iterations of one small disk in one work-group ----
 constexpr int FUSED_MAX_ROWS = MAX_TERMS;     // stage rows after f0, plus y_(n+1)
 constexpr int FUSED_MAX_BODIES = 4096;
+// One work-group runs the whole loop, so it pays only while the evaluations are small.
+// Measured on MI355X (RK4 steps; stepwise = one graph replay per step, to which the
+// driver loop adds about 50 us of per-step ejection and collision checks): 256-body disk,
+// 1527 interacting pairs per evaluation: 25 us fused, 55 us stepwise; 128 self-interacting
+// bodies, 16384 pairs: 95 against 40 (+50); 256 of them, 65536 pairs: 181 against 47 (+50);
+// 1024 of them: 2.8 ms against 64 us.  Above 16384 pairs (sum over the sweeps of sinks x
+// sources) the driver loop is faster launch by launch.
+constexpr long FUSED_MAX_PAIRS = 16384;
 constexpr int FUSED_STEPS_PER_LAUNCH = 4096;  // the host relaunches for longer runs
 struct FusedOut {
     double t, ps;                  // time and accumulated output step after the last step

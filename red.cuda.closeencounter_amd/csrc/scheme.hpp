@@ -89,7 +89,8 @@ inline Scheme make_scheme(int integrator, bool adaptive)
         S.np1.form = SF_ACCUM;
         for (int s = 0; s < 6; s++) if (bh[s] != 0.0) S.np1.add(s, bh[s]);
         S.err_form = EF_CK;
-        for (int s = 0; s < 6; s++) S.err.add(s, bh[s] - b[s]);
+        for (int s = 0; s < 6; s++)
+            if (bh[s] - b[s] != 0.0) S.err.add(s, bh[s] - b[s]);
         S.lambda = 1.0; S.exponent = 1.0 / 5.0;
         S.accept_on_event = true;
         break;

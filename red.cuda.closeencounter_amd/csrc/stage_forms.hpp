@@ -69,4 +69,40 @@ __device__ __forceinline__ d4 combine(const StageArgs& A, const d4& y, int i, bo
     return o;
 }
 
+// SF_ACCUM and SF_DTSUM with the term count known at compile time: every derivative record
+// is requested before the first is used (the loop of combine waits for each in turn), then
+// the terms are added with the same operations in the same order as combine.
+template <int FORM, int NT>
+__device__ __forceinline__ d4 combine_nt(const StageArgs& A, const d4& y, int i, bool vel, double dt)
+{
+    static_assert(FORM == SF_ACCUM || FORM == SF_DTSUM, "forms with a term loop");
+    d4 f[NT];
+#pragma unroll
+    for (int k = 0; k < NT; ++k) f[k] = vel ? A.t[k].fa[i] : A.t[k].fv[i];
+    d4 o;
+    o.w = y.w;
+    if (FORM == SF_ACCUM) {
+        o.x = y.x; o.y = y.y; o.z = y.z;
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            const double c = dt * A.t[k].coef;
+            o.x += c * f[k].x;
+            o.y += c * f[k].y;
+            o.z += c * f[k].z;
+        }
+    } else {
+        double sx = A.t[0].coef * f[0].x, sy = A.t[0].coef * f[0].y, sz = A.t[0].coef * f[0].z;
+#pragma unroll
+        for (int k = 1; k < NT; ++k) {
+            sx = sx + A.t[k].coef * f[k].x;
+            sy = sy + A.t[k].coef * f[k].y;
+            sz = sz + A.t[k].coef * f[k].z;
+        }
+        o.x = y.x + dt * sx;
+        o.y = y.y + dt * sy;
+        o.z = y.z + dt * sz;
+    }
+    return o;
+}
+
 }  // namespace red

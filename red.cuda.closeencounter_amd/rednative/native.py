@@ -440,7 +440,7 @@ class Disk:
                 "n_close_encounter", "n_event_last_clear")
         return dict(zip(keys, (int(x) for x in c)))
 
-    PROF_CLASSES = ("si", "nsi", "ni", "stage", "err", "pack", "learn", "r7")
+    PROF_CLASSES = ("si", "nsi", "ni", "stage", "err", "pack", "learn", "eject")
 
     def set_profiling(self, on: bool):
         self._chk(self.L.red_set_profiling(self.ctx, int(on)))

@@ -85,24 +85,35 @@ def test_small_disk_full_length_bit_exact(oracle_mod, native_mod, hill):
     assert len(got[0]) > 20
 
 
-@pytest.mark.parametrize("maker", ["few_sinks_many_tiles", "per_sink_many_tiles"])
-def test_fused_loop_multi_tile_sweeps(oracle_mod, native_mod, maker):
-    """Sweeps that need more than one LDS tile of sources in the fused loop: 3 sinks against
-    1203 sources (few-sinks mode, 512 sources per tile) and a power-law disk whose ~68
-    self-interacting sinks see 2101 sources (one sink per work-item, 1536 sources per tile)."""
-    if maker == "few_sinks_many_tiles":
-        b = generate.to_internal(generate.disk_small(n_pl=1200, seed=17))
-    else:
-        b = generate.to_internal(generate.disk_powerlaw(n=2100, seed=18))
-    nsi = sum(b.counts[:4])
-    print(f"case fused multi-tile {maker} n={b.n} n_si={nsi}", flush=True)
-    assert (nsi <= 64 and b.n > 1024) or (nsi > 64 and b.n > 1536)
+def test_fused_loop_multi_tile_sweep(oracle_mod, native_mod):
+    """A sweep that needs more than one LDS tile of sources in the fused loop: the star and
+    two giants against 1203 sources (few-sinks mode, 512 sources per tile)."""
+    b = generate.to_internal(generate.disk_small(n_pl=1200, seed=17))
+    print(f"case fused multi-tile n={b.n}", flush=True)
     kw = dict(integrator=2, adaptive=False, dt_day=0.5, radii_enhance_factor=1.0)
     prm = loop_prm(0.5, 40, 10, radii_enhance_factor=1.0)
     ref = run_loop(oracle_mod.Oracle(b, **kw), prm)
     got = run_loop(native_mod.Disk(b, **kw), prm)
     compare_runs(ref, got)
-    assert got[2]["fused_steps"] > 0
+    assert got[2]["fused_steps"] == got[2]["steps"] == 40
+
+
+def test_fused_loop_refuses_large_evaluations(oracle_mod, native_mod):
+    """Above 16384 interacting pairs per evaluation the fused loop is refused (stepping launch
+    by launch is faster there) and the main loop steps one at a time: a power-law disk of
+    2100 bodies, 70 of them self-interacting (289k pairs per evaluation)."""
+    b = generate.to_internal(generate.disk_powerlaw(n=2100, seed=18))
+    print(f"case fused refused n={b.n}", flush=True)
+    kw = dict(integrator=2, adaptive=False, dt_day=0.5, radii_enhance_factor=1.0)
+    d = native_mod.Disk(b, **kw)
+    with pytest.raises(native_mod.RedError) as ex:
+        d.run_fixed(10, 1e9, 1e9)
+    assert ex.value.code == native_mod.ERR_UNSUPPORTED
+    prm = loop_prm(0.5, 20, 10, radii_enhance_factor=1.0)
+    ref = run_loop(oracle_mod.Oracle(b, **kw), prm)
+    got = run_loop(native_mod.Disk(b, **kw), prm)
+    compare_runs(ref, got)
+    assert got[2]["fused_steps"] == 0 and got[2]["steps"] == 20
 
 
 @pytest.mark.parametrize("integ", [0, 1, 2, 3, 4])
