@@ -224,6 +224,16 @@ int red_counters(red_ctx* ctx, int out[8]);
 /* Not in the reference: total energy E = sum 1/2 m v^2 - sum m_i m_j / r_ij over
  * interacting pairs (NSI-NSI excluded) and angular momentum L = sum m r x v, G = 1. */
 int red_energy(red_ctx* ctx, double* E, double L[3]);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): red_energy
+ * without any allocation or host synchronisation.  Its potential partials live in a static
+ * 1 MiB device array per GPU (disks whose n_si ceil(n_massive / 4096) partials exceed 2^17
+ * doubles, e.g. more than about 20000 self-interacting bodies, are refused with RED_ERR_ARG);
+ * the sums are formed on the device in
+ * red_energy's order, so out_dev[4] = {E, Lx, Ly, Lz} equals red_energy's results bit for bit.
+ * out_dev is caller-owned device memory; the call is asynchronous on the context stream
+ * (red_sync waits).  Calls that overlap on one GPU share the array and race. */
+int red_energy_noalloc(red_ctx* ctx, double* out_dev);
 
 /* Host-side transforms (tools.cpp:122-178, pp_disk.cu:1512-1551): barycentric shift
  * (sums run from the last body down, as the reference), and t*k / v/k scaling. */

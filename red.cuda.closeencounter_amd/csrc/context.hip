@@ -1375,6 +1375,22 @@ int red_energy(red_ctx* c, double* E, double L[3])
     return RED_OK;
 }
 
+int red_energy_noalloc(red_ctx* c, double* out_dev)
+{
+    if (!c || !out_dev) return RED_ERR_ARG;
+    const int nm = c->nMassive(), nsi = c->nSI();
+    const size_t need = (size_t)std::max(nsi, 0) * energy_chunks(nm);
+    if (need > ENERGY_SCRATCH_DOUBLES)
+        return c->fail(RED_ERR_ARG, "red_energy_noalloc: the disk needs " + std::to_string(need) +
+                                        " doubles of scratch, more than the static buffer's " +
+                                        std::to_string(ENERGY_SCRATCH_DOUBLES) + " (use red_energy)");
+    double* scr = nullptr;
+    CTX_HIP(c, energy_scratch_ptr(c->prm.device, &scr));
+    CTX_HIP(c, launch_energy_noalloc(c->d_r[c->cur], c->d_v[c->cur], c->d_mass, c->d_id, nm, nsi, scr, out_dev,
+                                     c->stream));
+    return RED_OK;
+}
+
 int red_set_profiling(red_ctx* c, int on)
 {
     if (!c) return RED_ERR_ARG;

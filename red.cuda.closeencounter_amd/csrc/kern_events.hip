@@ -13,8 +13,18 @@
 // the potential sum_{j>i, interacting} m_i m_j / r_ij in partials per (SI body, source
 // chunk), all added on the host in a fixed order.
 #include <algorithm>
+#include <mutex>
 
 #include "red_internal.hpp"
+
+// Scratch of the allocation-free energy entry point (red_energy_noalloc): the potential
+// partials per (SI body, source chunk) that red_energy keeps in a buffer the library
+// allocates.  One copy per GPU, held until the process ends; every call on a GPU reuses it,
+// so calls that overlap on one GPU race.  Host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (energy_scratch_ptr below).  It carries
+// an initialiser, so the array is stored in the code object's file image like the rest of
+// its data: the code object then has no zero-filled segment larger than its file.
+static __device__ __align__(256) double scratch[red::ENERGY_SCRATCH_DOUBLES] = {0.0, 0.0, 0.0, 1.0};
 
 namespace red {
 namespace {
@@ -122,7 +132,114 @@ __global__ __launch_bounds__(256) void k_energy_pairs(const d4* __restrict__ r, 
     if (threadIdx.x == 0) part[(size_t)i * nch + blockIdx.x] = red[0];
 }
 
+__device__ __forceinline__ double lane_value(double x, int l)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// red_energy's sums on the device, in the same order: the kinetic energy and angular
+// momentum of every massive body in index order (formed here from the state with k_energy's
+// expressions, so no per-body partials are stored), the potential partials in (SI body,
+// chunk) order, E = T - U.  One wave: lane k forms the terms of body b + k of a block of 64
+// (the next block is loaded before the current one is added), then every lane adds the
+// block's terms lane by lane.
+__global__ __launch_bounds__(64) void k_energy_sum(const d4* __restrict__ r, const d4* __restrict__ v,
+                                                   const double* __restrict__ mass, const int* __restrict__ id,
+                                                   int n_massive, const double* __restrict__ part, long n_part,
+                                                   double* __restrict__ res)
+{
+    const int lane = threadIdx.x;
+    double T = 0.0, lx = 0.0, ly = 0.0, lz = 0.0, U = 0.0;
+    struct Body {
+        d4 q, w;
+        double m;
+        bool on;
+    };
+    auto load = [&](int i) {
+        Body b{};
+        b.on = i < n_massive && id[i] > 0;
+        if (b.on) {
+            b.q = r[i];
+            b.w = v[i];
+            b.m = mass[i];
+        }
+        return b;
+    };
+    Body cur = load(lane);
+    for (int b = 0; b < n_massive; b += 64) {
+        const Body nxt = load(b + 64 + lane);
+        double t = 0.0, x = 0.0, y = 0.0, z = 0.0;
+        if (cur.on) {
+            const d4 q = cur.q, w = cur.w;
+            const double m = cur.m;
+            t = 0.5 * m * ((w.x * w.x) + (w.y * w.y) + (w.z * w.z));
+            x = m * (q.y * w.z - q.z * w.y);
+            y = m * (q.z * w.x - q.x * w.z);
+            z = m * (q.x * w.y - q.y * w.x);
+        }
+        const int cnt = min(64, n_massive - b);
+        for (int l = 0; l < cnt; ++l) {
+            T += lane_value(t, l);
+            lx += lane_value(x, l);
+            ly += lane_value(y, l);
+            lz += lane_value(z, l);
+        }
+        cur = nxt;
+    }
+    double u = lane < n_part ? part[lane] : 0.0;
+    for (long b = 0; b < n_part; b += 64) {
+        const double un = b + 64 + lane < n_part ? part[b + 64 + lane] : 0.0;
+        const int cnt = (int)min(64L, n_part - b);
+        for (int l = 0; l < cnt; ++l) U += lane_value(u, l);
+        u = un;
+    }
+    if (lane == 0) {
+        res[0] = T - U;
+        res[1] = lx;
+        res[2] = ly;
+        res[3] = lz;
+    }
+}
+
 }  // namespace
+
+hipError_t energy_scratch_ptr(int device, double** out)
+{
+    static std::mutex m;
+    static double* ptrs[64] = {};
+    *out = nullptr;
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(m);
+    if (!ptrs[device]) {
+        int prev = 0;
+        RED_HIP_TRY(hipGetDevice(&prev));
+        if (prev != device) RED_HIP_TRY(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != device) (void)hipSetDevice(prev);
+        RED_HIP_TRY(e);
+        ptrs[device] = static_cast<double*>(p);
+    }
+    *out = ptrs[device];
+    return hipSuccess;
+}
+
+hipError_t launch_energy_noalloc(const d4* r, const d4* v, const double* mass, const int* id, int n_massive,
+                                 int n_si, double* scratch_dev, double* res, hipStream_t s)
+{
+    const int nm = std::max(n_massive, 0);
+    const long n_part = nm > 0 && n_si > 0 ? (long)n_si * energy_chunks(nm) : 0;
+    if (n_part > 0) {
+        hipLaunchKernelGGL(k_energy_pairs, dim3(energy_chunks(nm), n_si), dim3(256), 0, s, r, mass, id, nm,
+                           scratch_dev);
+        RED_HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(64), 0, s, r, v, mass, id, nm, scratch_dev, n_part, res);
+    return hipGetLastError();
+}
 
 hipError_t launch_ejection_scan(const d4* r, const d4* v, int* id, const int* type, int n, double ej, double hc,
                                 Ctl* ctl, red_event* buf, int cap, hipStream_t s)

@@ -195,6 +195,14 @@ hipError_t launch_apply_updates(const BodyUpdate* u, int nu, d4* r, d4* v, doubl
 int energy_chunks(int n_massive);
 hipError_t launch_energy(const d4* r, const d4* v, const double* mass, const int* id, int n_massive, int n_si,
                          double* out6, double* part, hipStream_t s);
+// Allocation-free variant: the potential partials in a static device array of
+// ENERGY_SCRATCH_DOUBLES (1 MiB per GPU), the sums of red_energy done on the device in the
+// same order into res[4] = {E, Lx, Ly, Lz}.  energy_scratch_ptr looks the array's address
+// up once per device.
+constexpr size_t ENERGY_SCRATCH_DOUBLES = (size_t)1 << 17;
+hipError_t energy_scratch_ptr(int device, double** out);
+hipError_t launch_energy_noalloc(const d4* r, const d4* v, const double* mass, const int* id, int n_massive,
+                                 int n_si, double* scratch_dev, double* res, hipStream_t s);
 
 // ---- fused fixed-step loop: main-loop iterations of one small disk in one work-group ----
 constexpr int FUSED_MAX_ROWS = MAX_TERMS;     // stage rows after f0, plus y_(n+1)

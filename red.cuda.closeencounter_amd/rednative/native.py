@@ -106,6 +106,7 @@ SIGNATURES = {
     "red_last_rebuild_map": (_I, [_P, _IP, _I, _IP]),
     "red_counters": (_I, [_P, _IP]),
     "red_energy": (_I, [_P, _DP, _DP]),
+    "red_energy_noalloc": (_I, [_P, _P]),
     "red_host_transform_to_bc": (_I, [_I, _P, _P, _P]),
     "red_host_transform_to_ac": (_I, [_I, _P, _P, _P]),
     "red_host_transform_velocity": (_I, [_I, _P]),
