@@ -258,14 +258,37 @@ def read_result_snapshot(path: str) -> Tuple[float, BodySet]:
     )
 
 
+def _c_e16(x: float) -> str:
+    """x as C++ iostream prints it with scientific, precision 16 and width 25 (a NaN keeps its
+    sign, as printf's "%e" does)."""
+    if x != x:
+        return ("-nan" if math.copysign(1.0, x) < 0 else "nan").rjust(25)
+    return f"{x: .16e}".rjust(25)
+
+
 def format_result_line(id_, name, btype, t_day, p4, mig_type, mig_stop, r, v_day) -> str:
     """One line of pp_disk.cu:1707-1722 (C++ iostream scientific, precision 16)."""
-    def e(x):
-        return f"{x: .16e}".rjust(25)
+    e = _c_e16
     fields = [str(int(id_)).rjust(8), name.rjust(30), str(int(btype)).rjust(2), e(t_day),
               e(p4[0]), e(p4[1]), e(p4[2]), e(p4[3]), str(int(mig_type)).rjust(2), e(mig_stop),
               e(r[0]), e(r[1]), e(r[2]), e(v_day[0]), e(v_day[1]), e(v_day[2])]
     return " ".join(fields)
+
+
+def format_event_line(e: Dict) -> str:
+    """One line of the event file (pp_disk.cu:1786-1837): name, t in days, d, id1, id2, then
+    for the survivor before, the merger before and the survivor after: mass, density,
+    radius, x y z, vx vy vz in AU/day and the six orbital elements (C++ iostream
+    scientific, precision 16)."""
+    e16 = _c_e16
+    f = [EVENT_NAMES[e["name"]].rjust(16), e16(e["t"] / GAUSS), e16(e["d"]), str(int(e["id1"])).rjust(8),
+         str(int(e["id2"])).rjust(8)]
+    for p, r, v, oe in (("p1", "r1", "v1", "oe1"), ("p2", "r2", "v2", "oe2"), ("ps", "rs", "vs", "oes")):
+        f += [e16(e[p][0]), e16(e[p][2]), e16(e[p][1])]
+        f += [e16(x) for x in e[r][:3]]
+        f += [e16(x * GAUSS) for x in e[v][:3]]
+        f += [e16(x) for x in e[oe][:6]]
+    return " ".join(f)
 
 
 def mutual_hill_radius(m_i: float, m_j: float, m_star: float, ri: float, rj: float) -> float:

@@ -249,3 +249,159 @@ def test_driver_restart_from_snapshot(tmp_path, native_mod):
     assert np.array_equal(res.ids, full.ids)
     assert np.allclose(res.r, full.r, rtol=0, atol=1e-12)
     assert np.allclose(res.v, full.v, rtol=0, atol=1e-13)
+
+
+def _oracle_loop(oracle_mod, body_path, prm, dt_day):
+    """The same run through the oracle and the Python restatement of the driver loop:
+    snapshots (time, state) and the events handed to the event file, in order."""
+    from rednative import mainloop
+    bi = generate.to_internal(files.read_body_file(body_path))
+    o = oracle_mod.Oracle(bi, integrator=prm["integrator"], adaptive=prm["adaptive"], tolerance=prm["tolerance"],
+                          dt_day=dt_day, hit_centrum=prm["hit_centrum"], ejection=prm["ejection"],
+                          radii_enhance_factor=prm["radii_enhance_factor"],
+                          encounter_hill_factor=prm["encounter_hill_factor"])
+    o.t = prm["start_time"]
+    snaps, events = [], []
+    mainloop.run_main_loop(o, prm, lambda e: snaps.append((e.t, e.state())), lambda evs: events.extend(evs))
+    return bi, snaps, events
+
+
+@pytest.mark.gpu
+def test_driver_event_file_all_columns_match_oracle(tmp_path, oracle_mod):
+    """Every one of the 50 columns of every event-file line (pp_disk.cu:1786-1837: name, t, d,
+    id1, id2, then mass, density, radius, position, velocity and six orbital elements of the
+    survivor before, the merger before and the survivor after) equals the oracle's record of
+    the same run, printed in the reference's format."""
+    b = generate.disk_allsi(150, seed=17, a_range=(1.0, 1.05), m_range=(1e-10, 1e-8))
+    files.write_body_file(str(tmp_path / "disk.txt"), b)
+    (tmp_path / "p.txt").write_text("integrator = rk4\nlength = 0.2\noutput_interval = 0.05\n"
+                                    "collision_factor = 300\nejection = 50\ndt = 0.5\n")
+    print("case driver event file columns", flush=True)
+    r = run_red(["-iDir", str(tmp_path), "-p", "p.txt", "-ic", "disk.txt"])
+    assert r.returncode == 0, r.stderr
+    lines = [ln.split() for ln in open(tmp_path / "event.txt") if ln.strip()]
+    prm = files.read_parameters(str(tmp_path / "p.txt"))
+    _, _, events = _oracle_loop(oracle_mod, str(tmp_path / "disk.txt"), prm, prm["dt"])
+    assert len(lines) == len(events) > 0
+    for k, (line, e) in enumerate(zip(lines, events)):
+        want = files.format_event_line(e).split()
+        assert len(line) == len(want) == 50, (k, len(line))
+        assert line == want, (k, [(c, a, w) for c, (a, w) in enumerate(zip(line, want)) if a != w][:4])
+
+
+@pytest.mark.gpu
+def test_driver_binary_result_matches_text(tmp_path):
+    """output_type = binary (pp_disk.cu:1727-1770): per snapshot the time in days, then per
+    active body id, name (NUL-terminated), type, mass, radius, density, cd, migration type
+    and stop, position and velocity (AU/day); the same run in text holds the same values."""
+    import struct
+    b = generate.disk_allsi(150, seed=17, a_range=(1.0, 1.05), m_range=(1e-10, 1e-8))
+    print("case driver binary result", flush=True)
+    out = {}
+    for kind in ("text", "binary"):
+        d = tmp_path / kind
+        d.mkdir()
+        files.write_body_file(str(d / "disk.txt"), b)
+        (d / "p.txt").write_text(f"integrator = rk4\nlength = 0.2\noutput_interval = 0.05\ncollision_factor = 300\n"
+                                 f"dt = 0.5\noutput_type = {kind}\n")
+        r = run_red(["-iDir", str(d), "-p", "p.txt", "-ic", "disk.txt"])
+        assert r.returncode == 0, r.stderr
+        out[kind] = d
+    rows = files.parse_result_ascii(str(out["text"] / "result.txt"))
+    raw = (out["binary"] / "result.bin").read_bytes()
+    # group the text rows into snapshots (consecutive rows sharing the time column)
+    snaps = []
+    for row in rows:
+        if snaps and snaps[-1][0] == row[3]:
+            snaps[-1][1].append(row)
+        else:
+            snaps.append((row[3], [row]))
+    assert len(snaps) >= 5 and len(snaps[-1][1]) < len(snaps[0][1])   # collisions removed bodies
+    pos = 0
+
+    def take(fmt):
+        nonlocal pos
+        v = struct.unpack_from("<" + fmt, raw, pos)
+        pos += struct.calcsize("<" + fmt)
+        return v
+
+    for t, recs in snaps:
+        (tb,) = take("d")
+        assert tb == float(t)
+        for row in recs:
+            (bid,) = take("i")
+            end = raw.index(b"\0", pos)
+            name = raw[pos:end].decode()
+            pos = end + 1
+            (btype,) = take("i")
+            p4 = take("4d")
+            (mig,) = take("i")
+            (mig_stop,) = take("d")
+            rv = take("6d")
+            assert (bid, name, btype, mig) == (int(row[0]), row[1], int(row[2]), int(row[8]))
+            assert list(p4) == [float(x) for x in row[4:8]] and mig_stop == float(row[9])
+            assert list(rv) == [float(x) for x in row[10:16]]
+    assert pos == len(raw)
+
+
+@pytest.mark.gpu
+def test_driver_reference_rkf78_input(tmp_path, oracle_mod, native_mod):
+    """The reference's own RKF7(8) input (TestRun/InputTest/Release/N1_massive_N3_test: star,
+    10 protoplanets, 10 test particles, tol 1e-10, collision_factor 100, hit_centrum 0.1 AU,
+    ejection 100 AU).  Its test particle testp0 starts exactly on protoplanet proto0 (same
+    position, velocity differing by 1e-11 AU/day), so their pair term is 0/0 and the test
+    particle's state is NaN after the first step, which still passes: test particles are
+    outside the error norm (error_check_for_tp = false).  The collision pass merges it into
+    proto0 with the NaN stage velocity recorded in the event, and the second step's error
+    vector holds NaN.  The oracle passes NaN components over as the reference's CPU loop does
+    (integrator.cu:237-253): estimate 0, an infinite next trial step (no clamp,
+    int_rungekutta8.cu:776-777), t = inf after the third step.  The library follows the first
+    step and the merge bit for bit, NaN bit patterns included, and reports the second step's
+    NaN estimate as RED_ERR_NAN; the drop-in driver writes the same event line (all columns)
+    and initial snapshot, then stops with that error."""
+    src = os.path.join(GOLDEN, "N1_massive_N3_test")
+    for f in ("N1_massive_N3_test.txt", "parameters.txt"):
+        (tmp_path / f).write_text(open(os.path.join(src, f)).read())
+    prm = files.read_parameters(str(tmp_path / "parameters.txt"))
+    assert prm["integrator"] == 4 and prm["tolerance"] == 1e-10 and prm["radii_enhance_factor"] == 100.0
+    print("case reference rkf78 input", flush=True)
+    bi = generate.to_internal(files.read_body_file(str(tmp_path / "N1_massive_N3_test.txt")))
+    assert np.array_equal(bi.r[1], bi.r[11]) and bi.types[11] == 6
+    kw = dict(integrator=4, adaptive=True, tolerance=1e-10, dt_day=0.001, hit_centrum=0.1, ejection=100.0,
+              radii_enhance_factor=100.0)
+    o = oracle_mod.Oracle(bi, **kw)
+    d = native_mod.Disk(bi, **kw)
+    assert o.check_for_ejection_hit_centrum() == d.check_for_ejection_hit_centrum() == False
+    assert o.step() == d.step()
+    assert o.last_err() == d.last_step.err and o.dt_try == d.dt_try
+    so, sg = o.state(), d.state()
+    assert np.isnan(so["v"][11]).all() and np.isfinite(so["v"][:11]).all()
+    for k in ("r", "v"):
+        assert np.array_equal(so[k].view(np.uint64), sg[k].view(np.uint64)), k
+    assert o.check_for_collision() and d.check_for_collision()
+    assert [(e["id1"], e["id2"]) for e in d.sp_events()] == [(2, 12)]
+    assert files.format_event_line(o.sp_events()[0]) == files.format_event_line(d.sp_events()[0])
+    so, sg = o.state(), d.state()
+    for k in ("r", "v", "p"):
+        assert np.array_equal(so[k].view(np.uint64), sg[k].view(np.uint64)), k
+    assert np.isnan(sg["v"][1]).all()
+    o.clear_event_counter(); d.clear_event_counter()
+    o.step()
+    assert o.last_err() == 0.0 and o.dt_try == float("inf")
+    with pytest.raises(native_mod.RedError) as ex:
+        d.step()
+    assert ex.value.code == 4
+    d.close()
+    # the drop-in driver on the same files
+    r = run_red(["-iDir", str(tmp_path), "-p", "parameters.txt", "-ic", "N1_massive_N3_test.txt"])
+    assert r.returncode != 0 and "non-finite error estimate" in r.stderr
+    _, snaps, events = _oracle_loop(oracle_mod, str(tmp_path / "N1_massive_N3_test.txt"), prm, 0.001)
+    lines = [ln.split() for ln in open(tmp_path / "event.txt") if ln.strip()]
+    assert len(events) == 1 and lines == [files.format_event_line(events[0]).split()]
+    t0, s0 = snaps[0]
+    rows = files.parse_result_ascii(str(tmp_path / "result.txt"))
+    assert len(rows) == bi.n
+    for i, row in enumerate(rows):
+        want = files.format_result_line(s0["id"][i], row[1], s0["type"][i], t0 / GAUSS, s0["p"][i], 0, 0.0,
+                                        s0["r"][i], s0["v"][i] * GAUSS).split()
+        assert row == want, (i, row, want)
