@@ -272,8 +272,11 @@ int red_copy_to_host(red_ctx* ctx, void* host, const void* dev, unsigned long lo
  * the attempt is captured once in a HIP graph (params->use_graph) and replayed, and each
  * disk runs the reference's step control (integrator.cu:211-263, int_rungekutta8.cu:704-797)
  * on its own time and trial step, so it takes exactly the steps it would take alone.
- * FP64, barycentric frame; pair tests, ejection and hit-centrum checks do not run inside
- * an ensemble, and parameters asking for them are refused with RED_ERR_UNSUPPORTED. */
+ * Every disk also runs the main loop's event checks in the reference's order
+ * (main.cu:201-255) with the single-disk code: the ejection / hit-centrum check before a step,
+ * pair tests inside the force launches, the collision or close-encounter pass after a step
+ * and the compaction after 8 events, so a member's events, merges and removals are those it
+ * would have alone.  FP64, barycentric frame. */
 typedef struct red_ensemble red_ensemble;
 typedef struct red_ensemble_info {
     int    supersteps;            /* attempts launched (one graph replay each)      */
@@ -284,9 +287,19 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
 int red_ensemble_destroy(red_ensemble* ens);
 const char* red_ensemble_last_error(const red_ensemble* ens);
 int red_ensemble_set_time(red_ensemble* ens, double t);
-/* Steps every disk while its t <= t_stop (main.cu:176), at most max_attempts attempts. */
+/* Steps every disk while its t <= t_stop (main.cu:176-255), at most max_attempts attempts. */
 int red_ensemble_advance(red_ensemble* ens, double t_stop, int max_attempts, red_ensemble_info* info);
 int red_ensemble_get_state(red_ensemble* ens, int disk, double* r, double* v);
+/* A disk's counts after removals and its bodies' parameters (arrays of the current count;
+ * any pointer may be NULL): the ensemble's pp_disk::copy_to_host (pp_disk.cu:1411-1439). */
+int red_ensemble_get_bodies(red_ensemble* ens, int disk, int counts[7], double* mass, double* radius,
+                            double* density, double* cd, int* id, int* type);
+/* out[8] of one disk, as red_counters: tried failed passed, collisions ejections hit-centrum
+ * close-encounters, events since the last rebuild. */
+int red_ensemble_disk_counters(red_ensemble* ens, int disk, int out[8]);
+/* The records of every event pass a disk has handled (each pass's sp list, in order) since
+ * the last call; copies min(n, cap), *n = records available, the copied ones are removed. */
+int red_ensemble_disk_events(red_ensemble* ens, int disk, red_event* buf, int cap, int* n);
 /* out[3] = tried failed passed of one disk; its time and trial step. */
 int red_ensemble_disk_info(red_ensemble* ens, int disk, int out[3], double* t, double* dt_try);
 /* Picks the force-launch shape (threads per block x source tile, each in {64,128,256}) by

@@ -21,10 +21,8 @@ using namespace red;
 namespace {
 
 const double GAUSS = tools::GAUSS;
-const double GAUSS2 = tools::GAUSS2;
 
 enum { BT_STAR = 0, BT_GIANT = 1, BT_ROCKY = 2, BT_PROTO = 3, BT_SUPERPL = 4, BT_PL = 5, BT_TP = 6, BT_PAD = 7 };
-enum { CNT_TOTAL = 0, CNT_LAST_CLEAR = 1, CNT_LAST_STEP = 2 };
 
 __global__ void k_set_ctl(Ctl* ctl, double t, double dt, int attempt, int reset_err, int clear_events, int clear_hc)
 {
@@ -42,14 +40,6 @@ hipError_t dmalloc(T** p, size_t count)
     *p = nullptr;
     if (count == 0) count = 1;
     return hipMalloc((void**)p, count * sizeof(T));
-}
-
-bool event_key_less(const red_event& a, const red_event& b)
-{
-    if (a.attempt != b.attempt) return a.attempt < b.attempt;
-    if (a.stage != b.stage) return a.stage < b.stage;
-    if (a.sink != b.sink) return a.sink < b.sink;
-    return a.source < b.source;
 }
 
 }  // namespace
@@ -106,8 +96,7 @@ struct red_ctx {
     int n_tried = 0, n_failed = 0, n_passed = 0;
     int attempts_cap = 1000;
     // event counters (pp_disk.h:129-133)
-    int n_hit_centrum[3] = {0}, n_ejection[3] = {0}, n_collision[3] = {0}, n_close_encounter[3] = {0};
-    int n_event[3] = {0};
+    events::Counters evc;
     std::vector<red_event> raw;           // pair events, canonical order
     bool raw_valid = true;
     std::vector<red_event> sp;
@@ -143,6 +132,12 @@ struct red_ctx {
     bool have_prev_feat = false;
 
     int nSI() const { return cnt[0] + cnt[1] + cnt[2] + cnt[3]; }
+    // the host mirror of the per-body parameters as the event pipeline sees it
+    events::BodyView view()
+    {
+        return events::BodyView{h_mass.data(), h_radius.data(), h_density.data(), h_cd.data(), h_id.data(),
+                                h_type.data(), nMassive()};
+    }
     int nNSI() const { return cnt[4] + cnt[5]; }
     int nNI() const { return cnt[6]; }
     int nMassive() const { return nSI() + nNSI(); }
@@ -416,53 +411,9 @@ int fetch_raw_events(red_ctx* c)
     c->raw.resize(n);
     if (n > 0) {
         CTX_HIP(c, stream_copy(c->stream, c->raw.data(), c->d_events, sizeof(red_event) * n, hipMemcpyDeviceToHost));
-        std::stable_sort(c->raw.begin(), c->raw.end(), event_key_less);
+        std::stable_sort(c->raw.begin(), c->raw.end(), events::event_key_less);
     }
     c->raw_valid = true;
-    return RED_OK;
-}
-
-void increment_event_counter(red_ctx* c, int* cnt)  // pp_disk.cu:1059-1067
-{
-    for (int i = 0; i < 3; i++) { cnt[i]++; c->n_event[i]++; }
-}
-
-void set_event_counter(red_ctx* c, int field, int value)  // pp_disk.cu:1069-1076
-{
-    c->n_hit_centrum[field] = c->n_ejection[field] = c->n_collision[field] = c->n_close_encounter[field] =
-        c->n_event[field] = value;
-}
-
-// pp_disk.cu:1151-1202 on the host mirror; the device copy is updated from 'ups'.
-int handle_collision_pair(red_ctx* c, red_event* e, bool kernel_negated, std::vector<BodyUpdate>& ups)
-{
-    const int surv = e->idx1, merg = e->idx2;
-    if (c->h_type[merg] == BT_SUPERPL)
-        return c->fail(RED_ERR_UNSUPPORTED, "Collision between a massive body and a super-planetesimal is not yet implemented.");
-    const double p1[4] = {c->h_mass[surv], c->h_radius[surv], c->h_density[surv], c->h_cd[surv]};
-    const double p2[4] = {c->h_mass[merg], c->h_radius[merg], c->h_density[merg], c->h_cd[merg]};
-    memcpy(e->p1, p1, sizeof(p1));
-    memcpy(e->p2, p2, sizeof(p2));
-    tools::mass_weighted_mean(p1[0], p2[0], e->r1, e->r2, e->rs);
-    tools::mass_weighted_mean(p1[0], p2[0], e->v1, e->v2, e->vs);
-    tools::calc_physical_properties(p1, p2, e->ps);
-    c->h_mass[surv] = e->ps[0]; c->h_radius[surv] = e->ps[1]; c->h_density[surv] = e->ps[2]; c->h_cd[surv] = e->ps[3];
-    // merger: inactive, massless, far away (pp_disk.cu:1177-1189)
-    if (kernel_negated) c->h_id[merg] = -std::abs(c->h_id[merg]);
-    else c->h_id[merg] *= -1;
-    c->h_mass[merg] = 0.0; c->h_density[merg] = 0.0; c->h_radius[merg] = 0.0;
-    BodyUpdate us;
-    us.idx = surv; us.id = c->h_id[surv];
-    us.r[0] = e->rs[0]; us.r[1] = e->rs[1]; us.r[2] = e->rs[2]; us.r[3] = c->h_mass[surv];
-    us.v[0] = e->vs[0]; us.v[1] = e->vs[1]; us.v[2] = e->vs[2]; us.v[3] = 0.0;
-    us.mass = c->h_mass[surv]; us.radius = c->h_radius[surv];
-    BodyUpdate um;
-    um.idx = merg; um.id = c->h_id[merg];
-    um.r[0] = 1.0e20; um.r[1] = 0.0; um.r[2] = 0.0; um.r[3] = 0.0;
-    um.v[0] = um.v[1] = um.v[2] = um.v[3] = 0.0;
-    um.mass = 0.0; um.radius = 0.0;
-    ups.push_back(us);
-    ups.push_back(um);
     return RED_OK;
 }
 
@@ -482,29 +433,6 @@ int apply_updates(red_ctx* c, const std::vector<BodyUpdate>& ups)
     refresh_pair_test(c);
     destroy_graphs(c);   // captured launches hold the pair-test parameters by value
     return RED_OK;
-}
-
-// Astro-centric elements of a close encounter from the recorded stage states
-// (pp_disk.cu:526-561 with tools::transform_to_ac on the stage state).
-void close_encounter_elements(red_ctx* c, red_event* e)
-{
-    const double ms = mass_of_star(c);
-    const double mu1 = GAUSS2 * (ms + c->h_mass[e->idx1]);
-    const double mu2 = GAUSS2 * (ms + c->h_mass[e->idx2]);
-    double r1[3], v1[3], r2[3], v2[3];
-    for (int k = 0; k < 3; k++) {
-        r1[k] = e->r1[k] - e->rs[k];
-        r2[k] = e->r2[k] - e->rs[k];
-        v1[k] = (e->v1[k] - e->vs[k]) * GAUSS;
-        v2[k] = (e->v2[k] - e->vs[k]) * GAUSS;
-    }
-    tools::calculate_orbital_element(mu1, r1, v1, e->oe1);
-    tools::calculate_orbital_element(mu2, r2, v2, e->oe2);
-    const double p1[4] = {c->h_mass[e->idx1], c->h_radius[e->idx1], c->h_density[e->idx1], c->h_cd[e->idx1]};
-    const double p2[4] = {c->h_mass[e->idx2], c->h_radius[e->idx2], c->h_density[e->idx2], c->h_cd[e->idx2]};
-    memcpy(e->p1, p1, sizeof(p1));
-    memcpy(e->p2, p2, sizeof(p2));
-    for (int k = 0; k < 3; k++) { e->rs[k] = 0.0; e->vs[k] = 0.0; }
 }
 
 int upload_all(red_ctx* c)
@@ -1182,34 +1110,15 @@ int red_check_ejection_hit_centrum(red_ctx* c, int* handled)
     if (ne <= 0) return RED_OK;
     std::vector<red_event> ev(ne);
     CTX_HIP(c, stream_copy(c->stream, ev.data(), c->d_events_hc, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
-    std::stable_sort(ev.begin(), ev.end(), [](const red_event& a, const red_event& b) { return a.sink < b.sink; });
-    for (auto& e : ev) c->h_id[e.idx2] = -c->h_id[e.idx2];   // mirror the kernel's negation
     std::vector<BodyUpdate> ups;
+    events::BodyView B = c->view();
+    std::string msg;
+    rc = events::ejection_pass(B, ev, c->evc, ups, msg);
     c->sp = ev;
-    // ejection records carry the parameters at detection (pp_disk.cu:58-66), before any
-    // hit-centrum merge of this pass changes the star
-    for (auto& e : c->sp) {
-        if (e.name != RED_EV_EJECTION) continue;
-        const int i = e.idx2;
-        const double p0[4] = {c->h_mass[0], c->h_radius[0], c->h_density[0], c->h_cd[0]};
-        const double pi[4] = {c->h_mass[i], c->h_radius[i], c->h_density[i], c->h_cd[i]};
-        memcpy(e.p1, p0, sizeof(p0)); memcpy(e.p2, pi, sizeof(pi));
-        memcpy(e.rs, e.r1, sizeof(e.rs)); memcpy(e.vs, e.v1, sizeof(e.vs)); memcpy(e.ps, e.p1, sizeof(e.ps));
-    }
-    for (auto& e : c->sp) {
-        if (e.name == RED_EV_EJECTION) {
-            increment_event_counter(c, c->n_ejection);
-        } else {
-            rc = handle_collision_pair(c, &e, true, ups);
-            if (rc) return rc;
-            increment_event_counter(c, c->n_hit_centrum);
-        }
-    }
+    if (rc) return c->fail(rc, msg);
     // ids negated by the kernel are already on the device; updates carry the merge results
     rc = apply_updates(c, ups);
     if (rc) return rc;
-    c->n_ejection[CNT_LAST_STEP] = 0;
-    c->n_hit_centrum[CNT_LAST_STEP] = 0;
     if (handled) *handled = 1;
     return RED_OK;
 }
@@ -1222,41 +1131,13 @@ int red_check_collision(red_ctx* c, int* handled)
     int rc = fetch_raw_events(c);
     if (rc) return rc;
     if (c->raw.empty()) return RED_OK;
-    // create_sp_events: dedupe by (id1, id2), keep the earliest t, first on ties
-    std::vector<red_event>& ev = c->raw;
-    const int ec = (int)ev.size();
-    std::vector<char> processed(ec, 0);
-    c->sp.clear();
-    for (int k = 0; k < ec; k++) {
-        if (processed[k]) continue;
-        processed[k] = 1;
-        red_event cur = ev[k];
-        for (int i = k + 1; i < ec; i++) {
-            if (cur.id1 == ev[i].id1 && cur.id2 == ev[i].id2) {
-                processed[i] = 1;
-                if (cur.t > ev[i].t) cur = ev[i];
-            }
-        }
-        c->sp.push_back(cur);
-    }
-    // close-encounter records take the parameters the bodies had at detection, i.e.
-    // before any merge of this pass
-    for (auto& e : c->sp)
-        if (e.name != RED_EV_COLLISION) close_encounter_elements(c, &e);
     std::vector<BodyUpdate> ups;
-    for (auto& e : c->sp) {
-        if (e.name == RED_EV_COLLISION) {
-            rc = handle_collision_pair(c, &e, false, ups);
-            if (rc) return rc;
-            increment_event_counter(c, c->n_collision);
-        } else {
-            increment_event_counter(c, c->n_close_encounter);
-        }
-    }
+    events::BodyView B = c->view();
+    std::string msg;
+    rc = events::collision_pass(B, c->raw, c->sp, c->evc, ups, msg);
+    if (rc) return c->fail(rc, msg);
     rc = apply_updates(c, ups);
     if (rc) return rc;
-    c->n_collision[CNT_LAST_STEP] = 0;
-    c->n_close_encounter[CNT_LAST_STEP] = 0;
     if (handled) *handled = 1;
     return RED_OK;
 }
@@ -1269,12 +1150,7 @@ int red_check_close_encounter(red_ctx* c, int* handled)
     int rc = fetch_raw_events(c);
     if (rc) return rc;
     if (c->raw.empty()) return RED_OK;
-    c->sp = c->raw;
-    for (auto& e : c->sp) {
-        close_encounter_elements(c, &e);
-        increment_event_counter(c, c->n_close_encounter);
-    }
-    c->n_close_encounter[CNT_LAST_STEP] = 0;
+    events::close_encounter_pass(c->view(), c->raw, c->sp, c->evc);
     if (handled) *handled = 1;
     return RED_OK;
 }
@@ -1284,7 +1160,7 @@ int red_check_rebuild(red_ctx* c, int threshold, int* rebuilt)
 {
     if (!c) return RED_ERR_ARG;
     if (rebuilt) *rebuilt = 0;
-    if (!(c->n_event[CNT_LAST_CLEAR] >= threshold)) return RED_OK;
+    if (!(c->evc.event[events::CNT_LAST_CLEAR] >= threshold)) return RED_OK;
     std::vector<d4> R, V;
     int rc = download_state(c, R, V);
     if (rc) return rc;
@@ -1323,7 +1199,7 @@ int red_check_rebuild(red_ctx* c, int threshold, int* rebuilt)
     destroy_graphs(c);
     c->gcfg.n_fp64_src = c->cnt[0] + c->cnt[1];
     refresh_pair_test(c);
-    set_event_counter(c, CNT_LAST_CLEAR, 0);
+    c->evc.set(events::CNT_LAST_CLEAR, 0);
     if (rebuilt) *rebuilt = 1;
     return RED_OK;
 }
@@ -1340,8 +1216,10 @@ int red_counters(red_ctx* c, int out[8])
 {
     if (!c || !out) return RED_ERR_ARG;
     out[0] = c->n_tried; out[1] = c->n_failed; out[2] = c->n_passed;
-    out[3] = c->n_collision[CNT_TOTAL]; out[4] = c->n_ejection[CNT_TOTAL]; out[5] = c->n_hit_centrum[CNT_TOTAL];
-    out[6] = c->n_close_encounter[CNT_TOTAL]; out[7] = c->n_event[CNT_LAST_CLEAR];
+    const events::Counters& E = c->evc;
+    out[3] = E.collision[events::CNT_TOTAL]; out[4] = E.ejection[events::CNT_TOTAL];
+    out[5] = E.hit_centrum[events::CNT_TOTAL]; out[6] = E.close_encounter[events::CNT_TOTAL];
+    out[7] = E.event[events::CNT_LAST_CLEAR];
     return RED_OK;
 }
 

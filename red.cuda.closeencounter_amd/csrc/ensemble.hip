@@ -8,15 +8,26 @@
 // host reads one error maximum per disk per attempt and runs the reference's controller
 // for every disk with the same arithmetic as a single disk, so an ensemble member
 // follows exactly the steps it would take alone (int_rungekutta8.cu:704-797 and
-// siblings).  Pair tests (collisions, ejections) are not applied inside an ensemble.
+// siblings).
+//
+// Events: the force launches run the pair tests of every disk (collision at
+// f (R_i + R_j), close encounter at f mutual Hill radii; pp_disk.cu:265-291) into one
+// record list per attempt, tagged with the disk and taken apart by the host; every attempt
+// also evaluates the ejection / hit-centrum predicate (pp_disk.cu:72-142) on its result.
+// Each member then runs the main loop's checks in the single-disk order (main.cu:201-255)
+// with the single-disk host code (host_tools.cpp, namespace events): the collision or
+// close-encounter pass after an accepted step, the compaction after 8 events (within the
+// disk's slot of the buffers), and at the start of the next step the ejection scan, done
+// exactly on the host whenever the predicate found a body or the state was changed since.
 //
 // Force tiles (FP64, the reference's operation order and ascending-j summation):
 //   kind 0: up to `block` sinks, one work-item each; sources staged through LDS in
 //           chunks of `jtile` bodies;
-//   kind 1: one sink (the star against its disk): the block computes `jtile` pair terms in
-//           parallel into LDS and three lanes add them in source order.
+//   kind 1: one sink (the star or a giant against its disk): the block computes `jtile`
+//           pair terms in parallel into LDS and three lanes add them in source order.
 // The tile shape (block, jtile) is picked by red_ensemble_autotune from measured launches.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -36,10 +47,33 @@ struct EnsTile {
     int disk, kind;
 };
 
+// Pair-test constants of one disk (the single-disk PairTest's per-disk part).
+struct EnsDiskPair {
+    double three_mstar;            // 3 M_star, for the mutual Hill radius
+    double rmax;                   // largest body radius (screen)
+    double mmax;                   // largest non-star mass (screen)
+    int n_star;                    // bodies [b0, b0 + n_star) are stars: no Hill test with them
+    int b0;                        // the disk's first body
+};
+
+// What a force launch needs for the pair tests of every disk.
+struct EnsPair {
+    double radii_f, hill_f;
+    int close_encounter, enabled;
+    const EnsDiskPair* dp;
+    const double* disk_t;          // step-start time per disk
+    const double* disk_dt;         // trial step per disk
+    double c_stage;
+    int stage;
+    red_event* buf;                // records of the attempt, all disks
+    int cap;
+    int* count;
+};
+
 constexpr int MAX_JTILE = 256;
 
-__device__ __forceinline__ void fp64_term(const d4& q, double xi, double yi, double zi, bool use, double& tx,
-                                          double& ty, double& tz)
+__device__ __forceinline__ double fp64_term(const d4& q, double xi, double yi, double zi, bool use, double& tx,
+                                            double& ty, double& tz)
 {
     const double dx = q.x - xi;
     const double dy = q.y - yi;
@@ -50,37 +84,115 @@ __device__ __forceinline__ void fp64_term(const d4& q, double xi, double yi, dou
     tx = sc * dx;
     ty = sc * dy;
     tz = sc * dz;
+    return d;
+}
+
+// Distance below which ens_pair_test can report an event for sink i (as the single-disk
+// pair_screen): the collision reach with the disk's largest radius and the Hill reach with
+// its largest non-star mass, using |r_j| <= |r_i| + d; 0.1 % margin for rounding.
+__device__ __forceinline__ double ens_screen(const EnsPair& P, const EnsDiskPair& D, double Ri, double mi,
+                                             const d4& qi)
+{
+    double cut = P.radii_f * (Ri + D.rmax);
+    if (P.hill_f > 0.0) {
+        const double fc = P.hill_f * cbrt((mi + D.mmax) / D.three_mstar);
+        const double hr = sqrt((qi.x * qi.x) + (qi.y * qi.y) + (qi.z * qi.z));
+        cut = fc < 1.0 ? fmax(cut, fc * hr / (1.0 - 0.5 * fc)) : 1e300;
+    }
+    return cut * 1.001;
+}
+
+// The single-disk pair test (kern_gravity.hip pair_test / append_pair_event) for sink i and
+// source j of one disk, with the disk's own indices in the record (the host sets the
+// attempt; pad carries the disk).
+__device__ void ens_pair_test(const EnsPair& P, int disk, int i, int j, double d, double Ri, const d4* r,
+                              const d4* v, const double* mass, const double* radius, const int* id)
+{
+    const EnsDiskPair D = P.dp[disk];
+    const int il = i - D.b0, jl = j - D.b0;
+    if (!(0 < il && il > jl)) return;
+    const bool collide = d < P.radii_f * (Ri + radius[j]);
+    bool hill = false;
+    if (!collide && P.hill_f > 0.0 && jl >= D.n_star) {
+        const d4 qi = r[i], qj = r[j];
+        double ri = sqrt((qi.x * qi.x) + (qi.y * qi.y) + (qi.z * qi.z));
+        double rj = sqrt((qj.x * qj.x) + (qj.y * qj.y) + (qj.z * qj.z));
+        double rh = cbrt_ieee((mass[i] + mass[j]) / D.three_mstar) * ((ri + rj) * 0.5);
+        hill = d < P.hill_f * rh;
+    }
+    if (!(collide || hill)) return;
+    int surv = i, merg = j;
+    int name;
+    if (collide && !P.close_encounter) {
+        if (mass[merg] > mass[surv]) { int t = surv; surv = merg; merg = t; }
+        name = RED_EV_COLLISION;
+    } else {
+        name = RED_EV_CLOSE_ENCOUNTER;
+    }
+    const int k = atomicAdd(P.count, 1);
+    if (k >= P.cap) return;   // counted but dropped: the host reports an overflow
+    red_event* e = P.buf + k;
+    e->name = name;
+    e->id1 = id[surv]; e->idx1 = surv - D.b0;
+    e->id2 = id[merg]; e->idx2 = merg - D.b0;
+    e->attempt = 0; e->stage = P.stage;
+    e->sink = il; e->source = jl; e->pad = disk;
+    e->t = P.disk_t[disk] + P.c_stage * P.disk_dt[disk];
+    e->d = d;
+    for (int c = 0; c < 4; c++) { e->p1[c] = 0.0; e->p2[c] = 0.0; e->ps[c] = 0.0; }
+    put3(e->r1, r[surv]); put3(e->v1, v[surv]);
+    put3(e->r2, r[merg]); put3(e->v2, v[merg]);
+    if (name == RED_EV_CLOSE_ENCOUNTER) {
+        put3(e->rs, r[D.b0]); put3(e->vs, v[D.b0]);
+    } else {
+        for (int c = 0; c < 3; c++) { e->rs[c] = 0.0; e->vs[c] = 0.0; }
+    }
+    for (int c = 0; c < 6; c++) { e->oe1[c] = 0.0; e->oe2[c] = 0.0; e->oes[c] = 0.0; }
 }
 
 // stage 0 runs only for disks starting a new step (bit 1), the other stages for every
-// disk in the attempt (bit 0)
+// disk in the attempt (bit 0).  Inactive sources add +0, inactive sinks get a = 0, as in the
+// single-disk sweeps.
+template <bool EV>
 __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ tiles, const d4* __restrict__ r,
-                                                   d4* __restrict__ a, const int* __restrict__ flags, int need_bit,
-                                                   int jtile)
+                                                   const d4* __restrict__ v, d4* __restrict__ a,
+                                                   const double* __restrict__ mass, const double* __restrict__ radius,
+                                                   const int* __restrict__ id, const int* __restrict__ flags,
+                                                   int need_bit, int jtile, EnsPair P)
 {
     __shared__ d4 src[MAX_JTILE];
+    __shared__ int sact[MAX_JTILE];
     __shared__ double term[2][3][MAX_JTILE];
     const EnsTile T = tiles[blockIdx.x];
     if (!(flags[T.disk] & need_bit)) return;
     if (T.kind == 0) {
         const int i = T.sink0 + threadIdx.x;
         const bool in = i < T.sink1;
-        const d4 p = r[in ? i : T.sink0];
+        const int ic = in ? i : T.sink0;
+        const d4 p = r[ic];
+        const bool live = in && id[ic] > 0;
+        const double Ri = EV ? radius[ic] : 0.0;
+        const double scr = EV && live ? ens_screen(P, P.dp[T.disk], Ri, mass[ic], p) : 0.0;
         double ax = 0.0, ay = 0.0, az = 0.0;
         for (int j0 = T.src0; j0 < T.src1; j0 += jtile) {
             const int nj = min(jtile, T.src1 - j0);
             __syncthreads();
-            for (int k = threadIdx.x; k < nj; k += blockDim.x) src[k] = r[j0 + k];
+            for (int k = threadIdx.x; k < nj; k += blockDim.x) {
+                src[k] = r[j0 + k];
+                sact[k] = id[j0 + k] >= 0;
+            }
             __syncthreads();
             for (int k = 0; k < nj; ++k) {
                 double tx, ty, tz;
-                fp64_term(src[k], p.x, p.y, p.z, (j0 + k) != i, tx, ty, tz);
+                const bool use = sact[k] && (j0 + k) != i;
+                const double d = fp64_term(src[k], p.x, p.y, p.z, use, tx, ty, tz);
                 ax += tx;
                 ay += ty;
                 az += tz;
+                if (EV && use && live && d < scr) ens_pair_test(P, T.disk, i, j0 + k, d, Ri, r, v, mass, radius, id);
             }
         }
-        if (in) a[i] = d4{ax, ay, az, 0.0};
+        if (in) a[i] = d4{live ? ax : 0.0, live ? ay : 0.0, live ? az : 0.0, 0.0};
     } else {
         // One sink against many sources (a disk's star): its three sums are serial chains in
         // ascending j, the kernel's critical path.  With more than one wave, waves 1.. form
@@ -89,6 +201,8 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
         // with one wave it forms, then adds, each chunk.
         const int i = T.sink0;
         const d4 p = r[i];
+        const bool live = id[i] > 0;
+        const double Ri = EV ? radius[i] : 0.0;
         const int c = threadIdx.x;
         const int nsrc = T.src1 - T.src0;
         const int nch = (nsrc + jtile - 1) / jtile;
@@ -103,10 +217,12 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
             double* t2 = term[ch & 1][2];
             for (int k = (int)threadIdx.x - p0; k < nj; k += (int)blockDim.x - p0) {
                 double tx, ty, tz;
-                fp64_term(r[j0 + k], p.x, p.y, p.z, (j0 + k) != i, tx, ty, tz);
+                const bool use = id[j0 + k] >= 0 && (j0 + k) != i;
+                const double d = fp64_term(r[j0 + k], p.x, p.y, p.z, use, tx, ty, tz);
                 t0[k] = tx;
                 t1[k] = ty;
                 t2[k] = tz;
+                if (EV && use && live && j0 + k < i) ens_pair_test(P, T.disk, i, j0 + k, d, Ri, r, v, mass, radius, id);
             }
         };
         double acc = 0.0;
@@ -151,13 +267,17 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
             const int tp = (int)threadIdx.x - p0;
             const int np = (int)blockDim.x - p0;
             d4 q[MS];
+            int qa[MS];
             auto fetch = [&](int ch) {
                 const int j0 = T.src0 + ch * jtile;
                 const int nj = min(jtile, T.src1 - j0);
 #pragma unroll
                 for (int m = 0; m < MS; ++m) {
                     const int k = tp + m * np;
-                    if (k < nj) q[m] = r[j0 + k];
+                    if (k < nj) {
+                        q[m] = r[j0 + k];
+                        qa[m] = id[j0 + k];
+                    }
                 }
             };
             auto form = [&](int ch) {
@@ -168,10 +288,13 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
                     const int k = tp + m * np;
                     if (k < nj) {
                         double tx, ty, tz;
-                        fp64_term(q[m], p.x, p.y, p.z, (j0 + k) != i, tx, ty, tz);
+                        const bool use = qa[m] >= 0 && (j0 + k) != i;
+                        const double d = fp64_term(q[m], p.x, p.y, p.z, use, tx, ty, tz);
                         term[ch & 1][0][k] = tx;
                         term[ch & 1][1][k] = ty;
                         term[ch & 1][2][k] = tz;
+                        if (EV && use && live && j0 + k < i)
+                            ens_pair_test(P, T.disk, i, j0 + k, d, Ri, r, v, mass, radius, id);
                     }
                 }
             };
@@ -200,9 +323,24 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
                 __syncthreads();
             }
         }
-        if (c < 3) reinterpret_cast<double*>(a + i)[c] = acc;
+        if (c < 3) reinterpret_cast<double*>(a + i)[c] = live ? acc : 0.0;
         if (c == 3) reinterpret_cast<double*>(a + i)[3] = 0.0;
     }
+}
+
+// The ejection / hit-centrum predicate on each attempt's y_(n+1): one flag per disk in the
+// attempt (bit 0) when an active non-star body meets it.
+__global__ __launch_bounds__(256) void k_ens_probe(const d4* __restrict__ outr, const int* __restrict__ id,
+                                                   const int* __restrict__ type, const int* __restrict__ disk_of,
+                                                   const int* __restrict__ flags, int n, double ej, double hc,
+                                                   int* __restrict__ probe)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int d = disk_of[i];
+    if (d < 0 || !(flags[d] & 1)) return;
+    if (!(id[i] > 0 && type[i] != 0)) return;
+    if (ejection_name(outr[i], ej, hc) != RED_EV_NONE) atomicOr(&probe[d], 1);
 }
 
 // y <- y_(n+1) for the bodies of disks whose last attempt was accepted (bit 2)
@@ -212,15 +350,19 @@ __global__ __launch_bounds__(256) void k_ens_accept(d4* __restrict__ yr, d4* __r
                                                     int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !(flags[disk_of[i]] & 4)) return;
+    if (i >= n) return;
+    const int d = disk_of[i];
+    if (d < 0 || !(flags[d] & 4)) return;
     yr[i] = outr[i];
     yv[i] = outv[i];
 }
 
-__global__ void k_ens_reset(unsigned long long* err, int* nan, int n)
+// Per-disk error maxima, NaN flags and ejection flags, and the attempt's record count.
+__global__ void k_ens_reset(unsigned long long* err, int* nan, int* probe, int n, int* ev_count)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d < n) { err[d] = 0ull; nan[d] = 0; }
+    if (d < n) { err[d] = 0ull; nan[d] = 0; probe[d] = 0; }
+    if (d == 0) *ev_count = 0;
 }
 
 template <typename T>
@@ -238,32 +380,61 @@ struct red_ensemble {
     hipStream_t stream = nullptr;
     std::string err;
     int nd = 0, n = 0;
-    std::vector<int> b0, b1, nsi, nm;
-    std::vector<int> disk_of_h;
+    std::vector<int> b0, b1;                 // disk d's bodies are [b0, b1); its slot ends at the next b0
+    std::vector<std::array<int, 7>> cnt;     // per disk: ns ngp nrp npp nspl npl ntp after removals
+    std::vector<int> disk_of_h;              // -1 for the slot tail left by a compaction
+    // host mirror of the per-body parameters (pp_disk::h_p, h_body_md), all disks
+    std::vector<double> h_mass, h_radius, h_density, h_cd;
+    std::vector<int> h_id, h_type;
     // device
     d4 *yr = nullptr, *yv = nullptr, *outr = nullptr, *outv = nullptr;
     std::vector<d4*> sr, sv, acc;
     int* disk_of = nullptr;
     unsigned char* err_on = nullptr;
-    double* d_dt = nullptr;
+    double *d_mass = nullptr, *d_radius = nullptr;
+    int *d_id = nullptr, *d_type = nullptr;
+    double *d_dt = nullptr, *d_t = nullptr;
     int* d_flags = nullptr;
     unsigned long long* d_err = nullptr;
     int* d_nan = nullptr;
+    int* d_probe = nullptr;
+    EnsDiskPair* d_pair = nullptr;
+    red_event* d_events = nullptr;
+    int ev_cap = 0;
+    int* d_ev_count = nullptr;
+    BodyUpdate* d_updates = nullptr;
+    int updates_cap = 0;
     EnsTile* d_tiles = nullptr;
     int n_tiles = 0;
     int block = 256, jtile = 256;
+    bool pair_tests = false, scan_on = false;
     // host per-disk controller state
     std::vector<double> t, dt_try, dt_did, last_err;
     std::vector<int> iter, tried, failed, passed, phase;   // phase 0: new step, 1: retry, 2: done
     std::vector<int> accepted_prev;
-    double t_stop = 0.0;
+    std::vector<char> need_scan;             // the next step starts with an exact ejection scan
+    std::vector<events::Counters> evc;
+    std::vector<std::vector<red_event>> pending;   // pair records since the last clear
+    std::vector<std::vector<red_event>> log;       // handled records (sp lists) not yet read
+    std::vector<red_event> h_events;
     double* h_dt = nullptr;                // pinned staging
+    double* h_t = nullptr;
     int* h_flags = nullptr;
     unsigned long long* h_err = nullptr;
     int* h_nan = nullptr;
+    int* h_probe = nullptr;
+    int* h_ev_count = nullptr;
     hipGraphExec_t graph = nullptr;
     bool use_graph = true;
 
+    int nSI(int d) const { return cnt[d][0] + cnt[d][1] + cnt[d][2] + cnt[d][3]; }
+    int nMassive(int d) const { return nSI(d) + cnt[d][4] + cnt[d][5]; }
+    events::BodyView view(int d)
+    {
+        const int o = b0[d];
+        return events::BodyView{h_mass.data() + o, h_radius.data() + o, h_density.data() + o, h_cd.data() + o,
+                                h_id.data() + o,   h_type.data() + o,   nMassive(d)};
+    }
     int fail(int code, const std::string& m) { err = m; return code; }
     int hipfail(hipError_t e, const char* where) { err = std::string(where) + ": " + hipGetErrorString(e); return RED_ERR_HIP; }
 };
@@ -281,7 +452,7 @@ std::vector<EnsTile> build_tiles(const red_ensemble* E, int block)
 {
     std::vector<EnsTile> tiles;
     for (int d = 0; d < E->nd; d++) {
-        const int b0 = E->b0[d], b1 = E->b1[d], si = E->nsi[d], m = E->nm[d];
+        const int b0 = E->b0[d], b1 = E->b1[d], si = E->nSI(d), m = E->nMassive(d);
         auto add = [&](int s0, int s1, int j0, int j1) {
             if (s1 <= s0) return;
             if (s1 - s0 <= 4 && j1 - j0 >= 64) {
@@ -300,6 +471,34 @@ std::vector<EnsTile> build_tiles(const red_ensemble* E, int block)
         return cx > cy;
     });
     return tiles;
+}
+
+// The pair-test constants of disk d from the host mirror (as refresh_pair_test of a single
+// disk): 3 M_star, the largest radius, the largest non-star mass, the number of stars.
+EnsDiskPair pair_consts(red_ensemble* E, int d)
+{
+    events::BodyView B = E->view(d);
+    const double ms = events::mass_of_star(B);
+    EnsDiskPair P;
+    P.three_mstar = 3.0 * (ms > 0 ? ms : 1.0);
+    double rall = 0.0, mns = 0.0;
+    const int nb = E->b1[d] - E->b0[d];
+    for (int i = 0; i < nb; i++) {
+        rall = std::max(rall, B.radius[i]);
+        if (i >= E->cnt[d][0]) mns = std::max(mns, B.mass[i]);
+    }
+    P.rmax = rall;
+    P.mmax = mns;
+    P.n_star = E->cnt[d][0];
+    P.b0 = E->b0[d];
+    return P;
+}
+
+int upload_pair(red_ensemble* E, int d)
+{
+    const EnsDiskPair P = pair_consts(E, d);
+    ENS_HIP(E, stream_copy(E->stream, E->d_pair + d, &P, sizeof(P), hipMemcpyHostToDevice));
+    return RED_OK;
 }
 
 const d4* ens_stage_v(red_ensemble* E, int k)
@@ -325,11 +524,29 @@ StageArgs ens_args(red_ensemble* E, const Row& row, d4* outr, d4* outv)
     return A;
 }
 
-hipError_t ens_force(red_ensemble* E, int stage, const d4* r)
+hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v)
 {
     if (E->n_tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ens_force, dim3(E->n_tiles), dim3(E->block), 0, E->stream, E->d_tiles, r, E->acc[stage],
-                       E->d_flags, stage == 0 ? 2 : 1, E->jtile);
+    EnsPair P;
+    P.radii_f = E->prm.radii_enhance_factor;
+    P.hill_f = E->prm.encounter_hill_factor;
+    P.close_encounter = E->prm.close_encounter;
+    P.enabled = E->pair_tests ? 1 : 0;
+    P.dp = E->d_pair;
+    P.disk_t = E->d_t;
+    P.disk_dt = E->d_dt;
+    P.c_stage = E->S.c[stage];
+    P.stage = stage;
+    P.buf = E->d_events;
+    P.cap = E->ev_cap;
+    P.count = E->d_ev_count;
+    const int need = stage == 0 ? 2 : 1;
+    if (E->pair_tests)
+        hipLaunchKernelGGL(k_ens_force<true>, dim3(E->n_tiles), dim3(E->block), 0, E->stream, E->d_tiles, r, v,
+                           E->acc[stage], E->d_mass, E->d_radius, E->d_id, E->d_flags, need, E->jtile, P);
+    else
+        hipLaunchKernelGGL(k_ens_force<false>, dim3(E->n_tiles), dim3(E->block), 0, E->stream, E->d_tiles, r, v,
+                           E->acc[stage], E->d_mass, E->d_radius, E->d_id, E->d_flags, need, E->jtile, P);
     return hipGetLastError();
 }
 
@@ -340,24 +557,30 @@ hipError_t ens_attempt(red_ensemble* E)
     const dim3 gb((E->n + 255) / 256), bb(256);
     hipLaunchKernelGGL(k_ens_accept, gb, bb, 0, E->stream, E->yr, E->yv, E->outr, E->outv, E->disk_of, E->d_flags, E->n);
     RED_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_ens_reset, dim3((E->nd + 255) / 256), dim3(256), 0, E->stream, E->d_err, E->d_nan, E->nd);
+    hipLaunchKernelGGL(k_ens_reset, dim3((E->nd + 255) / 256), dim3(256), 0, E->stream, E->d_err, E->d_nan, E->d_probe,
+                       E->nd, E->d_ev_count);
     RED_HIP_TRY(hipGetLastError());
-    RED_HIP_TRY(ens_force(E, 0, E->yr));
+    RED_HIP_TRY(ens_force(E, 0, E->yr, E->yv));
     for (int r = 1; r < S.n_main; r++) {
         RED_HIP_TRY(launch_stage(S.rows[r].form, ens_args(E, S.rows[r], E->sr[r], E->sv[r]), E->stream));
-        RED_HIP_TRY(ens_force(E, r, E->sr[r]));
+        RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r]));
     }
     RED_HIP_TRY(launch_stage(S.np1.form, ens_args(E, S.np1, E->outr, E->outv), E->stream));
     if (E->prm.adaptive && S.err_form >= 0) {
         for (int r = S.n_main; r < S.n_stage; r++) {
             if (S.extra_on_yout) {
-                RED_HIP_TRY(ens_force(E, r, E->outr));
+                RED_HIP_TRY(ens_force(E, r, E->outr, E->outv));
             } else {
                 RED_HIP_TRY(launch_stage(S.rows[r].form, ens_args(E, S.rows[r], E->sr[r], E->sv[r]), E->stream));
-                RED_HIP_TRY(ens_force(E, r, E->sr[r]));
+                RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r]));
             }
         }
         RED_HIP_TRY(launch_err(S.err_form, ens_args(E, S.err, nullptr, nullptr), E->stream));
+    }
+    if (E->scan_on) {
+        hipLaunchKernelGGL(k_ens_probe, gb, bb, 0, E->stream, E->outr, E->d_id, E->d_type, E->disk_of, E->d_flags,
+                           E->n, E->prm.ejection, E->prm.hit_centrum, E->d_probe);
+        RED_HIP_TRY(hipGetLastError());
     }
     return hipSuccess;
 }
@@ -391,6 +614,7 @@ int upload_tiles(red_ensemble* E)
 {
     SetupLock lk;
     std::vector<EnsTile> tiles = build_tiles(E, E->block);
+    (void)hipStreamSynchronize(E->stream);
     if (E->d_tiles) (void)hipFree(E->d_tiles);
     E->d_tiles = nullptr;
     E->n_tiles = (int)tiles.size();
@@ -398,6 +622,162 @@ int upload_tiles(red_ensemble* E)
     if (!tiles.empty())
         ENS_HIP(E, stream_copy(E->stream, E->d_tiles, tiles.data(), sizeof(EnsTile) * tiles.size(), hipMemcpyHostToDevice));
     destroy_graph(E);
+    return RED_OK;
+}
+
+// The buffers holding disk d's current state: y_(n+1) of an accepted attempt until the next
+// attempt's first launch moves it into y.
+d4* cur_r(red_ensemble* E, int d) { return E->accepted_prev[d] ? E->outr : E->yr; }
+d4* cur_v(red_ensemble* E, int d) { return E->accepted_prev[d] ? E->outv : E->yv; }
+
+// Merge results (disk-local indices) into the current state and the device parameters.
+int apply_updates(red_ensemble* E, int d, std::vector<BodyUpdate> ups)
+{
+    if (ups.empty()) return RED_OK;
+    for (auto& u : ups) u.idx += E->b0[d];
+    if ((int)ups.size() > E->updates_cap) {
+        SetupLock lk;
+        (void)hipStreamSynchronize(E->stream);
+        if (E->d_updates) (void)hipFree(E->d_updates);
+        E->d_updates = nullptr;
+        E->updates_cap = (int)ups.size() * 2;
+        ENS_HIP(E, dmalloc(&E->d_updates, E->updates_cap));
+    }
+    ENS_HIP(E, hipMemcpyAsync(E->d_updates, ups.data(), sizeof(BodyUpdate) * ups.size(), hipMemcpyHostToDevice,
+                              E->stream));
+    ENS_HIP(E, launch_apply_updates(E->d_updates, (int)ups.size(), cur_r(E, d), cur_v(E, d), E->d_mass, E->d_radius,
+                                    E->d_id, E->stream));
+    ENS_HIP(E, hipStreamSynchronize(E->stream));
+    return upload_pair(E, d);
+}
+
+// pp_disk.cu:683-715 for disk d: the scan on its current state, its records and ids, the
+// hit-centrum merges (main.cu:201-208); the pair records since the last clear are dropped.
+int ejection_check(red_ensemble* E, int d)
+{
+    const int o = E->b0[d], nb = E->b1[d] - o;
+    if (nb <= 0) return RED_OK;
+    std::vector<d4> R(nb), V(nb);
+    ENS_HIP(E, stream_copy(E->stream, R.data(), cur_r(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+    ENS_HIP(E, stream_copy(E->stream, V.data(), cur_v(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+    std::vector<int> ids(E->h_id.begin() + o, E->h_id.begin() + o + nb);   // the device's ids
+    std::vector<red_event> ev;
+    events::ejection_scan(nb, &R[0].x, &V[0].x, ids.data(), E->h_type.data() + o, E->t[d], E->prm.ejection,
+                          E->prm.hit_centrum, ev);
+    if (ev.empty()) return RED_OK;
+    for (auto& e : ev) e.pad = d;
+    events::BodyView B = E->view(d);
+    std::vector<BodyUpdate> ups;
+    std::string msg;
+    const int rc = events::ejection_pass(B, ev, E->evc[d], ups, msg);
+    if (rc) return E->fail(rc, "disk " + std::to_string(d) + ": " + msg);
+    // the negated ids (the host mirror now), then the merge results
+    ENS_HIP(E, stream_copy(E->stream, E->d_id + o, E->h_id.data() + o, sizeof(int) * nb, hipMemcpyHostToDevice));
+    const int rc2 = apply_updates(E, d, ups);
+    if (rc2) return rc2;
+    E->log[d].insert(E->log[d].end(), ev.begin(), ev.end());
+    E->pending[d].clear();   // clear_event_counter (main.cu:205)
+    return upload_pair(E, d);
+}
+
+// pp_disk.cu:893-907, 1324-1409 and number_of_bodies.cpp:27-104 for disk d: its active
+// bodies move to the front of its slot in order, the rest of the slot is left out of every
+// launch.
+int compact(red_ensemble* E, int d)
+{
+    const int o = E->b0[d], nb = E->b1[d] - o;
+    std::vector<d4> R(std::max(nb, 1)), V(std::max(nb, 1));
+    if (nb > 0) {
+        ENS_HIP(E, stream_copy(E->stream, R.data(), cur_r(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+        ENS_HIP(E, stream_copy(E->stream, V.data(), cur_v(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+    }
+    int removed[7] = {0};
+    std::vector<int> keep;
+    for (int i = 0; i < nb; i++) {
+        const int t = E->h_type[o + i];
+        if (0 < E->h_id[o + i] && 7 > t) { keep.push_back(i); continue; }
+        if (t < 0 || t > 7) return E->fail(RED_ERR_ARG, "Undefined body type!");
+        if (t == 7) continue;
+        removed[t]++;
+    }
+    for (int k = 0; k < 7; k++) E->cnt[d][k] -= removed[k];
+    const int m = (int)keep.size();
+    int nn = 0;
+    for (int k = 0; k < 7; k++) nn += E->cnt[d][k];
+    if (nn != m) return E->fail(RED_ERR_ARG, "Error: number of copied bodies does not equal to the number of active bodies.");
+    std::vector<d4> R2(std::max(m, 1)), V2(std::max(m, 1));
+    for (int k = 0; k < m; k++) {
+        const int i = keep[k];
+        R2[k] = R[i]; V2[k] = V[i];
+        E->h_mass[o + k] = E->h_mass[o + i]; E->h_radius[o + k] = E->h_radius[o + i];
+        E->h_density[o + k] = E->h_density[o + i]; E->h_cd[o + k] = E->h_cd[o + i];
+        E->h_id[o + k] = E->h_id[o + i]; E->h_type[o + k] = E->h_type[o + i];
+    }
+    // the slot's tail: massless, inactive padding that no launch visits
+    for (int k = m; k < nb; k++) {
+        E->h_mass[o + k] = 0.0; E->h_radius[o + k] = 0.0; E->h_density[o + k] = 0.0; E->h_cd[o + k] = 0.0;
+        E->h_id[o + k] = 0; E->h_type[o + k] = 7;
+        E->disk_of_h[o + k] = -1;
+        R2.resize(std::max(nb, 1)); V2.resize(std::max(nb, 1));
+        R2[k] = d4{0.0, 0.0, 0.0, 0.0}; V2[k] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    E->b1[d] = o + m;
+    const int nm = E->nMassive(d);
+    std::vector<unsigned char> eon(std::max(nb, 1));
+    for (int k = 0; k < nb; k++) eon[k] = (unsigned char)(k < m && (E->prm.error_check_for_tp || k < nm) ? 1 : 0);
+    if (nb > 0) {
+        for (d4* q : {E->yr, E->outr}) ENS_HIP(E, stream_copy(E->stream, q + o, R2.data(), sizeof(d4) * nb, hipMemcpyHostToDevice));
+        for (d4* q : {E->yv, E->outv}) ENS_HIP(E, stream_copy(E->stream, q + o, V2.data(), sizeof(d4) * nb, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->d_mass + o, E->h_mass.data() + o, sizeof(double) * nb, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->d_radius + o, E->h_radius.data() + o, sizeof(double) * nb, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->d_id + o, E->h_id.data() + o, sizeof(int) * nb, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->d_type + o, E->h_type.data() + o, sizeof(int) * nb, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->disk_of + o, E->disk_of_h.data() + o, sizeof(int) * nb, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->err_on + o, eon.data(), nb, hipMemcpyHostToDevice));
+    }
+    E->evc[d].set(events::CNT_LAST_CLEAR, 0);
+    int rc = upload_pair(E, d);
+    if (rc) return rc;
+    return upload_tiles(E);
+}
+
+// The checks after an accepted step of disk d (main.cu:225-255), on its new state.
+// *changed: the state or the bodies changed, so the ejection predicate of the attempt no
+// longer describes the state the next step starts from.
+int after_step(red_ensemble* E, int d, bool* changed)
+{
+    *changed = false;
+    const red_params& P = E->prm;
+    std::vector<red_event>& raw = E->pending[d];
+    if (P.close_encounter) {
+        if (E->pair_tests && !raw.empty()) {
+            std::stable_sort(raw.begin(), raw.end(), events::event_key_less);
+            std::vector<red_event> sp;
+            events::close_encounter_pass(E->view(d), raw, sp, E->evc[d]);
+            E->log[d].insert(E->log[d].end(), sp.begin(), sp.end());
+            raw.clear();
+        }
+        return RED_OK;
+    }
+    if (E->pair_tests && !raw.empty()) {
+        std::stable_sort(raw.begin(), raw.end(), events::event_key_less);
+        std::vector<red_event> sp;
+        std::vector<BodyUpdate> ups;
+        std::string msg;
+        events::BodyView B = E->view(d);
+        int rc = events::collision_pass(B, raw, sp, E->evc[d], ups, msg);
+        if (rc) return E->fail(rc, "disk " + std::to_string(d) + ": " + msg);
+        rc = apply_updates(E, d, ups);
+        if (rc) return rc;
+        if (!ups.empty()) *changed = true;
+        E->log[d].insert(E->log[d].end(), sp.begin(), sp.end());
+        raw.clear();
+    }
+    if (E->evc[d].event[events::CNT_LAST_CLEAR] >= 8) {
+        const int rc = compact(E, d);
+        if (rc) return rc;
+        *changed = true;
+    }
     return RED_OK;
 }
 
@@ -412,12 +792,17 @@ int red_ensemble_destroy(red_ensemble* E)
     destroy_graph(E);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     for (void* p : {(void*)E->yr, (void*)E->yv, (void*)E->outr, (void*)E->outv, (void*)E->disk_of, (void*)E->err_on,
-                    (void*)E->d_dt, (void*)E->d_flags, (void*)E->d_err, (void*)E->d_nan, (void*)E->d_tiles})
+                    (void*)E->d_dt, (void*)E->d_t, (void*)E->d_flags, (void*)E->d_err, (void*)E->d_nan,
+                    (void*)E->d_probe, (void*)E->d_tiles, (void*)E->d_mass, (void*)E->d_radius, (void*)E->d_id,
+                    (void*)E->d_type, (void*)E->d_pair, (void*)E->d_events, (void*)E->d_ev_count,
+                    (void*)E->d_updates})
         (void)hipFree(p);
     for (auto p : E->sr) (void)hipFree(p);
     for (auto p : E->sv) (void)hipFree(p);
     for (auto p : E->acc) (void)hipFree(p);
-    for (void* p : {(void*)E->h_dt, (void*)E->h_flags, (void*)E->h_err, (void*)E->h_nan}) if (p) (void)hipHostFree(p);
+    for (void* p : {(void*)E->h_dt, (void*)E->h_t, (void*)E->h_flags, (void*)E->h_err, (void*)E->h_nan,
+                    (void*)E->h_probe, (void*)E->h_ev_count})
+        if (p) (void)hipHostFree(p);
     if (E->stream) (void)hipStreamDestroy(E->stream);
     delete E;
     return RED_OK;
@@ -433,35 +818,48 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     if (p->integrator < RED_EULER || p->integrator > RED_RKF78 || p->precision != RED_PREC_FP64 ||
         p->frame_center != RED_FRAME_BARY)
         return RED_ERR_UNSUPPORTED;
-    // no pair tests, ejection or hit-centrum checks run inside an ensemble: asking for them
-    // is refused rather than ignored
-    if (p->radii_enhance_factor > 0.0 || p->encounter_hill_factor > 0.0 || p->close_encounter ||
-        p->ejection > 0.0 || p->hit_centrum > 0.0)
-        return RED_ERR_UNSUPPORTED;
+    if (!p->inner_steps && (p->radii_enhance_factor > 0.0 || p->encounter_hill_factor > 0.0)) return RED_ERR_UNSUPPORTED;
     red_ensemble* E = new red_ensemble();
     E->prm = *p;
     E->S = make_scheme(p->integrator, p->adaptive != 0);
     E->nd = n_disks;
     E->use_graph = p->use_graph != 0;
+    E->pair_tests = p->inner_steps && (p->radii_enhance_factor > 0.0 || p->encounter_hill_factor > 0.0);
+    E->scan_on = p->ejection > 0.0 || p->hit_centrum > 0.0;
     std::vector<d4> r, v;
     std::vector<unsigned char> eon;
     for (int d = 0; d < n_disks; d++) {
         const red_bodies& b = disks[d];
         int nb = 0;
-        for (int k = 0; k < 7; k++) nb += b.counts[k];
-        const int si = b.counts[0] + b.counts[1] + b.counts[2] + b.counts[3];
-        const int m = si + b.counts[4] + b.counts[5];
+        std::array<int, 7> c{};
+        for (int k = 0; k < 7; k++) {
+            if (b.counts[k] < 0) { delete E; return RED_ERR_ARG; }
+            c[k] = b.counts[k];
+            nb += b.counts[k];
+        }
+        if (nb > 0 && (!b.id || !b.type || !b.mass || !b.radius || !b.density || !b.cd || !b.r || !b.v)) {
+            delete E;
+            return RED_ERR_ARG;
+        }
+        const int si = c[0] + c[1] + c[2] + c[3];
+        const int m = si + c[4] + c[5];
         E->b0.push_back(E->n);
         E->b1.push_back(E->n + nb);
-        E->nsi.push_back(si);
-        E->nm.push_back(m);
+        E->cnt.push_back(c);
         for (int i = 0; i < nb; i++) {
             r.push_back(d4{b.r[3 * i], b.r[3 * i + 1], b.r[3 * i + 2], b.mass[i]});
             v.push_back(d4{b.v[3 * i], b.v[3 * i + 1], b.v[3 * i + 2], 0.0});
             E->disk_of_h.push_back(d);
             eon.push_back((unsigned char)(p->error_check_for_tp ? 1 : (i < m ? 1 : 0)));
+            E->h_mass.push_back(b.mass[i]); E->h_radius.push_back(b.radius[i]);
+            E->h_density.push_back(b.density[i]); E->h_cd.push_back(b.cd[i]);
+            E->h_id.push_back(b.id[i]); E->h_type.push_back(b.type[i]);
         }
         E->n += nb;
+        if (nb > 0 && (p->encounter_hill_factor > 0.0 || p->close_encounter) && events::mass_of_star(E->view(d)) < 0.0) {
+            delete E;
+            return RED_ERR_NO_STAR;
+        }
     }
     *out = E;
     ENS_HIP(E, hipSetDevice(p->device));
@@ -476,14 +874,28 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     }
     ENS_HIP(E, dmalloc(&E->disk_of, n));
     ENS_HIP(E, dmalloc(&E->err_on, n));
+    ENS_HIP(E, dmalloc(&E->d_mass, n));
+    ENS_HIP(E, dmalloc(&E->d_radius, n));
+    ENS_HIP(E, dmalloc(&E->d_id, n));
+    ENS_HIP(E, dmalloc(&E->d_type, n));
     ENS_HIP(E, dmalloc(&E->d_dt, E->nd));
+    ENS_HIP(E, dmalloc(&E->d_t, E->nd));
     ENS_HIP(E, dmalloc(&E->d_flags, E->nd));
     ENS_HIP(E, dmalloc(&E->d_err, E->nd));
     ENS_HIP(E, dmalloc(&E->d_nan, E->nd));
+    ENS_HIP(E, dmalloc(&E->d_probe, E->nd));
+    ENS_HIP(E, dmalloc(&E->d_pair, E->nd));
+    E->ev_cap = p->event_capacity > 0 ? p->event_capacity : std::max(4096, std::min(E->n, 65536));
+    ENS_HIP(E, dmalloc(&E->d_events, E->ev_cap));
+    ENS_HIP(E, dmalloc(&E->d_ev_count, 1));
+    ENS_HIP(E, hipMemsetAsync(E->d_ev_count, 0, sizeof(int), E->stream));
     ENS_HIP(E, hipHostMalloc((void**)&E->h_dt, sizeof(double) * E->nd, hipHostMallocDefault));
+    ENS_HIP(E, hipHostMalloc((void**)&E->h_t, sizeof(double) * E->nd, hipHostMallocDefault));
     ENS_HIP(E, hipHostMalloc((void**)&E->h_flags, sizeof(int) * E->nd, hipHostMallocDefault));
     ENS_HIP(E, hipHostMalloc((void**)&E->h_err, sizeof(unsigned long long) * E->nd, hipHostMallocDefault));
     ENS_HIP(E, hipHostMalloc((void**)&E->h_nan, sizeof(int) * E->nd, hipHostMallocDefault));
+    ENS_HIP(E, hipHostMalloc((void**)&E->h_probe, sizeof(int) * E->nd, hipHostMallocDefault));
+    ENS_HIP(E, hipHostMalloc((void**)&E->h_ev_count, sizeof(int), hipHostMallocDefault));
     if (E->n > 0) {
         ENS_HIP(E, stream_copy(E->stream, E->yr, r.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->yv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
@@ -491,12 +903,24 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
         ENS_HIP(E, stream_copy(E->stream, E->outv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->disk_of, E->disk_of_h.data(), sizeof(int) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->err_on, eon.data(), E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->d_mass, E->h_mass.data(), sizeof(double) * E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->d_radius, E->h_radius.data(), sizeof(double) * E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->d_id, E->h_id.data(), sizeof(int) * E->n, hipMemcpyHostToDevice));
+        ENS_HIP(E, stream_copy(E->stream, E->d_type, E->h_type.data(), sizeof(int) * E->n, hipMemcpyHostToDevice));
     }
     const double dt0 = p->dt_day * tools::GAUSS;   // integrator.cu:29
     E->t.assign(E->nd, 0.0); E->dt_try.assign(E->nd, dt0); E->dt_did.assign(E->nd, 0.0);
     E->last_err.assign(E->nd, 0.0);
     E->iter.assign(E->nd, 0); E->tried.assign(E->nd, 0); E->failed.assign(E->nd, 0); E->passed.assign(E->nd, 0);
     E->phase.assign(E->nd, 0); E->accepted_prev.assign(E->nd, 0);
+    E->need_scan.assign(E->nd, 1);
+    E->evc.assign(E->nd, events::Counters());
+    E->pending.assign(E->nd, {});
+    E->log.assign(E->nd, {});
+    for (int d = 0; d < E->nd; d++) {
+        int rc = upload_pair(E, d);
+        if (rc) return rc;
+    }
     if (p->block >= 64) E->block = std::min(256, p->block / 64 * 64);
     int rc = upload_tiles(E);
     if (rc) return rc;
@@ -511,8 +935,9 @@ int red_ensemble_set_time(red_ensemble* E, double t)
     return RED_OK;
 }
 
-// Integrates every disk with the reference loop condition `while (t <= t_stop) step()`
-// (main.cu:176) until all disks are done or max_attempts attempts have been launched.
+// Integrates every disk with the reference loop `while (t <= t_stop) { ejection check; step;
+// collision check; rebuild check }` (main.cu:176-255) until all disks are done or
+// max_attempts attempts have been launched.
 int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_ensemble_info* info)
 {
     if (!E) return RED_ERR_ARG;
@@ -528,26 +953,60 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         active = 0;
         for (int d = 0; d < E->nd; d++) {
             int f = 0;
-            if (E->accepted_prev[d]) f |= 4;
             if (E->phase[d] == 0 && !(E->t[d] <= t_stop)) E->phase[d] = 2;
-            if (E->phase[d] == 0) { f |= 1 | 2; E->iter[d] = 0; }
-            else if (E->phase[d] == 1) f |= 1;
+            if (E->phase[d] == 0) {
+                // the loop iteration's ejection / hit-centrum check, before the step
+                if (E->scan_on && E->need_scan[d]) {
+                    const int rc = ejection_check(E, d);
+                    if (rc) return rc;
+                    E->need_scan[d] = 0;
+                }
+                f |= 1 | 2;
+                E->iter[d] = 0;
+            } else if (E->phase[d] == 1) {
+                f |= 1;
+            }
+            if (E->accepted_prev[d]) f |= 4;
             if (f & 1) { active++; E->dt_did[d] = E->dt_try[d]; }
             E->h_flags[d] = f;
             E->h_dt[d] = E->dt_try[d];
+            E->h_t[d] = E->t[d];
         }
         if (active == 0 || attempts >= max_attempts) break;
         ENS_HIP(E, hipMemcpyAsync(E->d_flags, E->h_flags, sizeof(int) * E->nd, hipMemcpyHostToDevice, E->stream));
         ENS_HIP(E, hipMemcpyAsync(E->d_dt, E->h_dt, sizeof(double) * E->nd, hipMemcpyHostToDevice, E->stream));
+        ENS_HIP(E, hipMemcpyAsync(E->d_t, E->h_t, sizeof(double) * E->nd, hipMemcpyHostToDevice, E->stream));
         ENS_HIP(E, run_attempt(E));
         ENS_HIP(E, hipMemcpyAsync(E->h_err, E->d_err, sizeof(unsigned long long) * E->nd, hipMemcpyDeviceToHost, E->stream));
         ENS_HIP(E, hipMemcpyAsync(E->h_nan, E->d_nan, sizeof(int) * E->nd, hipMemcpyDeviceToHost, E->stream));
+        if (E->scan_on)
+            ENS_HIP(E, hipMemcpyAsync(E->h_probe, E->d_probe, sizeof(int) * E->nd, hipMemcpyDeviceToHost, E->stream));
+        if (E->pair_tests)
+            ENS_HIP(E, hipMemcpyAsync(E->h_ev_count, E->d_ev_count, sizeof(int), hipMemcpyDeviceToHost, E->stream));
         ENS_HIP(E, hipStreamSynchronize(E->stream));
         attempts++;
+        // the attempt's pair records, to their disks
+        const int ne = E->pair_tests ? *E->h_ev_count : 0;
+        if (ne > E->ev_cap)
+            return E->fail(RED_ERR_EVENT_OVERFLOW, "pair events exceed the event capacity (" + std::to_string(ne) +
+                                                       " > " + std::to_string(E->ev_cap) + ")");
+        E->h_events.resize(ne);
+        if (ne > 0)
+            ENS_HIP(E, stream_copy(E->stream, E->h_events.data(), E->d_events, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
+        for (int d = 0; d < E->nd; d++)
+            if ((E->h_flags[d] & 1) && S.clear_events_each_attempt) E->pending[d].clear();   // int_rungekutta8.cu:730-733
+        for (const red_event& e0 : E->h_events) {
+            red_event e = e0;
+            const int d = e.pad;
+            if (S.clear_events_each_attempt && e.stage == 0) continue;   // f0's records, cleared with the attempt
+            e.attempt = E->iter[d];
+            e.pad = 0;
+            E->pending[d].push_back(e);
+        }
         for (int d = 0; d < E->nd; d++) {
             E->accepted_prev[d] = 0;
             if (!(E->h_flags[d] & 1)) continue;
-            bool accept = true;
+            bool accept = true, counted = true;
             if (adaptive) {
                 if (E->h_nan[d]) return E->fail(RED_ERR_NAN, "non-finite error estimate in disk " + std::to_string(d));
                 double m;
@@ -555,8 +1014,15 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
                 const double max_err = tools::controller_update(&E->dt_try[d], S.lambda, m, E->prm.tolerance, S.exponent);
                 E->last_err[d] = max_err;
                 accept = !(max_err > E->prm.tolerance);
+                // RK4 and Cash-Karp accept an attempt that recorded pair events and leave the
+                // loop before counting it (int_rungekutta4.cu:255-264)
+                if (S.accept_on_event && !E->pending[d].empty()) {
+                    if (E->dt_try[d] < E->dt_did[d]) E->dt_try[d] = E->dt_did[d];
+                    accept = true;
+                    counted = false;
+                }
             }
-            E->iter[d]++;
+            if (counted) E->iter[d]++;
             if (accept) {
                 E->tried[d] += E->iter[d];
                 E->failed[d] += E->iter[d] - 1;
@@ -564,6 +1030,10 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
                 E->t[d] += E->dt_did[d];
                 E->phase[d] = 0;
                 E->accepted_prev[d] = 1;
+                bool changed = false;
+                const int rc = after_step(E, d, &changed);
+                if (rc) return rc;
+                if (E->scan_on) E->need_scan[d] = E->h_probe[d] || changed;
             } else {
                 E->phase[d] = 1;
                 if (E->iter[d] >= 1000) return E->fail(RED_ERR_STEP_LIMIT, "disk " + std::to_string(d) + " did not converge");
@@ -598,12 +1068,29 @@ int red_ensemble_get_state(red_ensemble* E, int disk, double* r, double* v)
     std::vector<d4> R(std::max(nb, 1)), V(std::max(nb, 1));
     ENS_HIP(E, hipStreamSynchronize(E->stream));
     if (nb > 0) {
-        ENS_HIP(E, stream_copy(E->stream, R.data(), E->yr + b0, sizeof(d4) * nb, hipMemcpyDeviceToHost));
-        ENS_HIP(E, stream_copy(E->stream, V.data(), E->yv + b0, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+        ENS_HIP(E, stream_copy(E->stream, R.data(), cur_r(E, disk) + b0, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+        ENS_HIP(E, stream_copy(E->stream, V.data(), cur_v(E, disk) + b0, sizeof(d4) * nb, hipMemcpyDeviceToHost));
     }
     for (int i = 0; i < nb; i++) {
         if (r) { r[3 * i] = R[i].x; r[3 * i + 1] = R[i].y; r[3 * i + 2] = R[i].z; }
         if (v) { v[3 * i] = V[i].x; v[3 * i + 1] = V[i].y; v[3 * i + 2] = V[i].z; }
+    }
+    return RED_OK;
+}
+
+int red_ensemble_get_bodies(red_ensemble* E, int disk, int counts[7], double* mass, double* radius, double* density,
+                            double* cd, int* id, int* type)
+{
+    if (!E || disk < 0 || disk >= E->nd) return RED_ERR_ARG;
+    if (counts) for (int k = 0; k < 7; k++) counts[k] = E->cnt[disk][k];
+    const int o = E->b0[disk], nb = E->b1[disk] - o;
+    for (int i = 0; i < nb; i++) {
+        if (mass) mass[i] = E->h_mass[o + i];
+        if (radius) radius[i] = E->h_radius[o + i];
+        if (density) density[i] = E->h_density[o + i];
+        if (cd) cd[i] = E->h_cd[o + i];
+        if (id) id[i] = E->h_id[o + i];
+        if (type) type[i] = E->h_type[o + i];
     }
     return RED_OK;
 }
@@ -615,6 +1102,28 @@ int red_ensemble_disk_info(red_ensemble* E, int disk, int out[3], double* t, dou
     if (out) { out[0] = E->tried[disk]; out[1] = E->failed[disk]; out[2] = E->passed[disk]; }
     if (t) *t = E->t[disk];
     if (dt_try) *dt_try = E->dt_try[disk];
+    return RED_OK;
+}
+
+int red_ensemble_disk_counters(red_ensemble* E, int disk, int out[8])
+{
+    if (!E || !out || disk < 0 || disk >= E->nd) return RED_ERR_ARG;
+    const events::Counters& C = E->evc[disk];
+    out[0] = E->tried[disk]; out[1] = E->failed[disk]; out[2] = E->passed[disk];
+    out[3] = C.collision[events::CNT_TOTAL]; out[4] = C.ejection[events::CNT_TOTAL];
+    out[5] = C.hit_centrum[events::CNT_TOTAL]; out[6] = C.close_encounter[events::CNT_TOTAL];
+    out[7] = C.event[events::CNT_LAST_CLEAR];
+    return RED_OK;
+}
+
+int red_ensemble_disk_events(red_ensemble* E, int disk, red_event* buf, int cap, int* n)
+{
+    if (!E || disk < 0 || disk >= E->nd || cap < 0 || (cap > 0 && !buf)) return RED_ERR_ARG;
+    std::vector<red_event>& L = E->log[disk];
+    const int k = std::min((int)L.size(), cap);
+    for (int i = 0; i < k; i++) buf[i] = L[i];
+    if (n) *n = (int)L.size();
+    L.erase(L.begin(), L.begin() + k);
     return RED_OK;
 }
 
@@ -639,10 +1148,11 @@ int red_ensemble_autotune(red_ensemble* E, int reps, int* best_block, int* best_
         for (int y = 0; y < 3; y++) {
             E->jtile = choices[y];
             // stage 0 recomputes f0 of the current state: identical values, so a retry
-            // that reuses f0 is unaffected
-            ENS_HIP(E, ens_force(E, 0, E->yr));   // warm-up
+            // that reuses f0 is unaffected; the records it may append are cleared with the
+            // next attempt's first launch
+            ENS_HIP(E, ens_force(E, 0, E->yr, E->yv));   // warm-up
             ENS_HIP(E, hipEventRecord(a, E->stream));
-            for (int k = 0; k < reps; k++) ENS_HIP(E, ens_force(E, 0, E->yr));
+            for (int k = 0; k < reps; k++) ENS_HIP(E, ens_force(E, 0, E->yr, E->yv));
             ENS_HIP(E, hipEventRecord(b, E->stream));
             ENS_HIP(E, hipEventSynchronize(b));
             float t = 0.0f;

@@ -2,6 +2,7 @@
 // Built with g++ -ffp-contract=off (see host_tools.hpp for why not with the HIP compiler).
 #include "host_tools.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace red {
@@ -158,4 +159,218 @@ double tolerance_step(double dt, double err, double tol, double exponent)
 }
 
 }  // namespace tools
+}  // namespace red
+
+// ---- event pipeline -------------------------------------------------------------------------
+namespace red {
+namespace events {
+
+namespace {
+enum { BT_STAR = 0, BT_SUPERPL = 4 };
+}
+
+void Counters::increment(int* cnt)
+{
+    for (int i = 0; i < 3; i++) {
+        cnt[i]++;
+        event[i]++;
+    }
+}
+
+void Counters::set(int field, int value)
+{
+    hit_centrum[field] = ejection[field] = collision[field] = close_encounter[field] = event[field] = value;
+}
+
+double mass_of_star(const BodyView& B)
+{
+    for (int j = 0; j < B.n_massive; j++)
+        if (B.type[j] == BT_STAR) return B.mass[j];
+    return -1.0;
+}
+
+bool event_key_less(const red_event& a, const red_event& b)
+{
+    if (a.attempt != b.attempt) return a.attempt < b.attempt;
+    if (a.stage != b.stage) return a.stage < b.stage;
+    if (a.sink != b.sink) return a.sink < b.sink;
+    return a.source < b.source;
+}
+
+void create_sp_events(const std::vector<red_event>& ev, std::vector<red_event>& sp)
+{
+    const int ec = (int)ev.size();
+    std::vector<char> processed(ec, 0);
+    sp.clear();
+    for (int k = 0; k < ec; k++) {
+        if (processed[k]) continue;
+        processed[k] = 1;
+        red_event cur = ev[k];
+        for (int i = k + 1; i < ec; i++) {
+            if (cur.id1 == ev[i].id1 && cur.id2 == ev[i].id2) {
+                processed[i] = 1;
+                if (cur.t > ev[i].t) cur = ev[i];
+            }
+        }
+        sp.push_back(cur);
+    }
+}
+
+int handle_collision_pair(BodyView& B, red_event* e, bool kernel_negated, std::vector<BodyUpdate>& ups,
+                          std::string& msg)
+{
+    const int surv = e->idx1, merg = e->idx2;
+    if (B.type[merg] == BT_SUPERPL) {
+        msg = "Collision between a massive body and a super-planetesimal is not yet implemented.";
+        return RED_ERR_UNSUPPORTED;
+    }
+    const double p1[4] = {B.mass[surv], B.radius[surv], B.density[surv], B.cd[surv]};
+    const double p2[4] = {B.mass[merg], B.radius[merg], B.density[merg], B.cd[merg]};
+    std::memcpy(e->p1, p1, sizeof(p1));
+    std::memcpy(e->p2, p2, sizeof(p2));
+    tools::mass_weighted_mean(p1[0], p2[0], e->r1, e->r2, e->rs);
+    tools::mass_weighted_mean(p1[0], p2[0], e->v1, e->v2, e->vs);
+    tools::calc_physical_properties(p1, p2, e->ps);
+    B.mass[surv] = e->ps[0]; B.radius[surv] = e->ps[1]; B.density[surv] = e->ps[2]; B.cd[surv] = e->ps[3];
+    // merger: inactive, massless, far away (pp_disk.cu:1177-1189)
+    if (kernel_negated) B.id[merg] = -std::abs(B.id[merg]);
+    else B.id[merg] *= -1;
+    B.mass[merg] = 0.0; B.density[merg] = 0.0; B.radius[merg] = 0.0;
+    BodyUpdate us;
+    us.idx = surv; us.id = B.id[surv];
+    us.r[0] = e->rs[0]; us.r[1] = e->rs[1]; us.r[2] = e->rs[2]; us.r[3] = B.mass[surv];
+    us.v[0] = e->vs[0]; us.v[1] = e->vs[1]; us.v[2] = e->vs[2]; us.v[3] = 0.0;
+    us.mass = B.mass[surv]; us.radius = B.radius[surv];
+    BodyUpdate um;
+    um.idx = merg; um.id = B.id[merg];
+    um.r[0] = 1.0e20; um.r[1] = 0.0; um.r[2] = 0.0; um.r[3] = 0.0;
+    um.v[0] = um.v[1] = um.v[2] = um.v[3] = 0.0;
+    um.mass = 0.0; um.radius = 0.0;
+    ups.push_back(us);
+    ups.push_back(um);
+    return RED_OK;
+}
+
+void close_encounter_elements(const BodyView& B, red_event* e)
+{
+    const double ms = mass_of_star(B);
+    const double mu1 = tools::GAUSS2 * (ms + B.mass[e->idx1]);
+    const double mu2 = tools::GAUSS2 * (ms + B.mass[e->idx2]);
+    double r1[3], v1[3], r2[3], v2[3];
+    for (int k = 0; k < 3; k++) {
+        r1[k] = e->r1[k] - e->rs[k];
+        r2[k] = e->r2[k] - e->rs[k];
+        v1[k] = (e->v1[k] - e->vs[k]) * tools::GAUSS;
+        v2[k] = (e->v2[k] - e->vs[k]) * tools::GAUSS;
+    }
+    tools::calculate_orbital_element(mu1, r1, v1, e->oe1);
+    tools::calculate_orbital_element(mu2, r2, v2, e->oe2);
+    const double p1[4] = {B.mass[e->idx1], B.radius[e->idx1], B.density[e->idx1], B.cd[e->idx1]};
+    const double p2[4] = {B.mass[e->idx2], B.radius[e->idx2], B.density[e->idx2], B.cd[e->idx2]};
+    std::memcpy(e->p1, p1, sizeof(p1));
+    std::memcpy(e->p2, p2, sizeof(p2));
+    for (int k = 0; k < 3; k++) { e->rs[k] = 0.0; e->vs[k] = 0.0; }
+}
+
+int ejection_pass(BodyView& B, std::vector<red_event>& ev, Counters& C, std::vector<BodyUpdate>& ups,
+                  std::string& msg)
+{
+    std::stable_sort(ev.begin(), ev.end(), [](const red_event& a, const red_event& b) { return a.sink < b.sink; });
+    for (auto& e : ev) B.id[e.idx2] = -B.id[e.idx2];   // mirror the scan's negation
+    // ejection records carry the parameters at detection (pp_disk.cu:58-66), before any
+    // hit-centrum merge of this pass changes the star
+    for (auto& e : ev) {
+        if (e.name != RED_EV_EJECTION) continue;
+        const int i = e.idx2;
+        const double p0[4] = {B.mass[0], B.radius[0], B.density[0], B.cd[0]};
+        const double pi[4] = {B.mass[i], B.radius[i], B.density[i], B.cd[i]};
+        std::memcpy(e.p1, p0, sizeof(p0));
+        std::memcpy(e.p2, pi, sizeof(pi));
+        std::memcpy(e.rs, e.r1, sizeof(e.rs));
+        std::memcpy(e.vs, e.v1, sizeof(e.vs));
+        std::memcpy(e.ps, e.p1, sizeof(e.ps));
+    }
+    for (auto& e : ev) {
+        if (e.name == RED_EV_EJECTION) {
+            C.increment(C.ejection);
+        } else {
+            const int rc = handle_collision_pair(B, &e, true, ups, msg);
+            if (rc) return rc;
+            C.increment(C.hit_centrum);
+        }
+    }
+    C.ejection[CNT_LAST_STEP] = 0;
+    C.hit_centrum[CNT_LAST_STEP] = 0;
+    return RED_OK;
+}
+
+int collision_pass(BodyView& B, const std::vector<red_event>& raw, std::vector<red_event>& sp, Counters& C,
+                   std::vector<BodyUpdate>& ups, std::string& msg)
+{
+    create_sp_events(raw, sp);
+    // close-encounter records take the parameters the bodies had at detection, i.e. before
+    // any merge of this pass
+    for (auto& e : sp)
+        if (e.name != RED_EV_COLLISION) close_encounter_elements(B, &e);
+    for (auto& e : sp) {
+        if (e.name == RED_EV_COLLISION) {
+            const int rc = handle_collision_pair(B, &e, false, ups, msg);
+            if (rc) return rc;
+            C.increment(C.collision);
+        } else {
+            C.increment(C.close_encounter);
+        }
+    }
+    C.collision[CNT_LAST_STEP] = 0;
+    C.close_encounter[CNT_LAST_STEP] = 0;
+    return RED_OK;
+}
+
+void close_encounter_pass(const BodyView& B, const std::vector<red_event>& raw, std::vector<red_event>& sp,
+                          Counters& C)
+{
+    sp = raw;
+    for (auto& e : sp) {
+        close_encounter_elements(B, &e);
+        C.increment(C.close_encounter);
+    }
+    C.close_encounter[CNT_LAST_STEP] = 0;
+}
+
+int ejection_name(const double* q, double ej, double hc)
+{
+    const double r2 = (q[0] * q[0]) + (q[1] * q[1]) + (q[2] * q[2]);
+    if (0.0 < ej && ej * ej < r2) return RED_EV_EJECTION;
+    if (0.0 < hc && hc * hc > r2) return RED_EV_HIT_CENTRUM;
+    return RED_EV_NONE;
+}
+
+void ejection_scan(int n, const double* r4, const double* v4, int* id, const int* type, double t, double ej,
+                   double hc, std::vector<red_event>& out)
+{
+    out.clear();
+    for (int i = 0; i < n; i++) {
+        if (!(id[i] > 0 && type[i] != BT_STAR)) continue;
+        const double* q = r4 + 4 * i;
+        const int name = ejection_name(q, ej, hc);
+        if (name == RED_EV_NONE) continue;
+        const double r2 = (q[0] * q[0]) + (q[1] * q[1]) + (q[2] * q[2]);
+        red_event e;
+        std::memset(&e, 0, sizeof(e));
+        e.name = name;
+        e.id1 = id[0]; e.idx1 = 0;
+        e.id2 = id[i]; e.idx2 = i;
+        e.attempt = -1; e.stage = -1; e.sink = i; e.source = 0; e.pad = 0;
+        e.t = t;
+        e.d = std::sqrt(r2);
+        for (int c = 0; c < 3; c++) {
+            e.r1[c] = r4[c]; e.v1[c] = v4[c];
+            e.r2[c] = q[c]; e.v2[c] = v4[4 * i + c];
+        }
+        id[i] = -id[i];
+        out.push_back(e);
+    }
+}
+
+}  // namespace events
 }  // namespace red

@@ -41,25 +41,6 @@ namespace {
 constexpr int CHUNK = 64;   // mixed sweep: sources per FP32 partial sum
 
 
-__device__ __forceinline__ void put3(double* d, const d4& q) { d[0] = q.x; d[1] = q.y; d[2] = q.z; }
-
-// Mutual Hill radius helper: cube root by bit-seeded Newton iterations using only
-// +, -, *, / so that host and device produce identical results.
-__device__ __forceinline__ double cbrt_ieee(double x)
-{
-    if (!(x > 0.0)) return 0.0;
-    unsigned long long b = __double_as_longlong(x);
-    unsigned long long e = (b >> 52) & 0x7ff;
-    long long eb = (long long)e - 1023;
-    long long q = eb >= 0 ? eb / 3 : -((-eb + 2) / 3);
-    double y = __longlong_as_double((unsigned long long)(q + 1023) << 52);
-    for (int k = 0; k < 60; k++) {
-        double yn = y - (y * y * y - x) / (3.0 * y * y);
-        if (yn == y) break;
-        y = yn;
-    }
-    return y;
-}
 
 // Appends one pair event (pp_disk.cu:32-67, 265-291).
 __device__ void append_pair_event(const PairTest& P, const EventSink& E, Ctl* ctl, int i, int j, double d,

@@ -41,7 +41,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A)
     double dt;
     if (A.disk_of) {
         const int d = A.disk_of[i];
-        if (!(A.disk_flags[d] & 1)) return;   // disk done or waiting: its buffers stay as they are
+        // a slot's tail left by a compaction, or a disk done or waiting: buffers stay as they are
+        if (d < 0 || !(A.disk_flags[d] & 1)) return;
         dt = A.disk_dt[d];
     } else {
         dt = A.ctl->dt;
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(256) void k_err_disks(StageArgs A)
     int d = -1;
     if (i < A.n) {
         d = A.disk_of[i];
-        if ((A.disk_flags[d] & 1) && A.err_on[i]) {
+        if (d >= 0 && (A.disk_flags[d] & 1) && A.err_on[i]) {
             const double3 er = err_rec<EF>(A, i, false);
             const double3 ev = err_rec<EF>(A, i, true);
             upd_max(m, nan, er.x); upd_max(m, nan, er.y); upd_max(m, nan, er.z);

@@ -19,6 +19,7 @@
 #include <mutex>
 
 #include "../../include/rednative.h"
+#include "host_tools.hpp"
 
 namespace red {
 
@@ -164,6 +165,26 @@ hipError_t launch_stage(int form, const StageArgs& A, hipStream_t s);
 hipError_t launch_err(int form, const StageArgs& A, hipStream_t s);
 hipError_t launch_reset_ctl(Ctl* ctl, int clear_events, hipStream_t s);
 
+__device__ __forceinline__ void put3(double* d, const d4& q) { d[0] = q.x; d[1] = q.y; d[2] = q.z; }
+
+// Mutual Hill radius helper: cube root by bit-seeded Newton iterations using only
+// +, -, *, / so that host and device produce identical results.
+__device__ __forceinline__ double cbrt_ieee(double x)
+{
+    if (!(x > 0.0)) return 0.0;
+    unsigned long long b = __double_as_longlong(x);
+    unsigned long long e = (b >> 52) & 0x7ff;
+    long long eb = (long long)e - 1023;
+    long long q = eb >= 0 ? eb / 3 : -((-eb + 2) / 3);
+    double y = __longlong_as_double((unsigned long long)(q + 1023) << 52);
+    for (int k = 0; k < 60; k++) {
+        double yn = y - (y * y * y - x) / (3.0 * y * y);
+        if (yn == y) break;
+        y = yn;
+    }
+    return y;
+}
+
 // Ejection / hit-centrum predicate of one active non-star body (pp_disk.cu:72-142):
 // |r|^2 > R_ej^2 (R_ej > 0) is an ejection, otherwise |r|^2 < R_hc^2 (R_hc > 0) a hit.
 __device__ __forceinline__ int ejection_name(const d4& q, double ej, double hc)
@@ -178,14 +199,7 @@ __device__ __forceinline__ int ejection_name(const d4& q, double ej, double hc)
 hipError_t launch_ejection_scan(const d4* r, const d4* v, int* id, const int* type, int n, double ej,
                                 double hc, Ctl* ctl, red_event* buf, int cap, hipStream_t s);
 
-// Scatter of merged-body updates.
-struct BodyUpdate {
-    int idx;
-    int id;
-    double r[4];
-    double v[4];
-    double mass, radius;
-};
+// Scatter of merged-body updates (BodyUpdate: host_tools.hpp).
 hipError_t launch_apply_updates(const BodyUpdate* u, int nu, d4* r, d4* v, double* mass, double* radius,
                                 int* id, hipStream_t s);
 

@@ -21,6 +21,8 @@ Workloads (masses in M_sun, lengths in AU, velocities in AU/day):
   and surface density ~ a^-1.5 on [0.5, 5] AU; bodies with m >= 1e-10 are
   protoplanets (type 3, self-interacting), the rest planetesimals (type 5).
 * ``disk_allsi`` n self-interacting protoplanets around a star (kernel tests).
+* ``ensemble``    independent disks of N log-uniform in [64, 4096] bodies, each drawn as
+  disk_small with its planetesimals out to a per-disk radius uniform in [3, 8] AU.
 """
 from __future__ import annotations
 
@@ -187,14 +189,17 @@ def disk_allsi(n: int, seed: int = 7, a_range=(0.4, 4.0), m_range=(1e-12, 1e-9))
     return B.build()
 
 
-def ensemble(n_disks: int = 512, seed: int = 404, n_range=(64, 4096), r_out_range=(3.0, 8.0)) -> List[BodySet]:
-    """Independent disks: N log-uniform, outer radius uniform; star + planetesimals."""
+def ensemble(n_disks: int = 512, seed: int = 404, n_range=(64, 4096), r_out_range=(3.0, 8.0),
+             giants: bool = True) -> List[BodySet]:
+    """Independent disks: N log-uniform, planetesimal outer radius uniform; each drawn as the
+    256-body disk (star, the two giants, planetesimals), N bodies in all."""
     rng = _rng(seed)
     out = []
     for d in range(n_disks):
         n = int(math.floor(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
         r_out = float(rng.uniform(*r_out_range))
-        out.append(disk_small(n_pl=n - 1, seed=seed * 1000 + d, a_range=(0.4, r_out), giants=False))
+        out.append(disk_small(n_pl=n - (3 if giants else 1), seed=seed * 1000 + d, a_range=(0.4, r_out),
+                              giants=giants))
     return out
 
 

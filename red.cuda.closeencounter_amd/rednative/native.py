@@ -122,6 +122,9 @@ SIGNATURES = {
     "red_ensemble_set_time": (_I, [_P, _D]),
     "red_ensemble_advance": (_I, [_P, _D, _I, _P]),
     "red_ensemble_get_state": (_I, [_P, _I, _P, _P]),
+    "red_ensemble_get_bodies": (_I, [_P, _I, _IP, _P, _P, _P, _P, _P, _P]),
+    "red_ensemble_disk_counters": (_I, [_P, _I, _IP]),
+    "red_ensemble_disk_events": (_I, [_P, _I, ctypes.POINTER(RedEvent), _I, _IP]),
     "red_ensemble_disk_info": (_I, [_P, _I, _IP, _DP, _DP]),
     "red_ensemble_autotune": (_I, [_P, _I, _IP, _IP, _DP]),
     "red_mlp_param_count": (_I, []),
@@ -486,7 +489,8 @@ class Disk:
 
 def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_check_for_tp=False,
             use_graph=True, device=0, block=0, frame_center=1, precision=PREC_FP64, radii_enhance_factor=0.0,
-            encounter_hill_factor=0.0, close_encounter=False, ejection=0.0, hit_centrum=0.0):
+            encounter_hill_factor=0.0, close_encounter=False, ejection=0.0, hit_centrum=0.0, inner_steps=True,
+            event_capacity=0):
     """Ensemble parameters; every option is passed on, so that the library can refuse the ones
     an ensemble does not support instead of their being dropped here."""
     prm = RedParams()
@@ -494,7 +498,8 @@ def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_ch
     prm.adaptive = int(adaptive)
     prm.tolerance = tolerance
     prm.error_check_for_tp = int(error_check_for_tp)
-    prm.inner_steps = 1
+    prm.inner_steps = int(inner_steps)
+    prm.event_capacity = event_capacity
     prm.frame_center = frame_center
     prm.precision = precision
     prm.radii_enhance_factor = radii_enhance_factor
@@ -563,11 +568,41 @@ class Ensemble:
         self._chk(self.L.red_ensemble_advance(self.h, float(t_stop), int(max_attempts), ctypes.byref(info)))
         return dict(supersteps=info.supersteps, active_disks=info.active_disks, seconds=info.seconds)
 
+    def counts(self, disk):
+        c = np.zeros(7, dtype=np.int32)
+        self._chk(self.L.red_ensemble_get_bodies(self.h, disk, c.ctypes.data_as(_IP), None, None, None, None, None,
+                                                 None))
+        return [int(x) for x in c]
+
     def state(self, disk):
-        n = self.sizes[disk]
+        n = sum(self.counts(disk))
         r = np.zeros((n, 3)); v = np.zeros((n, 3))
         self._chk(self.L.red_ensemble_get_state(self.h, disk, _ptr(r), _ptr(v)))
         return r, v
+
+    def bodies(self, disk):
+        """Parameters of a disk's current bodies: p = (mass, radius, density, cd), id, type."""
+        n = sum(self.counts(disk))
+        m = np.zeros(n); R = np.zeros(n); rho = np.zeros(n); cd = np.zeros(n)
+        ids = np.zeros(n, dtype=np.int32); types = np.zeros(n, dtype=np.int32)
+        self._chk(self.L.red_ensemble_get_bodies(self.h, disk, None, _ptr(m), _ptr(R), _ptr(rho), _ptr(cd),
+                                                 _ptr(ids), _ptr(types)))
+        return dict(p=np.stack([m, R, rho, cd], axis=1), id=ids, type=types)
+
+    def counters(self, disk):
+        c = np.zeros(8, dtype=np.int32)
+        self._chk(self.L.red_ensemble_disk_counters(self.h, disk, c.ctypes.data_as(_IP)))
+        keys = ("tried", "failed", "passed", "n_collision", "n_ejection", "n_hit_centrum",
+                "n_close_encounter", "n_event_last_clear")
+        return dict(zip(keys, (int(x) for x in c)))
+
+    def events(self, disk):
+        """Records of the event passes the disk handled since the last call, in order."""
+        n = ctypes.c_int()
+        self._chk(self.L.red_ensemble_disk_events(self.h, disk, None, 0, ctypes.byref(n)))
+        buf = (RedEvent * max(n.value, 1))()
+        self._chk(self.L.red_ensemble_disk_events(self.h, disk, buf, n.value, ctypes.byref(n)))
+        return [event_dict(buf[k]) for k in range(n.value)]
 
     def disk_info(self, disk):
         c = np.zeros(3, dtype=np.int32); t = ctypes.c_double(); dt = ctypes.c_double()

@@ -99,3 +99,113 @@ def test_ensemble_advance_in_parts(native_mod):
         assert parts.disk_info(k) == whole.disk_info(k)
         alone = run_alone(native_mod, b, t2, **kw)
         assert_bitwise(parts.state(k)[0], alone.state()["r"], f"disk {k} alone")
+
+
+def _alone_main_loop(native_mod, b, t_stop, loop_kw, **kw):
+    """One disk through the reference's driver loop (ejection check, step, collision or
+    close-encounter pass, rebuild check; main.cu:176-255) up to t_stop: the state and the
+    handled event records."""
+    from rednative import mainloop
+    d = native_mod.Disk(b, **kw)
+    events = []
+    prm = dict(stop_time=t_stop, output_interval=1e300, inner_steps=True, **loop_kw)
+    mainloop.run_main_loop(d, prm, lambda e: None, lambda evs: events.extend(evs))
+    return d, events
+
+
+def _member_equal(E, k, d, events, what):
+    from test_gpu_events import events_equal
+    r, v = E.state(k)
+    s = d.state()
+    assert E.counts(k) == d.counts(), (what, k)
+    assert_bitwise(r, s["r"], f"{what} disk {k} r")
+    assert_bitwise(v, s["v"], f"{what} disk {k} v")
+    bo = E.bodies(k)
+    assert_bitwise(bo["p"], s["p"], f"{what} disk {k} p")
+    assert np.array_equal(bo["id"], s["id"]), (what, k)
+    assert E.counters(k) == d.counters(), (what, k, E.counters(k), d.counters())
+    di = E.disk_info(k)
+    assert di["t"] == d.t and di["dt_try"] == d.dt_try, (what, k)
+    events_equal(events, E.events(k), f"{what} disk {k} events")
+
+
+EVENT_CASES = {
+    # integrator, step control, the pair and scan rules
+    "rk4_merge": dict(integrator=2, adaptive=False, dt_day=0.5, hit_centrum=0.3, ejection=6.0,
+                      radii_enhance_factor=2500.0),
+    "rkf78_merge": dict(integrator=4, adaptive=True, tolerance=1e-10, dt_day=0.5, hit_centrum=0.3, ejection=6.0,
+                        radii_enhance_factor=2500.0),
+    "ck_hill": dict(integrator=3, adaptive=True, tolerance=1e-10, dt_day=0.5, radii_enhance_factor=1.0,
+                    encounter_hill_factor=10.0),
+    "rk4_ce_mode": dict(integrator=2, adaptive=False, dt_day=0.5, close_encounter=True, radii_enhance_factor=500.0),
+}
+
+
+@pytest.mark.parametrize("case", sorted(EVENT_CASES))
+def test_ensemble_events_match_single_disk(native_mod, case):
+    """Members with collisions merged, close encounters reported (Hill test and close-encounter
+    mode), ejections, hit-centrum merges and compactions: every member's state, bodies,
+    counters and event records equal those of the same disk run alone through the driver
+    loop, bit for bit."""
+    from test_gpu_events import planted_disk
+    kw = EVENT_CASES[case]
+    gen = small_ensemble(n=3, seed=407)
+    for b in gen:
+        b.radius[0] = 1e-7   # as the planted disks' star: 2500 stellar radii must not swallow the disk
+    disks = [planted_disk(seed=41), planted_disk(seed=44)] + gen + [planted_disk(seed=45)]
+    print(f"case ensemble events {case} sizes={[d.n for d in disks]}", flush=True)
+    t_stop = 30.0 * 0.01720209895
+    loop_kw = {k: kw[k] for k in ("radii_enhance_factor", "encounter_hill_factor", "close_encounter") if k in kw}
+    E = native_mod.Ensemble(disks, event_capacity=200000, **kw)
+    info = E.advance(t_stop)
+    assert info["active_disks"] == 0
+    total = dict(n_collision=0, n_close_encounter=0, n_ejection=0, n_hit_centrum=0)
+    for k, b in enumerate(disks):
+        d, events = _alone_main_loop(native_mod, b, t_stop, loop_kw, event_capacity=200000, **kw)
+        _member_equal(E, k, d, events, case)
+        for key in total:
+            total[key] += d.counters()[key]
+        d.close()
+    print("events", total, flush=True)
+    if case.endswith("merge"):
+        assert total["n_collision"] > 0 and total["n_ejection"] > 0 and total["n_hit_centrum"] > 0
+    else:
+        assert total["n_close_encounter"] > 0
+
+
+def test_ensemble_config4_slice(native_mod, oracle_mod):
+    """A 64-disk slice of the 512-disk ensemble workload (seed 404: the 32 largest disks, up to
+    4096 bodies, and every 15th of the rest), each disk a star, the two giants and
+    planetesimals out to its own radius, with the 256-body disk's rules: collisions at
+    1 x (R_i + R_j), close encounters at 3 mutual Hill radii, ejection at 100 AU; RKF7(8) at
+    tolerance 1e-11 for two days.  Every member equals its disk run alone through the driver
+    loop bit for bit; the smallest and the largest equal the oracle."""
+    all_disks = generate.ensemble(n_disks=512, seed=404)
+    order = sorted(range(512), key=lambda k: -all_disks[k].n)
+    pick = sorted(order[:32] + order[32::15][:32])
+    disks = [generate.to_internal(all_disks[k]) for k in pick]
+    assert len(disks) == 64 and max(d.n for d in disks) >= 4000 and all(d.counts[1] == 2 for d in disks)
+    kw = dict(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, ejection=100.0, radii_enhance_factor=1.0,
+              encounter_hill_factor=3.0)
+    loop_kw = dict(radii_enhance_factor=1.0, encounter_hill_factor=3.0)
+    t_stop = 2.0 * 0.01720209895
+    print(f"case ensemble c4 slice n_disks={len(disks)} bodies={sum(d.n for d in disks)}", flush=True)
+    E = native_mod.Ensemble(disks, **kw)
+    info = E.advance(t_stop)
+    assert info["active_disks"] == 0
+    ce = 0
+    for k, b in enumerate(disks):
+        d, events = _alone_main_loop(native_mod, b, t_stop, loop_kw, **kw)
+        _member_equal(E, k, d, events, "c4")
+        ce += d.counters()["n_close_encounter"]
+        d.close()
+    print("close encounters", ce, flush=True)
+    from rednative import mainloop
+    for k in (min(range(64), key=lambda q: disks[q].n), max(range(64), key=lambda q: disks[q].n)):
+        o = oracle_mod.Oracle(disks[k], **kw)
+        prm = dict(stop_time=t_stop, output_interval=1e300, inner_steps=True, **loop_kw)
+        mainloop.run_main_loop(o, prm, lambda e: None)
+        r, v = E.state(k)
+        so = o.state()
+        assert_bitwise(r, so["r"], f"disk {k} vs oracle r")
+        assert_bitwise(v, so["v"], f"disk {k} vs oracle v")
