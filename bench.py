@@ -453,35 +453,45 @@ def small_disk_section(native, with_cpu=True, use_graph=True, fused_only=False):
 
 
 def ensemble_section(native, days, sample, use_graph=True):
-    """512 disks (N log-uniform in [64, 4096], seeded) integrated together against the same
-    disks stepped one context at a time (a sample of them, scaled): disk-steps per second.
-    use_graph=False: both without HIP graphs (for kernel traces)."""
-    from rednative import generate
+    """512 disks (N log-uniform in [64, 4096], seeded; each a star, the two giants and
+    planetesimals out to its own radius) integrated together with the 256-body disk's rules
+    (collisions merged at 1 x (R_i + R_j), close encounters at 3 mutual Hill radii reported,
+    ejection at 100 AU), against the same disks stepped one context at a time through the
+    same driver loop (a sample of them, scaled): disk-steps per second.  use_graph=False:
+    both without HIP graphs (for kernel traces)."""
+    from rednative import generate, mainloop
     disks = [generate.to_internal(d) for d in generate.ensemble(n_disks=512, seed=404)]
-    kw = dict(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001)
+    rules = dict(radii_enhance_factor=1.0, encounter_hill_factor=3.0)
+    kw = dict(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, ejection=100.0, **rules)
     t_stop = days * GAUSS
     E = native.Ensemble(disks, use_graph=use_graph, **kw)
     block, jtile, _ = E.autotune(reps=3)
     t0 = time.perf_counter()
     info = E.advance(t_stop)
     t_ens = time.perf_counter() - t0
-    steps_ens = sum(E.disk_info(k)["passed"] for k in range(len(disks)))
+    cnt = [E.counters(k) for k in range(len(disks))]
+    steps_ens = sum(c["passed"] for c in cnt)
+    ev = {k: sum(c[k] for c in cnt) for k in ("n_collision", "n_close_encounter", "n_ejection", "n_hit_centrum")}
     E.close()
+    prm = dict(stop_time=t_stop, output_interval=1e300, inner_steps=True, **rules)
     t1 = time.perf_counter()
     steps_one = 0
     for k in range(sample):
         d = native.Disk(disks[k], use_graph=use_graph, **kw)
-        while d.t <= t_stop:
-            d.step()
+        mainloop.run_main_loop(d, prm, lambda e: None)
         d.sync()
         steps_one += d.counters()["passed"]
         d.close()
     t_one = time.perf_counter() - t1
     rate_ens = steps_ens / t_ens
     rate_one = steps_one / t_one
-    return {"workload": f"512 disks, N log-uniform in [64, 4096] (seed 404), RKF7(8) tol 1e-11, {days} days each",
+    return {"workload": f"512 disks, N log-uniform in [64, 4096] (seed 404), each star + 2 giants + planetesimals, "
+                        f"RKF7(8) tol 1e-11, collisions 1x(R_i+R_j) merged, close encounters at 3 mutual Hill radii, "
+                        f"ejection 100 AU, {days} days each",
             "ensemble_disk_steps_per_s": rate_ens, "per_disk_disk_steps_per_s": rate_one,
             "per_disk_sample": sample, "speedup": rate_ens / rate_one, "supersteps": info["supersteps"],
+            "ms_per_superstep": 1e3 * t_ens / max(info["supersteps"], 1), "seconds": t_ens, "events": ev,
+            "host_event_seconds": info["event_seconds"], "host_rebuild_seconds": info["rebuild_seconds"],
             "autotuned_block": block, "autotuned_jtile": jtile}
 
 

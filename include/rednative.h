@@ -106,7 +106,9 @@ typedef struct red_params {
                                      occupancy (faster, not bit-identical to the
                                      reference's sequential order); 0 = sequential  */
     int    source_parts;          /* source-range split factor, 0 = automatic       */
-    int    reserved[2];
+    int    host_threads;          /* ensembles: host threads of the per-disk event
+                                     passes, 0 = min(8, hardware threads)           */
+    int    reserved[1];
 } red_params;
 
 /* Bodies in body-file order (host arrays, read during the call only). */
@@ -282,6 +284,8 @@ typedef struct red_ensemble_info {
     int    supersteps;            /* attempts launched (one graph replay each)      */
     int    active_disks;          /* disks still integrating after the call         */
     double seconds;               /* wall time of the call                          */
+    double event_seconds;         /* of which host event passes and ejection scans  */
+    double rebuild_seconds;       /* of which slot compactions                      */
 } red_ensemble_info;
 int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* params, red_ensemble** out);
 int red_ensemble_destroy(red_ensemble* ens);

@@ -411,7 +411,7 @@ int fetch_raw_events(red_ctx* c)
     c->raw.resize(n);
     if (n > 0) {
         CTX_HIP(c, stream_copy(c->stream, c->raw.data(), c->d_events, sizeof(red_event) * n, hipMemcpyDeviceToHost));
-        std::stable_sort(c->raw.begin(), c->raw.end(), events::event_key_less);
+        events::canonical_sort(c->raw);
     }
     c->raw_valid = true;
     return RED_OK;

@@ -28,10 +28,12 @@
 // The tile shape (block, jtile) is picked by red_ensemble_autotune from measured launches.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "host_tools.hpp"
@@ -357,6 +359,47 @@ __global__ __launch_bounds__(256) void k_ens_accept(d4* __restrict__ yr, d4* __r
     yv[i] = outv[i];
 }
 
+// Slot compaction of the disks being rebuilt: map[i] is -2 for bodies of other disks (left
+// alone), -1 for the slot tail (zeroed) and otherwise the index whose accepted state moves to
+// i.  The state is gathered from y_(n+1) into y, then copied back (k_ens_compact_back), so
+// no launch reads what it writes.
+struct EnsBodies {
+    d4 *yr, *yv, *outr, *outv;
+    double *mass, *radius;
+    int *id, *type, *disk_of;
+    unsigned char* err_on;
+    double *s_mass, *s_radius;   // scratch of the gather
+    int *s_id, *s_type;
+};
+
+// map[i] >= 0 is (source << 1) | err_on of the moved body.
+__global__ __launch_bounds__(256) void k_ens_compact(const int* __restrict__ map, EnsBodies B, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = map[i];
+    if (m == -2) return;
+    if (m == -1) {
+        const d4 z{0.0, 0.0, 0.0, 0.0};
+        B.yr[i] = z; B.yv[i] = z;
+        B.s_mass[i] = 0.0; B.s_radius[i] = 0.0; B.s_id[i] = 0; B.s_type[i] = 7;
+        B.disk_of[i] = -1; B.err_on[i] = 0;
+        return;
+    }
+    const int j = m >> 1;
+    B.yr[i] = B.outr[j]; B.yv[i] = B.outv[j];
+    B.s_mass[i] = B.mass[j]; B.s_radius[i] = B.radius[j]; B.s_id[i] = B.id[j]; B.s_type[i] = B.type[j];
+    B.err_on[i] = (unsigned char)(m & 1);
+}
+
+__global__ __launch_bounds__(256) void k_ens_compact_back(const int* __restrict__ map, EnsBodies B, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || map[i] == -2) return;
+    B.outr[i] = B.yr[i]; B.outv[i] = B.yv[i];
+    B.mass[i] = B.s_mass[i]; B.radius[i] = B.s_radius[i]; B.id[i] = B.s_id[i]; B.type[i] = B.s_type[i];
+}
+
 // Per-disk error maxima, NaN flags and ejection flags, and the attempt's record count.
 __global__ void k_ens_reset(unsigned long long* err, int* nan, int* probe, int n, int* ev_count)
 {
@@ -393,11 +436,18 @@ struct red_ensemble {
     unsigned char* err_on = nullptr;
     double *d_mass = nullptr, *d_radius = nullptr;
     int *d_id = nullptr, *d_type = nullptr;
+    // per-disk control, one block each way per attempt: in = dt, t, flags; out = error
+    // maxima, NaN flags, ejection flags, the record count
+    char* d_in = nullptr;
+    char* d_out = nullptr;
+    size_t in_bytes = 0, out_bytes = 0;
     double *d_dt = nullptr, *d_t = nullptr;
     int* d_flags = nullptr;
     unsigned long long* d_err = nullptr;
     int* d_nan = nullptr;
     int* d_probe = nullptr;
+    int* d_map = nullptr;                    // slot compaction map (k_ens_compact)
+    double* d_cscratch = nullptr;            // its gather scratch: mass, radius, id, type
     EnsDiskPair* d_pair = nullptr;
     red_event* d_events = nullptr;
     int ev_cap = 0;
@@ -414,10 +464,18 @@ struct red_ensemble {
     std::vector<int> accepted_prev;
     std::vector<char> need_scan;             // the next step starts with an exact ejection scan
     std::vector<events::Counters> evc;
-    std::vector<std::vector<red_event>> pending;   // pair records since the last clear
+    // pair records of each disk's last attempt, in the pinned read-back h_events: an attempt
+    // that recorded pair events is accepted (int_rungekutta4.cu:255-264, and every attempt of
+    // a fixed step) or its records are cleared with the next attempt (RKF7(8),
+    // int_rungekutta8.cu:730-733), so a step's records are those of its last attempt
+    std::vector<std::vector<const red_event*>> pending;
+    std::vector<int> pending_attempt;        // the attempt counter when they were recorded
     std::vector<std::vector<red_event>> log;       // handled records (sp lists) not yet read
-    std::vector<red_event> h_events;
-    double* h_dt = nullptr;                // pinned staging
+    red_event* h_events = nullptr;         // pinned staging of an attempt's pair records
+    int h_events_cap = 0;
+    char* h_in = nullptr;                  // pinned staging of the control blocks
+    char* h_out = nullptr;
+    double* h_dt = nullptr;
     double* h_t = nullptr;
     int* h_flags = nullptr;
     unsigned long long* h_err = nullptr;
@@ -426,6 +484,14 @@ struct red_ensemble {
     int* h_ev_count = nullptr;
     hipGraphExec_t graph = nullptr;
     bool use_graph = true;
+    std::vector<char> compact_due;           // the disk's slot is compacted after this attempt
+    double event_s = 0.0, rebuild_s = 0.0;   // host time of the current advance call
+    std::vector<unsigned char> h_err_on;     // host mirror of err_on
+    int* h_map = nullptr;                    // pinned; -2 outside the slots being compacted
+    std::vector<EnsDiskPair> h_pair;         // host copy of d_pair
+    int tiles_cap = 0;                       // d_tiles holds tiles_cap entries, empty ones at the end
+    int host_threads = 1;
+    d4* h_state = nullptr;                   // pinned copies of yr, yv, outr, outv (n each)
 
     int nSI(int d) const { return cnt[d][0] + cnt[d][1] + cnt[d][2] + cnt[d][3]; }
     int nMassive(int d) const { return nSI(d) + cnt[d][4] + cnt[d][5]; }
@@ -446,6 +512,11 @@ struct red_ensemble {
     } while (0)
 
 namespace {
+
+double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
 
 // Tiles of every disk's three sweeps (number_of_bodies.cpp:196-258 per disk), largest first.
 std::vector<EnsTile> build_tiles(const red_ensemble* E, int block)
@@ -496,8 +567,17 @@ EnsDiskPair pair_consts(red_ensemble* E, int d)
 
 int upload_pair(red_ensemble* E, int d)
 {
-    const EnsDiskPair P = pair_consts(E, d);
-    ENS_HIP(E, stream_copy(E->stream, E->d_pair + d, &P, sizeof(P), hipMemcpyHostToDevice));
+    E->h_pair[d] = pair_consts(E, d);
+    ENS_HIP(E, stream_copy(E->stream, E->d_pair + d, &E->h_pair[d], sizeof(EnsDiskPair), hipMemcpyHostToDevice));
+    return RED_OK;
+}
+
+// The pair-test constants of the disks in `list` recomputed, the whole table uploaded once.
+int upload_pairs(red_ensemble* E, const std::vector<int>& list)
+{
+    if (E->nd == 0) return RED_OK;
+    for (int d : list) E->h_pair[d] = pair_consts(E, d);
+    ENS_HIP(E, stream_copy(E->stream, E->d_pair, E->h_pair.data(), sizeof(EnsDiskPair) * E->nd, hipMemcpyHostToDevice));
     return RED_OK;
 }
 
@@ -541,7 +621,10 @@ hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v)
     P.cap = E->ev_cap;
     P.count = E->d_ev_count;
     const int need = stage == 0 ? 2 : 1;
-    if (E->pair_tests)
+    // RKF7(8) clears the records at every attempt start (int_rungekutta8.cu:730-733), so f0's
+    // would be dropped unread: its launch runs without pair tests
+    const bool ev = E->pair_tests && !(stage == 0 && E->S.clear_events_each_attempt);
+    if (ev)
         hipLaunchKernelGGL(k_ens_force<true>, dim3(E->n_tiles), dim3(E->block), 0, E->stream, E->d_tiles, r, v,
                            E->acc[stage], E->d_mass, E->d_radius, E->d_id, E->d_flags, need, E->jtile, P);
     else
@@ -610,18 +693,23 @@ hipError_t run_attempt(red_ensemble* E)
     return hipGraphLaunch(E->graph, E->stream);
 }
 
+// The force launches cover tiles_cap tiles, the unused ones empty (no sinks, no sources), so
+// a rebuild that does not need more tiles keeps the launch shape and the captured graph.
 int upload_tiles(red_ensemble* E)
 {
-    SetupLock lk;
     std::vector<EnsTile> tiles = build_tiles(E, E->block);
-    (void)hipStreamSynchronize(E->stream);
-    if (E->d_tiles) (void)hipFree(E->d_tiles);
-    E->d_tiles = nullptr;
-    E->n_tiles = (int)tiles.size();
-    ENS_HIP(E, dmalloc(&E->d_tiles, tiles.size()));
-    if (!tiles.empty())
-        ENS_HIP(E, stream_copy(E->stream, E->d_tiles, tiles.data(), sizeof(EnsTile) * tiles.size(), hipMemcpyHostToDevice));
-    destroy_graph(E);
+    if ((int)tiles.size() > E->tiles_cap) {
+        SetupLock lk;
+        (void)hipStreamSynchronize(E->stream);
+        if (E->d_tiles) (void)hipFree(E->d_tiles);
+        E->d_tiles = nullptr;
+        E->tiles_cap = (int)tiles.size() + (int)tiles.size() / 4 + 64;
+        ENS_HIP(E, dmalloc(&E->d_tiles, E->tiles_cap));
+        destroy_graph(E);
+    }
+    E->n_tiles = (int)tiles.size() > 0 ? E->tiles_cap : 0;
+    tiles.resize(E->tiles_cap, EnsTile{0, 0, 0, 0, 0, 0});
+    ENS_HIP(E, stream_copy(E->stream, E->d_tiles, tiles.data(), sizeof(EnsTile) * tiles.size(), hipMemcpyHostToDevice));
     return RED_OK;
 }
 
@@ -651,15 +739,42 @@ int apply_updates(red_ensemble* E, int d, std::vector<BodyUpdate> ups)
     return upload_pair(E, d);
 }
 
+// Host copies of all four state buffers, for the scans of many disks at once (one transfer
+// each instead of two per disk).
+int fetch_states(red_ensemble* E)
+{
+    const size_t n = (size_t)std::max(E->n, 1);
+    if (!E->h_state) {
+        SetupLock lk;
+        ENS_HIP(E, hipHostMalloc((void**)&E->h_state, sizeof(d4) * 4 * n, hipHostMallocDefault));
+    }
+    const d4* src[4] = {E->yr, E->yv, E->outr, E->outv};
+    for (int k = 0; k < 4; k++)
+        ENS_HIP(E, hipMemcpyAsync(E->h_state + k * n, src[k], sizeof(d4) * n, hipMemcpyDeviceToHost, E->stream));
+    ENS_HIP(E, hipStreamSynchronize(E->stream));
+    return RED_OK;
+}
+
 // pp_disk.cu:683-715 for disk d: the scan on its current state, its records and ids, the
 // hit-centrum merges (main.cu:201-208); the pair records since the last clear are dropped.
-int ejection_check(red_ensemble* E, int d)
+// from_states: the state is read from the copies fetch_states made.
+int ejection_check(red_ensemble* E, int d, bool from_states)
 {
     const int o = E->b0[d], nb = E->b1[d] - o;
     if (nb <= 0) return RED_OK;
-    std::vector<d4> R(nb), V(nb);
-    ENS_HIP(E, stream_copy(E->stream, R.data(), cur_r(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
-    ENS_HIP(E, stream_copy(E->stream, V.data(), cur_v(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+    std::vector<d4> Rl, Vl;
+    const d4 *R, *V;
+    if (from_states) {
+        const size_t n = (size_t)std::max(E->n, 1);
+        const size_t k = E->accepted_prev[d] ? 2 : 0;
+        R = E->h_state + k * n + o;
+        V = E->h_state + (k + 1) * n + o;
+    } else {
+        Rl.resize(nb); Vl.resize(nb);
+        ENS_HIP(E, stream_copy(E->stream, Rl.data(), cur_r(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+        ENS_HIP(E, stream_copy(E->stream, Vl.data(), cur_v(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+        R = Rl.data(); V = Vl.data();
+    }
     std::vector<int> ids(E->h_id.begin() + o, E->h_id.begin() + o + nb);   // the device's ids
     std::vector<red_event> ev;
     events::ejection_scan(nb, &R[0].x, &V[0].x, ids.data(), E->h_type.data() + o, E->t[d], E->prm.ejection,
@@ -680,104 +795,192 @@ int ejection_check(red_ensemble* E, int d)
     return upload_pair(E, d);
 }
 
-// pp_disk.cu:893-907, 1324-1409 and number_of_bodies.cpp:27-104 for disk d: its active
-// bodies move to the front of its slot in order, the rest of the slot is left out of every
-// launch.
-int compact(red_ensemble* E, int d)
+// pp_disk.cu:893-907, 1324-1409 and number_of_bodies.cpp:27-104 for every disk marked in
+// compact_due (each accepted its last attempt, so its state is y_(n+1)): its active bodies
+// move to the front of its slot in order, the rest of the slot is left out of every launch.
+// The host mirror is compacted here; the device arrays by two gather launches over one map
+// for all the disks rebuilt after an attempt.
+int compact_all(red_ensemble* E)
 {
-    const int o = E->b0[d], nb = E->b1[d] - o;
-    std::vector<d4> R(std::max(nb, 1)), V(std::max(nb, 1));
-    if (nb > 0) {
-        ENS_HIP(E, stream_copy(E->stream, R.data(), cur_r(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
-        ENS_HIP(E, stream_copy(E->stream, V.data(), cur_v(E, d) + o, sizeof(d4) * nb, hipMemcpyDeviceToHost));
+    std::vector<int> list;
+    for (int d = 0; d < E->nd; d++)
+        if (E->compact_due[d]) list.push_back(d);
+    if (list.empty()) return RED_OK;
+    int* map = E->h_map;
+    for (int d : list) {
+        if (!E->accepted_prev[d]) return E->fail(RED_ERR_ARG, "compaction of a disk whose state is not y_(n+1)");
+        const int o = E->b0[d], nb = E->b1[d] - o;
+        int removed[7] = {0};
+        std::vector<int> keep;
+        for (int i = 0; i < nb; i++) {
+            const int t = E->h_type[o + i];
+            if (0 < E->h_id[o + i] && 7 > t) { keep.push_back(i); continue; }
+            if (t < 0 || t > 7) return E->fail(RED_ERR_ARG, "Undefined body type!");
+            if (t == 7) continue;
+            removed[t]++;
+        }
+        for (int k = 0; k < 7; k++) E->cnt[d][k] -= removed[k];
+        const int m = (int)keep.size();
+        int nn = 0;
+        for (int k = 0; k < 7; k++) nn += E->cnt[d][k];
+        if (nn != m) return E->fail(RED_ERR_ARG, "Error: number of copied bodies does not equal to the number of active bodies.");
+        E->b1[d] = o + m;
+        const int nm = E->nMassive(d);
+        for (int k = 0; k < m; k++) {
+            const int i = keep[k];
+            const int eon = E->prm.error_check_for_tp || k < nm ? 1 : 0;
+            map[o + k] = ((o + i) << 1) | eon;
+            E->h_err_on[o + k] = (unsigned char)eon;
+            E->h_mass[o + k] = E->h_mass[o + i]; E->h_radius[o + k] = E->h_radius[o + i];
+            E->h_density[o + k] = E->h_density[o + i]; E->h_cd[o + k] = E->h_cd[o + i];
+            E->h_id[o + k] = E->h_id[o + i]; E->h_type[o + k] = E->h_type[o + i];
+        }
+        // the slot's tail: massless, inactive padding that no launch visits
+        for (int k = m; k < nb; k++) {
+            map[o + k] = -1;
+            E->h_err_on[o + k] = 0;
+            E->h_mass[o + k] = 0.0; E->h_radius[o + k] = 0.0; E->h_density[o + k] = 0.0; E->h_cd[o + k] = 0.0;
+            E->h_id[o + k] = 0; E->h_type[o + k] = 7;
+            E->disk_of_h[o + k] = -1;
+        }
+        E->evc[d].set(events::CNT_LAST_CLEAR, 0);
     }
-    int removed[7] = {0};
-    std::vector<int> keep;
-    for (int i = 0; i < nb; i++) {
-        const int t = E->h_type[o + i];
-        if (0 < E->h_id[o + i] && 7 > t) { keep.push_back(i); continue; }
-        if (t < 0 || t > 7) return E->fail(RED_ERR_ARG, "Undefined body type!");
-        if (t == 7) continue;
-        removed[t]++;
+    const int n = E->n;
+    if (n > 0) {
+        EnsBodies B;
+        B.yr = E->yr; B.yv = E->yv; B.outr = E->outr; B.outv = E->outv;
+        B.mass = E->d_mass; B.radius = E->d_radius; B.id = E->d_id; B.type = E->d_type;
+        B.disk_of = E->disk_of; B.err_on = E->err_on;
+        B.s_mass = E->d_cscratch; B.s_radius = E->d_cscratch + n;
+        B.s_id = (int*)(E->d_cscratch + 2 * (size_t)n); B.s_type = B.s_id + n;
+        const dim3 gb((n + 255) / 256), bb(256);
+        ENS_HIP(E, stream_copy(E->stream, E->d_map, map, sizeof(int) * n, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_ens_compact, gb, bb, 0, E->stream, E->d_map, B, n);
+        ENS_HIP(E, hipGetLastError());
+        hipLaunchKernelGGL(k_ens_compact_back, gb, bb, 0, E->stream, E->d_map, B, n);
+        ENS_HIP(E, hipGetLastError());
+        ENS_HIP(E, hipStreamSynchronize(E->stream));
     }
-    for (int k = 0; k < 7; k++) E->cnt[d][k] -= removed[k];
-    const int m = (int)keep.size();
-    int nn = 0;
-    for (int k = 0; k < 7; k++) nn += E->cnt[d][k];
-    if (nn != m) return E->fail(RED_ERR_ARG, "Error: number of copied bodies does not equal to the number of active bodies.");
-    std::vector<d4> R2(std::max(m, 1)), V2(std::max(m, 1));
-    for (int k = 0; k < m; k++) {
-        const int i = keep[k];
-        R2[k] = R[i]; V2[k] = V[i];
-        E->h_mass[o + k] = E->h_mass[o + i]; E->h_radius[o + k] = E->h_radius[o + i];
-        E->h_density[o + k] = E->h_density[o + i]; E->h_cd[o + k] = E->h_cd[o + i];
-        E->h_id[o + k] = E->h_id[o + i]; E->h_type[o + k] = E->h_type[o + i];
+    // the map back to "left alone" for the next rebuild
+    for (int d : list) {
+        const int end = d + 1 < E->nd ? E->b0[d + 1] : E->n;
+        for (int i = E->b0[d]; i < end; i++) map[i] = -2;
+        E->compact_due[d] = 0;
     }
-    // the slot's tail: massless, inactive padding that no launch visits
-    for (int k = m; k < nb; k++) {
-        E->h_mass[o + k] = 0.0; E->h_radius[o + k] = 0.0; E->h_density[o + k] = 0.0; E->h_cd[o + k] = 0.0;
-        E->h_id[o + k] = 0; E->h_type[o + k] = 7;
-        E->disk_of_h[o + k] = -1;
-        R2.resize(std::max(nb, 1)); V2.resize(std::max(nb, 1));
-        R2[k] = d4{0.0, 0.0, 0.0, 0.0}; V2[k] = d4{0.0, 0.0, 0.0, 0.0};
-    }
-    E->b1[d] = o + m;
-    const int nm = E->nMassive(d);
-    std::vector<unsigned char> eon(std::max(nb, 1));
-    for (int k = 0; k < nb; k++) eon[k] = (unsigned char)(k < m && (E->prm.error_check_for_tp || k < nm) ? 1 : 0);
-    if (nb > 0) {
-        for (d4* q : {E->yr, E->outr}) ENS_HIP(E, stream_copy(E->stream, q + o, R2.data(), sizeof(d4) * nb, hipMemcpyHostToDevice));
-        for (d4* q : {E->yv, E->outv}) ENS_HIP(E, stream_copy(E->stream, q + o, V2.data(), sizeof(d4) * nb, hipMemcpyHostToDevice));
-        ENS_HIP(E, stream_copy(E->stream, E->d_mass + o, E->h_mass.data() + o, sizeof(double) * nb, hipMemcpyHostToDevice));
-        ENS_HIP(E, stream_copy(E->stream, E->d_radius + o, E->h_radius.data() + o, sizeof(double) * nb, hipMemcpyHostToDevice));
-        ENS_HIP(E, stream_copy(E->stream, E->d_id + o, E->h_id.data() + o, sizeof(int) * nb, hipMemcpyHostToDevice));
-        ENS_HIP(E, stream_copy(E->stream, E->d_type + o, E->h_type.data() + o, sizeof(int) * nb, hipMemcpyHostToDevice));
-        ENS_HIP(E, stream_copy(E->stream, E->disk_of + o, E->disk_of_h.data() + o, sizeof(int) * nb, hipMemcpyHostToDevice));
-        ENS_HIP(E, stream_copy(E->stream, E->err_on + o, eon.data(), nb, hipMemcpyHostToDevice));
-    }
-    E->evc[d].set(events::CNT_LAST_CLEAR, 0);
-    int rc = upload_pair(E, d);
+    int rc = upload_pairs(E, list);
     if (rc) return rc;
     return upload_tiles(E);
 }
 
-// The checks after an accepted step of disk d (main.cu:225-255), on its new state.
-// *changed: the state or the bodies changed, so the ejection predicate of the attempt no
-// longer describes the state the next step starts from.
-int after_step(red_ensemble* E, int d, bool* changed)
+// The checks after an accepted step of disk d (main.cu:225-255), on its new state: the
+// event passes on the host mirror of this disk alone (disks run on separate threads).
+// ups: the merges' device updates (disk-local indices); *compact: the slot is rebuilt.
+int after_step_host(red_ensemble* E, int d, std::vector<BodyUpdate>& ups, bool* compact, std::string& msg)
 {
-    *changed = false;
+    *compact = false;
     const red_params& P = E->prm;
-    std::vector<red_event>& raw = E->pending[d];
+    std::vector<const red_event*>& raw = E->pending[d];
+    std::vector<red_event>& log = E->log[d];
+    // the pass's sp list is made in place at the end of the disk's log, the records stamped
+    // with the step's attempt
+    auto stamp = [&](size_t base) {
+        for (size_t k = base; k < log.size(); k++) { log[k].attempt = E->pending_attempt[d]; log[k].pad = 0; }
+    };
     if (P.close_encounter) {
         if (E->pair_tests && !raw.empty()) {
-            std::stable_sort(raw.begin(), raw.end(), events::event_key_less);
-            std::vector<red_event> sp;
-            events::close_encounter_pass(E->view(d), raw, sp, E->evc[d]);
-            E->log[d].insert(E->log[d].end(), sp.begin(), sp.end());
+            events::canonical_sort(raw);
+            const size_t base = log.size();
+            log.resize(base + raw.size());
+            for (size_t k = 0; k < raw.size(); k++) log[base + k] = *raw[k];
+            stamp(base);
+            events::close_encounter_sp_pass(E->view(d), log.data() + base, raw.size(), E->evc[d]);
             raw.clear();
         }
         return RED_OK;
     }
     if (E->pair_tests && !raw.empty()) {
-        std::stable_sort(raw.begin(), raw.end(), events::event_key_less);
-        std::vector<red_event> sp;
-        std::vector<BodyUpdate> ups;
-        std::string msg;
+        events::canonical_sort(raw);
+        const size_t base = log.size();
+        events::create_sp_events(raw, log);
+        stamp(base);
         events::BodyView B = E->view(d);
-        int rc = events::collision_pass(B, raw, sp, E->evc[d], ups, msg);
-        if (rc) return E->fail(rc, "disk " + std::to_string(d) + ": " + msg);
-        rc = apply_updates(E, d, ups);
+        const int rc = events::collision_sp_pass(B, log.data() + base, log.size() - base, E->evc[d], ups, msg);
         if (rc) return rc;
-        if (!ups.empty()) *changed = true;
-        E->log[d].insert(E->log[d].end(), sp.begin(), sp.end());
         raw.clear();
     }
-    if (E->evc[d].event[events::CNT_LAST_CLEAR] >= 8) {
-        const int rc = compact(E, d);
-        if (rc) return rc;
-        *changed = true;
+    *compact = E->evc[d].event[events::CNT_LAST_CLEAR] >= 8;
+    return RED_OK;
+}
+
+// f(d) for every disk of the list, on up to `threads` threads.
+template <class F>
+void for_disks(int threads, const std::vector<int>& list, F f)
+{
+    const int nt = std::min<int>(threads, (int)list.size());
+    if (nt <= 1) {
+        for (int d : list) f(d);
+        return;
     }
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t k; (k = next.fetch_add(1)) < list.size();) f(list[k]);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+}
+
+// The checks after the accepted steps of the disks in `list`: their event passes, then every
+// merge's device update in one launch (each disk's state is y_(n+1)), then the rebuilds.
+int after_steps(red_ensemble* E, const std::vector<int>& list)
+{
+    if (list.empty()) return RED_OK;
+    const auto t_ev = std::chrono::steady_clock::now();
+    size_t records = 0;
+    for (int d : list) records += E->pending[d].size();
+    std::vector<std::vector<BodyUpdate>> ups(E->nd);
+    std::vector<int> rcs(E->nd, RED_OK);
+    std::vector<std::string> msgs(E->nd);
+    std::vector<char> comp(E->nd, 0);
+    for_disks(records >= 1024 ? E->host_threads : 1, list, [&](int d) {
+        bool c = false;
+        rcs[d] = after_step_host(E, d, ups[d], &c, msgs[d]);
+        comp[d] = c;
+    });
+    std::vector<BodyUpdate> all;
+    std::vector<int> merged;
+    for (int d : list) {
+        if (rcs[d]) return E->fail(rcs[d], "disk " + std::to_string(d) + ": " + msgs[d]);
+        if (!ups[d].empty()) {
+            merged.push_back(d);
+            for (BodyUpdate u : ups[d]) { u.idx += E->b0[d]; all.push_back(u); }
+        }
+        if (comp[d]) E->compact_due[d] = 1;
+        if (E->scan_on) E->need_scan[d] = E->h_probe[d] || !ups[d].empty() || comp[d];
+    }
+    if (!all.empty()) {
+        if ((int)all.size() > E->updates_cap) {
+            SetupLock lk;
+            (void)hipStreamSynchronize(E->stream);
+            if (E->d_updates) (void)hipFree(E->d_updates);
+            E->d_updates = nullptr;
+            E->updates_cap = (int)all.size() * 2;
+            ENS_HIP(E, dmalloc(&E->d_updates, E->updates_cap));
+        }
+        ENS_HIP(E, hipMemcpyAsync(E->d_updates, all.data(), sizeof(BodyUpdate) * all.size(), hipMemcpyHostToDevice,
+                                  E->stream));
+        ENS_HIP(E, launch_apply_updates(E->d_updates, (int)all.size(), E->outr, E->outv, E->d_mass, E->d_radius,
+                                        E->d_id, E->stream));
+        ENS_HIP(E, hipStreamSynchronize(E->stream));
+        const int rc = upload_pairs(E, merged);
+        if (rc) return rc;
+    }
+    E->event_s += seconds_since(t_ev);
+    const auto t_rb = std::chrono::steady_clock::now();
+    const int rc = compact_all(E);
+    if (rc) return rc;
+    E->rebuild_s += seconds_since(t_rb);
     return RED_OK;
 }
 
@@ -792,16 +995,15 @@ int red_ensemble_destroy(red_ensemble* E)
     destroy_graph(E);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     for (void* p : {(void*)E->yr, (void*)E->yv, (void*)E->outr, (void*)E->outv, (void*)E->disk_of, (void*)E->err_on,
-                    (void*)E->d_dt, (void*)E->d_t, (void*)E->d_flags, (void*)E->d_err, (void*)E->d_nan,
-                    (void*)E->d_probe, (void*)E->d_tiles, (void*)E->d_mass, (void*)E->d_radius, (void*)E->d_id,
-                    (void*)E->d_type, (void*)E->d_pair, (void*)E->d_events, (void*)E->d_ev_count,
+                    (void*)E->d_in, (void*)E->d_out, (void*)E->d_map, (void*)E->d_cscratch, (void*)E->d_tiles,
+                    (void*)E->d_mass,
+                    (void*)E->d_radius, (void*)E->d_id, (void*)E->d_type, (void*)E->d_pair, (void*)E->d_events,
                     (void*)E->d_updates})
         (void)hipFree(p);
     for (auto p : E->sr) (void)hipFree(p);
     for (auto p : E->sv) (void)hipFree(p);
     for (auto p : E->acc) (void)hipFree(p);
-    for (void* p : {(void*)E->h_dt, (void*)E->h_t, (void*)E->h_flags, (void*)E->h_err, (void*)E->h_nan,
-                    (void*)E->h_probe, (void*)E->h_ev_count})
+    for (void* p : {(void*)E->h_in, (void*)E->h_out, (void*)E->h_events, (void*)E->h_state, (void*)E->h_map})
         if (p) (void)hipHostFree(p);
     if (E->stream) (void)hipStreamDestroy(E->stream);
     delete E;
@@ -878,24 +1080,29 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     ENS_HIP(E, dmalloc(&E->d_radius, n));
     ENS_HIP(E, dmalloc(&E->d_id, n));
     ENS_HIP(E, dmalloc(&E->d_type, n));
-    ENS_HIP(E, dmalloc(&E->d_dt, E->nd));
-    ENS_HIP(E, dmalloc(&E->d_t, E->nd));
-    ENS_HIP(E, dmalloc(&E->d_flags, E->nd));
-    ENS_HIP(E, dmalloc(&E->d_err, E->nd));
-    ENS_HIP(E, dmalloc(&E->d_nan, E->nd));
-    ENS_HIP(E, dmalloc(&E->d_probe, E->nd));
+    ENS_HIP(E, dmalloc(&E->d_map, n));
+    ENS_HIP(E, dmalloc(&E->d_cscratch, 3 * (size_t)n));
+    ENS_HIP(E, hipHostMalloc((void**)&E->h_map, sizeof(int) * n, hipHostMallocDefault));
+    for (int i = 0; i < n; i++) E->h_map[i] = -2;
+    const size_t nd = (size_t)std::max(E->nd, 1);
+    E->in_bytes = nd * (sizeof(double) * 2 + sizeof(int));
+    E->out_bytes = nd * (sizeof(unsigned long long) + 2 * sizeof(int)) + sizeof(int);
+    ENS_HIP(E, dmalloc(&E->d_in, E->in_bytes));
+    ENS_HIP(E, dmalloc(&E->d_out, E->out_bytes));
+    ENS_HIP(E, hipHostMalloc((void**)&E->h_in, E->in_bytes, hipHostMallocDefault));
+    ENS_HIP(E, hipHostMalloc((void**)&E->h_out, E->out_bytes, hipHostMallocDefault));
+    E->d_dt = (double*)E->d_in; E->d_t = E->d_dt + nd; E->d_flags = (int*)(E->d_t + nd);
+    E->h_dt = (double*)E->h_in; E->h_t = E->h_dt + nd; E->h_flags = (int*)(E->h_t + nd);
+    E->d_err = (unsigned long long*)E->d_out; E->d_nan = (int*)(E->d_err + nd);
+    E->d_probe = E->d_nan + nd; E->d_ev_count = E->d_probe + nd;
+    E->h_err = (unsigned long long*)E->h_out; E->h_nan = (int*)(E->h_err + nd);
+    E->h_probe = E->h_nan + nd; E->h_ev_count = E->h_probe + nd;
+    memset(E->h_in, 0, E->in_bytes);
+    memset(E->h_out, 0, E->out_bytes);
     ENS_HIP(E, dmalloc(&E->d_pair, E->nd));
-    E->ev_cap = p->event_capacity > 0 ? p->event_capacity : std::max(4096, std::min(E->n, 65536));
+    E->ev_cap = p->event_capacity > 0 ? p->event_capacity : std::max(4096, E->n);   // as a single disk's
     ENS_HIP(E, dmalloc(&E->d_events, E->ev_cap));
-    ENS_HIP(E, dmalloc(&E->d_ev_count, 1));
-    ENS_HIP(E, hipMemsetAsync(E->d_ev_count, 0, sizeof(int), E->stream));
-    ENS_HIP(E, hipHostMalloc((void**)&E->h_dt, sizeof(double) * E->nd, hipHostMallocDefault));
-    ENS_HIP(E, hipHostMalloc((void**)&E->h_t, sizeof(double) * E->nd, hipHostMallocDefault));
-    ENS_HIP(E, hipHostMalloc((void**)&E->h_flags, sizeof(int) * E->nd, hipHostMallocDefault));
-    ENS_HIP(E, hipHostMalloc((void**)&E->h_err, sizeof(unsigned long long) * E->nd, hipHostMallocDefault));
-    ENS_HIP(E, hipHostMalloc((void**)&E->h_nan, sizeof(int) * E->nd, hipHostMallocDefault));
-    ENS_HIP(E, hipHostMalloc((void**)&E->h_probe, sizeof(int) * E->nd, hipHostMallocDefault));
-    ENS_HIP(E, hipHostMalloc((void**)&E->h_ev_count, sizeof(int), hipHostMallocDefault));
+    ENS_HIP(E, hipMemsetAsync(E->d_out, 0, E->out_bytes, E->stream));
     if (E->n > 0) {
         ENS_HIP(E, stream_copy(E->stream, E->yr, r.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->yv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
@@ -903,6 +1110,7 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
         ENS_HIP(E, stream_copy(E->stream, E->outv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->disk_of, E->disk_of_h.data(), sizeof(int) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->err_on, eon.data(), E->n, hipMemcpyHostToDevice));
+        E->h_err_on = eon;
         ENS_HIP(E, stream_copy(E->stream, E->d_mass, E->h_mass.data(), sizeof(double) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->d_radius, E->h_radius.data(), sizeof(double) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->d_id, E->h_id.data(), sizeof(int) * E->n, hipMemcpyHostToDevice));
@@ -916,11 +1124,17 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     E->need_scan.assign(E->nd, 1);
     E->evc.assign(E->nd, events::Counters());
     E->pending.assign(E->nd, {});
+    E->pending_attempt.assign(E->nd, 0);
     E->log.assign(E->nd, {});
-    for (int d = 0; d < E->nd; d++) {
-        int rc = upload_pair(E, d);
-        if (rc) return rc;
-    }
+    E->compact_due.assign(E->nd, 0);
+    if (E->h_err_on.empty()) E->h_err_on.assign(n, 0);
+    E->h_pair.assign(E->nd, EnsDiskPair{});
+    std::vector<int> all_disks(E->nd);
+    for (int d = 0; d < E->nd; d++) all_disks[d] = d;
+    int rc0 = upload_pairs(E, all_disks);
+    if (rc0) return rc0;
+    E->host_threads = p->host_threads > 0 ? p->host_threads
+                                          : std::max(1, std::min(8, (int)std::thread::hardware_concurrency()));
     if (p->block >= 64) E->block = std::min(256, p->block / 64 * 64);
     int rc = upload_tiles(E);
     if (rc) return rc;
@@ -945,22 +1159,39 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
     const bool adaptive = E->prm.adaptive && S.err_form >= 0;
     int attempts = 0, active = 0;
     const auto t_begin = std::chrono::steady_clock::now();
+    E->event_s = 0.0;
+    E->rebuild_s = 0.0;
     // a disk done against an earlier t_stop continues (a run to t1, then to t2, steps as one
     // run to t2); a disk left in a retry by max_attempts resumes it
     for (int d = 0; d < E->nd; d++)
         if (E->phase[d] == 2) E->phase[d] = 0;
     for (;;) {
         active = 0;
+        // the loop iteration's ejection / hit-centrum check, before the step; many disks'
+        // scans read one host copy of the state buffers
+        int scans = 0;
+        for (int d = 0; d < E->nd; d++) {
+            if (E->phase[d] == 0 && !(E->t[d] <= t_stop)) E->phase[d] = 2;
+            scans += E->phase[d] == 0 && E->scan_on && E->need_scan[d];
+        }
+        if (scans > 0) {
+            const auto t_scan = std::chrono::steady_clock::now();
+            const bool bulk = scans >= 8;
+            if (bulk) {
+                const int rc = fetch_states(E);
+                if (rc) return rc;
+            }
+            for (int d = 0; d < E->nd; d++) {
+                if (!(E->phase[d] == 0 && E->scan_on && E->need_scan[d])) continue;
+                const int rc = ejection_check(E, d, bulk);
+                if (rc) return rc;
+                E->need_scan[d] = 0;
+            }
+            E->event_s += seconds_since(t_scan);
+        }
         for (int d = 0; d < E->nd; d++) {
             int f = 0;
-            if (E->phase[d] == 0 && !(E->t[d] <= t_stop)) E->phase[d] = 2;
             if (E->phase[d] == 0) {
-                // the loop iteration's ejection / hit-centrum check, before the step
-                if (E->scan_on && E->need_scan[d]) {
-                    const int rc = ejection_check(E, d);
-                    if (rc) return rc;
-                    E->need_scan[d] = 0;
-                }
                 f |= 1 | 2;
                 E->iter[d] = 0;
             } else if (E->phase[d] == 1) {
@@ -973,16 +1204,9 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
             E->h_t[d] = E->t[d];
         }
         if (active == 0 || attempts >= max_attempts) break;
-        ENS_HIP(E, hipMemcpyAsync(E->d_flags, E->h_flags, sizeof(int) * E->nd, hipMemcpyHostToDevice, E->stream));
-        ENS_HIP(E, hipMemcpyAsync(E->d_dt, E->h_dt, sizeof(double) * E->nd, hipMemcpyHostToDevice, E->stream));
-        ENS_HIP(E, hipMemcpyAsync(E->d_t, E->h_t, sizeof(double) * E->nd, hipMemcpyHostToDevice, E->stream));
+        ENS_HIP(E, hipMemcpyAsync(E->d_in, E->h_in, E->in_bytes, hipMemcpyHostToDevice, E->stream));
         ENS_HIP(E, run_attempt(E));
-        ENS_HIP(E, hipMemcpyAsync(E->h_err, E->d_err, sizeof(unsigned long long) * E->nd, hipMemcpyDeviceToHost, E->stream));
-        ENS_HIP(E, hipMemcpyAsync(E->h_nan, E->d_nan, sizeof(int) * E->nd, hipMemcpyDeviceToHost, E->stream));
-        if (E->scan_on)
-            ENS_HIP(E, hipMemcpyAsync(E->h_probe, E->d_probe, sizeof(int) * E->nd, hipMemcpyDeviceToHost, E->stream));
-        if (E->pair_tests)
-            ENS_HIP(E, hipMemcpyAsync(E->h_ev_count, E->d_ev_count, sizeof(int), hipMemcpyDeviceToHost, E->stream));
+        ENS_HIP(E, hipMemcpyAsync(E->h_out, E->d_out, E->out_bytes, hipMemcpyDeviceToHost, E->stream));
         ENS_HIP(E, hipStreamSynchronize(E->stream));
         attempts++;
         // the attempt's pair records, to their disks
@@ -990,19 +1214,29 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         if (ne > E->ev_cap)
             return E->fail(RED_ERR_EVENT_OVERFLOW, "pair events exceed the event capacity (" + std::to_string(ne) +
                                                        " > " + std::to_string(E->ev_cap) + ")");
-        E->h_events.resize(ne);
-        if (ne > 0)
-            ENS_HIP(E, stream_copy(E->stream, E->h_events.data(), E->d_events, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
-        for (int d = 0; d < E->nd; d++)
-            if ((E->h_flags[d] & 1) && S.clear_events_each_attempt) E->pending[d].clear();   // int_rungekutta8.cu:730-733
-        for (const red_event& e0 : E->h_events) {
-            red_event e = e0;
-            const int d = e.pad;
-            if (S.clear_events_each_attempt && e.stage == 0) continue;   // f0's records, cleared with the attempt
-            e.attempt = E->iter[d];
-            e.pad = 0;
-            E->pending[d].push_back(e);
+        if (ne > E->h_events_cap) {
+            SetupLock lk;
+            if (E->h_events) (void)hipHostFree(E->h_events);
+            E->h_events = nullptr;
+            E->h_events_cap = std::max(ne, 2 * E->h_events_cap);
+            ENS_HIP(E, hipHostMalloc((void**)&E->h_events, sizeof(red_event) * E->h_events_cap, hipHostMallocDefault));
         }
+        if (ne > 0)
+            ENS_HIP(E, stream_copy(E->stream, E->h_events, E->d_events, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
+        for (int d = 0; d < E->nd; d++) {
+            if (!(E->h_flags[d] & 1)) continue;
+            // records still pending here would point into the read-back just overwritten
+            if (!E->pending[d].empty() && !S.clear_events_each_attempt)
+                return E->fail(RED_ERR_ARG, "disk " + std::to_string(d) + ": pair records of a rejected attempt");
+            E->pending[d].clear();
+            E->pending_attempt[d] = E->iter[d];
+        }
+        for (int q = 0; q < ne; q++) {
+            const red_event& e = E->h_events[q];
+            if (S.clear_events_each_attempt && e.stage == 0) continue;   // f0's records, cleared with the attempt
+            E->pending[e.pad].push_back(&e);
+        }
+        std::vector<int> accepted;
         for (int d = 0; d < E->nd; d++) {
             E->accepted_prev[d] = 0;
             if (!(E->h_flags[d] & 1)) continue;
@@ -1030,15 +1264,14 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
                 E->t[d] += E->dt_did[d];
                 E->phase[d] = 0;
                 E->accepted_prev[d] = 1;
-                bool changed = false;
-                const int rc = after_step(E, d, &changed);
-                if (rc) return rc;
-                if (E->scan_on) E->need_scan[d] = E->h_probe[d] || changed;
+                accepted.push_back(d);
             } else {
                 E->phase[d] = 1;
                 if (E->iter[d] >= 1000) return E->fail(RED_ERR_STEP_LIMIT, "disk " + std::to_string(d) + " did not converge");
             }
         }
+        const int rc = after_steps(E, accepted);
+        if (rc) return rc;
     }
     // apply the last accepted results so that the state buffers hold y
     bool pending = false;
@@ -1056,7 +1289,9 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         int still = 0;
         for (int d = 0; d < E->nd; d++) still += E->phase[d] == 1 || (E->phase[d] == 0 && E->t[d] <= t_stop);
         info->active_disks = still;
-        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_begin).count();
+        info->seconds = seconds_since(t_begin);
+        info->event_seconds = E->event_s;
+        info->rebuild_seconds = E->rebuild_s;
     }
     return RED_OK;
 }

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <utility>
 
 namespace red {
 namespace tools {
@@ -197,23 +198,96 @@ bool event_key_less(const red_event& a, const red_event& b)
     return a.source < b.source;
 }
 
+namespace {
+
+// A record's canonical key packed into two words that compare as (attempt, stage) and
+// (sink, source) do as signed ints, and its position, which orders equal keys.
+struct PackedKey {
+    uint64_t hi, lo;
+    uint32_t idx;
+};
+
+inline uint64_t biased(int x) { return (uint64_t)((uint32_t)x ^ 0x80000000u); }
+
+template <class Get>
+std::vector<PackedKey> canonical_keys(size_t n, Get get)
+{
+    std::vector<PackedKey> k(n);
+    for (size_t i = 0; i < n; i++) {
+        const red_event& e = get(i);
+        k[i] = PackedKey{(biased(e.attempt) << 32) | biased(e.stage), (biased(e.sink) << 32) | biased(e.source),
+                         (uint32_t)i};
+    }
+    std::sort(k.begin(), k.end(), [](const PackedKey& a, const PackedKey& b) {
+        if (a.hi != b.hi) return a.hi < b.hi;
+        if (a.lo != b.lo) return a.lo < b.lo;
+        return a.idx < b.idx;
+    });
+    return k;
+}
+
+}  // namespace
+
+void canonical_sort(std::vector<const red_event*>& ev)
+{
+    const size_t n = ev.size();
+    if (n < 2) return;
+    const std::vector<PackedKey> k = canonical_keys(n, [&](size_t i) -> const red_event& { return *ev[i]; });
+    std::vector<const red_event*> out(n);
+    for (size_t i = 0; i < n; i++) out[i] = ev[k[i].idx];
+    ev.swap(out);
+}
+
+void canonical_sort(std::vector<red_event>& ev)
+{
+    const size_t n = ev.size();
+    if (n < 2) return;
+    const std::vector<PackedKey> k = canonical_keys(n, [&](size_t i) -> const red_event& { return ev[i]; });
+    std::vector<red_event> out(n);
+    for (size_t i = 0; i < n; i++) out[i] = ev[k[i].idx];
+    ev.swap(out);
+}
+
+// The reference compares every record with every later one (O(E^2)); its result is, for each
+// (id1, id2) in the order of its first record, the record of smallest t, the first of equal
+// times (a later record replaces the kept one only when its t is smaller).  Sorting the
+// records by (pair, position) gives the same list in O(E log E).
+void create_sp_events(const std::vector<const red_event*>& ev, std::vector<red_event>& sp)
+{
+    // (pair, position) sorted: each pair's records in list order, so its first record and
+    // the kept one (replaced only by a strictly smaller t) are found in one sweep
+    struct PairPos {
+        uint64_t pair;
+        uint32_t pos;
+    };
+    const size_t n = ev.size();
+    std::vector<PairPos> v(n);
+    for (size_t i = 0; i < n; i++)
+        v[i] = PairPos{((uint64_t)(uint32_t)ev[i]->id1 << 32) | (uint32_t)ev[i]->id2, (uint32_t)i};
+    std::sort(v.begin(), v.end(), [](const PairPos& a, const PairPos& b) {
+        return a.pair != b.pair ? a.pair < b.pair : a.pos < b.pos;
+    });
+    std::vector<std::pair<uint32_t, uint32_t>> groups;   // (first position, kept position)
+    for (size_t a = 0; a < n;) {
+        size_t b = a;
+        uint32_t kept = v[a].pos;
+        for (; b < n && v[b].pair == v[a].pair; b++)
+            if (ev[v[b].pos]->t < ev[kept]->t) kept = v[b].pos;
+        groups.emplace_back(v[a].pos, kept);
+        a = b;
+    }
+    std::sort(groups.begin(), groups.end());
+    const size_t base = sp.size();
+    sp.resize(base + groups.size());
+    for (size_t k = 0; k < groups.size(); k++) sp[base + k] = *ev[groups[k].second];
+}
+
 void create_sp_events(const std::vector<red_event>& ev, std::vector<red_event>& sp)
 {
-    const int ec = (int)ev.size();
-    std::vector<char> processed(ec, 0);
+    std::vector<const red_event*> p(ev.size());
+    for (size_t k = 0; k < ev.size(); k++) p[k] = &ev[k];
     sp.clear();
-    for (int k = 0; k < ec; k++) {
-        if (processed[k]) continue;
-        processed[k] = 1;
-        red_event cur = ev[k];
-        for (int i = k + 1; i < ec; i++) {
-            if (cur.id1 == ev[i].id1 && cur.id2 == ev[i].id2) {
-                processed[i] = 1;
-                if (cur.t > ev[i].t) cur = ev[i];
-            }
-        }
-        sp.push_back(cur);
-    }
+    create_sp_events(p, sp);
 }
 
 int handle_collision_pair(BodyView& B, red_event* e, bool kernel_negated, std::vector<BodyUpdate>& ups,
@@ -308,11 +382,18 @@ int collision_pass(BodyView& B, const std::vector<red_event>& raw, std::vector<r
                    std::vector<BodyUpdate>& ups, std::string& msg)
 {
     create_sp_events(raw, sp);
+    return collision_sp_pass(B, sp.data(), sp.size(), C, ups, msg);
+}
+
+int collision_sp_pass(BodyView& B, red_event* sp, size_t n, Counters& C, std::vector<BodyUpdate>& ups,
+                      std::string& msg)
+{
     // close-encounter records take the parameters the bodies had at detection, i.e. before
     // any merge of this pass
-    for (auto& e : sp)
-        if (e.name != RED_EV_COLLISION) close_encounter_elements(B, &e);
-    for (auto& e : sp) {
+    for (size_t k = 0; k < n; k++)
+        if (sp[k].name != RED_EV_COLLISION) close_encounter_elements(B, &sp[k]);
+    for (size_t k = 0; k < n; k++) {
+        red_event& e = sp[k];
         if (e.name == RED_EV_COLLISION) {
             const int rc = handle_collision_pair(B, &e, false, ups, msg);
             if (rc) return rc;
@@ -330,8 +411,13 @@ void close_encounter_pass(const BodyView& B, const std::vector<red_event>& raw, 
                           Counters& C)
 {
     sp = raw;
-    for (auto& e : sp) {
-        close_encounter_elements(B, &e);
+    close_encounter_sp_pass(B, sp.data(), sp.size(), C);
+}
+
+void close_encounter_sp_pass(const BodyView& B, red_event* sp, size_t n, Counters& C)
+{
+    for (size_t k = 0; k < n; k++) {
+        close_encounter_elements(B, &sp[k]);
         C.increment(C.close_encounter);
     }
     C.close_encounter[CNT_LAST_STEP] = 0;

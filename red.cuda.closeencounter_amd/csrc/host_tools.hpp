@@ -76,9 +76,16 @@ struct BodyView {
 double mass_of_star(const BodyView& B);
 // Canonical order of pair records: (attempt, stage, sink, source).
 bool event_key_less(const red_event& a, const red_event& b);
+// Puts records into canonical order, stably (the order of equal keys kept); sorts small keys
+// and moves each record once.
+void canonical_sort(std::vector<red_event>& ev);
+// The same order for a list of records held elsewhere (records stay where they are).
+void canonical_sort(std::vector<const red_event*>& ev);
 // pp_disk.cu:1111-1149: one record per (id1, id2) in first-occurrence order, the earliest t
 // kept (the first of equal times).
 void create_sp_events(const std::vector<red_event>& raw, std::vector<red_event>& sp);
+// The same for a list of records held elsewhere; the sp list is appended to out.
+void create_sp_events(const std::vector<const red_event*>& raw, std::vector<red_event>& out);
 // pp_disk.cu:1151-1202 for the record's pair (idx1 survives): the merge on the host mirror
 // and the two device updates it needs.  kernel_negated: the merger's id was already made
 // negative by the detection pass.  Returns RED_OK or RED_ERR_UNSUPPORTED (with msg).
@@ -96,9 +103,14 @@ int ejection_pass(BodyView& B, std::vector<red_event>& ev, Counters& C, std::vec
 // merged, close-encounter records of the Hill test reported.
 int collision_pass(BodyView& B, const std::vector<red_event>& raw, std::vector<red_event>& sp, Counters& C,
                    std::vector<BodyUpdate>& ups, std::string& msg);
+// The same on the pass's sp list (n records) already made by create_sp_events.
+int collision_sp_pass(BodyView& B, red_event* sp, size_t n, Counters& C, std::vector<BodyUpdate>& ups,
+                      std::string& msg);
 // Close-encounter mode (pp_disk.cu:740-770, inner_steps): every raw record reported.
 void close_encounter_pass(const BodyView& B, const std::vector<red_event>& raw, std::vector<red_event>& sp,
                           Counters& C);
+// The same on sp (n records) = the raw records.
+void close_encounter_sp_pass(const BodyView& B, red_event* sp, size_t n, Counters& C);
 // The ejection / hit-centrum predicate of pp_disk.cu:72-142 on the host (the same IEEE
 // operations as the device's, so the same answer), and the scan of one disk's state:
 // records for every active non-star body that meets it, ids negated, as the device scan.

@@ -58,7 +58,8 @@ class RedParams(ctypes.Structure):
                 ("dt_day", ctypes.c_double), ("precision", ctypes.c_int), ("event_capacity", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("kahan_pairs", ctypes.c_int), ("device", ctypes.c_int),
                 ("block", ctypes.c_int), ("sinks_per_thread", ctypes.c_int), ("strategy", ctypes.c_int),
-                ("fp64_split", ctypes.c_int), ("source_parts", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
+                ("fp64_split", ctypes.c_int), ("source_parts", ctypes.c_int), ("host_threads", ctypes.c_int),
+                ("reserved", ctypes.c_int * 1)]
 
 
 class RedBodies(ctypes.Structure):
@@ -149,7 +150,8 @@ MLP_FEATURES = 12
 
 
 class RedEnsembleInfo(ctypes.Structure):
-    _fields_ = [("supersteps", ctypes.c_int), ("active_disks", ctypes.c_int), ("seconds", ctypes.c_double)]
+    _fields_ = [("supersteps", ctypes.c_int), ("active_disks", ctypes.c_int), ("seconds", ctypes.c_double),
+                ("event_seconds", ctypes.c_double), ("rebuild_seconds", ctypes.c_double)]
 
 _lib = None
 
@@ -490,7 +492,7 @@ class Disk:
 def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_check_for_tp=False,
             use_graph=True, device=0, block=0, frame_center=1, precision=PREC_FP64, radii_enhance_factor=0.0,
             encounter_hill_factor=0.0, close_encounter=False, ejection=0.0, hit_centrum=0.0, inner_steps=True,
-            event_capacity=0):
+            event_capacity=0, host_threads=0):
     """Ensemble parameters; every option is passed on, so that the library can refuse the ones
     an ensemble does not support instead of their being dropped here."""
     prm = RedParams()
@@ -511,6 +513,7 @@ def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_ch
     prm.use_graph = int(use_graph)
     prm.device = device
     prm.block = block
+    prm.host_threads = host_threads
     return prm
 
 
@@ -566,7 +569,8 @@ class Ensemble:
     def advance(self, t_stop, max_attempts=1 << 30):
         info = RedEnsembleInfo()
         self._chk(self.L.red_ensemble_advance(self.h, float(t_stop), int(max_attempts), ctypes.byref(info)))
-        return dict(supersteps=info.supersteps, active_disks=info.active_disks, seconds=info.seconds)
+        return dict(supersteps=info.supersteps, active_disks=info.active_disks, seconds=info.seconds,
+                    event_seconds=info.event_seconds, rebuild_seconds=info.rebuild_seconds)
 
     def counts(self, disk):
         c = np.zeros(7, dtype=np.int32)
