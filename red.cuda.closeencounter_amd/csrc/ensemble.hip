@@ -807,6 +807,7 @@ int compact_all(red_ensemble* E)
         if (E->compact_due[d]) list.push_back(d);
     if (list.empty()) return RED_OK;
     int* map = E->h_map;
+    std::vector<int> moved;   // disks whose slot changes
     for (int d : list) {
         if (!E->accepted_prev[d]) return E->fail(RED_ERR_ARG, "compaction of a disk whose state is not y_(n+1)");
         const int o = E->b0[d], nb = E->b1[d] - o;
@@ -824,6 +825,12 @@ int compact_all(red_ensemble* E)
         int nn = 0;
         for (int k = 0; k < 7; k++) nn += E->cnt[d][k];
         if (nn != m) return E->fail(RED_ERR_ARG, "Error: number of copied bodies does not equal to the number of active bodies.");
+        E->evc[d].set(events::CNT_LAST_CLEAR, 0);
+        if (m == nb) {   // every body kept in place: the rebuild leaves the disk as it is
+            E->compact_due[d] = 0;
+            continue;
+        }
+        moved.push_back(d);
         E->b1[d] = o + m;
         const int nm = E->nMassive(d);
         for (int k = 0; k < m; k++) {
@@ -843,8 +850,8 @@ int compact_all(red_ensemble* E)
             E->h_id[o + k] = 0; E->h_type[o + k] = 7;
             E->disk_of_h[o + k] = -1;
         }
-        E->evc[d].set(events::CNT_LAST_CLEAR, 0);
     }
+    if (moved.empty()) return RED_OK;
     const int n = E->n;
     if (n > 0) {
         EnsBodies B;
@@ -862,12 +869,12 @@ int compact_all(red_ensemble* E)
         ENS_HIP(E, hipStreamSynchronize(E->stream));
     }
     // the map back to "left alone" for the next rebuild
-    for (int d : list) {
+    for (int d : moved) {
         const int end = d + 1 < E->nd ? E->b0[d + 1] : E->n;
         for (int i = E->b0[d]; i < end; i++) map[i] = -2;
         E->compact_due[d] = 0;
     }
-    int rc = upload_pairs(E, list);
+    int rc = upload_pairs(E, moved);
     if (rc) return rc;
     return upload_tiles(E);
 }
@@ -957,7 +964,9 @@ int after_steps(red_ensemble* E, const std::vector<int>& list)
             for (BodyUpdate u : ups[d]) { u.idx += E->b0[d]; all.push_back(u); }
         }
         if (comp[d]) E->compact_due[d] = 1;
-        if (E->scan_on) E->need_scan[d] = E->h_probe[d] || !ups[d].empty() || comp[d];
+        // the attempt's ejection predicate holds for the state unless a merge moved a body; a
+        // rebuild only drops inactive bodies, which no scan visits
+        if (E->scan_on) E->need_scan[d] = E->h_probe[d] || !ups[d].empty();
     }
     if (!all.empty()) {
         if ((int)all.size() > E->updates_cap) {
