@@ -108,7 +108,10 @@ typedef struct red_params {
     int    source_parts;          /* source-range split factor, 0 = automatic       */
     int    host_threads;          /* ensembles: host threads of the per-disk event
                                      passes, 0 = min(8, hardware threads)           */
-    int    reserved[1];
+    int    padded_storage;        /* -ups (options.cpp:114): the padded kernel's
+                                     semantics (pp_disk.cu:144-215): no id tests,
+                                     inactive bodies keep pulling, being pulled and
+                                     being tested for collisions until compaction   */
 } red_params;
 
 /* Bodies in body-file order (host arrays, read during the call only). */

@@ -39,7 +39,7 @@ class OrcParams(ctypes.Structure):
                 ("n_threads", ctypes.c_int),
                 ("hit_centrum", ctypes.c_double), ("ejection", ctypes.c_double),
                 ("radii_enhance_factor", ctypes.c_double), ("encounter_hill_factor", ctypes.c_double),
-                ("dt_day", ctypes.c_double)]
+                ("dt_day", ctypes.c_double), ("padded", ctypes.c_int)]
 
 
 def build() -> None:
@@ -129,12 +129,12 @@ class Oracle:
     def __init__(self, bodies, integrator=0, adaptive=False, tolerance=1e-10, dt_day=0.001,
                  error_check_for_tp=False, close_encounter=False, inner_steps=True, frame_center=1,
                  hit_centrum=0.0, ejection=0.0, radii_enhance_factor=0.0, encounter_hill_factor=0.0,
-                 stage_time_events=True, hc_reactivate=False, n_threads=1, r=None, v=None):
+                 stage_time_events=True, hc_reactivate=False, n_threads=1, padded=False, r=None, v=None):
         L = lib()
         self.L = L
         prm = OrcParams(integrator, int(adaptive), tolerance, int(error_check_for_tp), int(close_encounter),
                         int(inner_steps), frame_center, int(stage_time_events), int(hc_reactivate), n_threads,
-                        hit_centrum, ejection, radii_enhance_factor, encounter_hill_factor, dt_day)
+                        hit_centrum, ejection, radii_enhance_factor, encounter_hill_factor, dt_day, int(padded))
         counts = np.ascontiguousarray(bodies.counts, dtype=np.int32)
         ids = np.ascontiguousarray(bodies.ids, dtype=np.int32)
         types = np.ascontiguousarray(bodies.types, dtype=np.int32)

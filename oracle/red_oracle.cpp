@@ -33,6 +33,9 @@
 //                       which re-activates it as a massless body (CPU path,
 //                       pp_disk.cu:959,1178); 0 = it stays inactive (GPU path, where the
 //                       merge negates the stale host copy).
+// padded = 1 restates the padded-storage kernel (-ups, pp_disk.cu:144-215): no id test on
+// sinks, sources or pairs.  Its padding particles (massless, id 0, pp_disk.cu:1550-1571)
+// add exact zeros to every sum and never meet a pair test, so none are created here.
 // Additions the reference does not have (marked NEW): total energy and angular momentum,
 // an OpenMP sink loop whose results are bit-identical to the serial one, and an optional
 // mutual-Hill-radius close-encounter criterion.
@@ -96,6 +99,7 @@ typedef struct orc_params {
     double hit_centrum, ejection, radii_enhance_factor;
     double encounter_hill_factor;   // NEW: 0 = off
     double dt_day;                  // initial step in days (reference hard-codes 0.001, main.cu:141)
+    int    padded;                  // the padded-storage kernel's semantics (pp_disk.cu:144-215)
 } orc_params;
 }
 
@@ -306,6 +310,7 @@ struct Disk {
     bool close_encounter = false, inner_steps = true;
     int frame_center = FC_BARY;
     bool stage_time_events = true, hc_reactivate = false;
+    bool padded = false;
     double hill_factor = 0.0;
     int n_threads = 1;
     std::vector<orc_event> events;      // raw events since the last clear
@@ -376,9 +381,10 @@ struct Disk {
             a[i].z = -mu / (r2 * sqrt(r2)) * r[i].z;
             a[i].w = 0.0;
         }
-        if (!(0 < md[i].id)) return;
+        // pp_disk.cu:231-241; the padded kernel (pp_disk.cu:158-167) tests no id
+        if (!padded && !(0 < md[i].id)) return;
         for (int j = b.src0; j < b.src1; j++) {
-            if (i == j || 0 > md[j].id) continue;
+            if (i == j || (!padded && 0 > md[j].id)) continue;
             double dx = r[j].x - r[i].x;
             double dy = r[j].y - r[i].y;
             double dz = r[j].z - r[i].z;
@@ -1024,6 +1030,7 @@ void* orc_create(const int* counts, const int* id, const int* type, const int* m
     d.frame_center = prm->frame_center;
     d.stage_time_events = prm->stage_time_events != 0;
     d.hc_reactivate = prm->hc_reactivate != 0;
+    d.padded = prm->padded != 0;
     d.hill_factor = prm->encounter_hill_factor;
     d.n_threads = prm->n_threads > 0 ? prm->n_threads : 1;
     double dt = prm->dt_day;

@@ -63,6 +63,8 @@ struct red_ctx {
     double* d_radius = nullptr;
     int* d_id = nullptr;
     int* d_type = nullptr;
+    int* d_ones = nullptr;                // padded storage: every body gates as active
+    bool padded = false;
     Ctl* d_ctl = nullptr;
     Ctl* h_ctl = nullptr;                 // pinned
     red_event* d_events = nullptr;
@@ -209,7 +211,7 @@ void refresh_pair_test(red_ctx* c)
     // holds the device's and the flag stays conservative
     P.mass_core = 1;
     for (int i = 0; i < c->nMassive(); i++) {
-        if (c->h_id[i] < 0) continue;
+        if (c->h_id[i] < 0 && !c->padded) continue;   // padded: inactive sources are summed too
         uint64_t bits;
         memcpy(&bits, &c->h_mass[i], sizeof(bits));
         if ((uint32_t)((uint32_t)(bits >> 32) - 0x2D300000u) >= 0x25800000u) P.mass_core = 0;
@@ -242,9 +244,10 @@ void sweeps_of(const red_ctx* c, Sweep sw[3])
 {
     const int nsi = c->nSI(), nm = c->nMassive(), n = c->n;
     const int s0 = c->gcfg.astro ? 1 : 0;
-    sw[0] = Sweep{s0, nsi, s0, nm};
-    sw[1] = Sweep{nsi, nm, 0, nsi};
-    sw[2] = Sweep{nm, n, 0, nm};
+    const int* gate = c->padded ? c->d_ones : c->d_id;
+    sw[0] = Sweep{s0, nsi, s0, nm, gate};
+    sw[1] = Sweep{nsi, nm, 0, nsi, gate};
+    sw[2] = Sweep{nm, n, 0, nm, gate};
 }
 
 template <typename F>
@@ -281,7 +284,7 @@ hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const
     EventSink E;
     E.buf = c->d_events; E.cap = c->ev_cap; E.stage = stage; E.c_stage = c_stage;
     if (c->gcfg.precision == RED_PREC_MIXED)
-        RED_HIP_TRY(timed(c, RED_PROF_PACK, [&] { return launch_pack_mixed(r, c->d_id, c->d_src_mixed, c->nMassive(), c->stream); }));
+        RED_HIP_TRY(timed(c, RED_PROF_PACK, [&] { return launch_pack_mixed(r, c->padded ? c->d_ones : c->d_id, c->d_src_mixed, c->nMassive(), c->stream); }));
     Sweep sw[3];
     sweeps_of(c, sw);
     // The three sweeps write disjoint rows of a and append events atomically (the list is
@@ -482,6 +485,7 @@ int red_ctx_destroy(red_ctx* c)
     for (auto p : c->d_sv) (void)hipFree(p);
     for (auto p : c->d_a) (void)hipFree(p);
     (void)hipFree(c->d_mass); (void)hipFree(c->d_radius); (void)hipFree(c->d_id); (void)hipFree(c->d_type);
+    (void)hipFree(c->d_ones);
     (void)hipFree(c->d_ctl); (void)hipFree(c->d_events); (void)hipFree(c->d_events_hc); (void)hipFree(c->d_scratch);
     (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy); (void)hipFree(c->d_energy_part);
     (void)hipFree(c->d_partial); (void)hipFree(c->d_seg);
@@ -556,6 +560,14 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     c->gcfg.kahan_pairs = p->kahan_pairs;
     c->gcfg.split_max_sinks = 2048;
     c->gcfg.fp64_split = p->fp64_split;
+    c->padded = p->padded_storage != 0;
+    if (c->padded && p->frame_center == RED_FRAME_ASTRO) {
+        // the padded kernel (pp_disk.cu:144-215) is the reference's GPU path, which has no
+        // astro-centric indirect term; the astro frame here follows its CPU path
+        int rc = c->fail(RED_ERR_UNSUPPORTED, "padded storage runs in the barycentric frame only");
+        *out = c;
+        return rc;
+    }
     if (p->frame_center == RED_FRAME_ASTRO) {
         // The reference's NSI and NI sweeps run the SI code with bounds that keep the star as
         // source 0 at the origin (number_of_bodies.cpp:227-257, pp_disk.cu:574-583): its
@@ -597,6 +609,11 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     CTX_HIP(c, dmalloc(&c->d_radius, cap));
     CTX_HIP(c, dmalloc(&c->d_id, cap));
     CTX_HIP(c, dmalloc(&c->d_type, cap));
+    if (c->padded) {
+        std::vector<int> ones(std::max(cap, 1), 1);
+        CTX_HIP(c, dmalloc(&c->d_ones, cap));
+        CTX_HIP(c, stream_copy(c->stream, c->d_ones, ones.data(), sizeof(int) * ones.size(), hipMemcpyHostToDevice));
+    }
     CTX_HIP(c, dmalloc(&c->d_ctl, 1));
     CTX_HIP(c, hipMemsetAsync(c->d_ctl, 0, sizeof(Ctl), c->stream));
     CTX_HIP(c, hipHostMalloc((void**)&c->h_ctl, sizeof(Ctl), hipHostMallocDefault));

@@ -70,6 +70,7 @@ struct EnsPair {
     red_event* buf;                // records of the attempt, all disks
     int cap;
     int* count;
+    const int* gate;               // activity: the ids, or ones under padded storage
 };
 
 constexpr int MAX_JTILE = 256;
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
         const bool in = i < T.sink1;
         const int ic = in ? i : T.sink0;
         const d4 p = r[ic];
-        const bool live = in && id[ic] > 0;
+        const bool live = in && P.gate[ic] > 0;
         const double Ri = EV ? radius[ic] : 0.0;
         const double scr = EV && live ? ens_screen(P, P.dp[T.disk], Ri, mass[ic], p) : 0.0;
         double ax = 0.0, ay = 0.0, az = 0.0;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
             __syncthreads();
             for (int k = threadIdx.x; k < nj; k += blockDim.x) {
                 src[k] = r[j0 + k];
-                sact[k] = id[j0 + k] >= 0;
+                sact[k] = P.gate[j0 + k] >= 0;
             }
             __syncthreads();
             for (int k = 0; k < nj; ++k) {
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
         // with one wave it forms, then adds, each chunk.
         const int i = T.sink0;
         const d4 p = r[i];
-        const bool live = id[i] > 0;
+        const bool live = P.gate[i] > 0;
         const double Ri = EV ? radius[i] : 0.0;
         const int c = threadIdx.x;
         const int nsrc = T.src1 - T.src0;
@@ -219,7 +220,7 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
             double* t2 = term[ch & 1][2];
             for (int k = (int)threadIdx.x - p0; k < nj; k += (int)blockDim.x - p0) {
                 double tx, ty, tz;
-                const bool use = id[j0 + k] >= 0 && (j0 + k) != i;
+                const bool use = P.gate[j0 + k] >= 0 && (j0 + k) != i;
                 const double d = fp64_term(r[j0 + k], p.x, p.y, p.z, use, tx, ty, tz);
                 t0[k] = tx;
                 t1[k] = ty;
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
                     const int k = tp + m * np;
                     if (k < nj) {
                         q[m] = r[j0 + k];
-                        qa[m] = id[j0 + k];
+                        qa[m] = P.gate[j0 + k];
                     }
                 }
             };
@@ -447,6 +448,8 @@ struct red_ensemble {
     int* d_nan = nullptr;
     int* d_probe = nullptr;
     int* d_map = nullptr;                    // slot compaction map (k_ens_compact)
+    int* d_ones = nullptr;                   // padded storage: every body gates as active
+    bool padded = false;
     double* d_cscratch = nullptr;            // its gather scratch: mass, radius, id, type
     EnsDiskPair* d_pair = nullptr;
     red_event* d_events = nullptr;
@@ -620,6 +623,7 @@ hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v)
     P.buf = E->d_events;
     P.cap = E->ev_cap;
     P.count = E->d_ev_count;
+    P.gate = E->padded ? E->d_ones : E->d_id;
     const int need = stage == 0 ? 2 : 1;
     // RKF7(8) clears the records at every attempt start (int_rungekutta8.cu:730-733), so f0's
     // would be dropped unread: its launch runs without pair tests
@@ -1004,7 +1008,7 @@ int red_ensemble_destroy(red_ensemble* E)
     destroy_graph(E);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     for (void* p : {(void*)E->yr, (void*)E->yv, (void*)E->outr, (void*)E->outv, (void*)E->disk_of, (void*)E->err_on,
-                    (void*)E->d_in, (void*)E->d_out, (void*)E->d_map, (void*)E->d_cscratch, (void*)E->d_tiles,
+                    (void*)E->d_in, (void*)E->d_out, (void*)E->d_map, (void*)E->d_ones, (void*)E->d_cscratch, (void*)E->d_tiles,
                     (void*)E->d_mass,
                     (void*)E->d_radius, (void*)E->d_id, (void*)E->d_type, (void*)E->d_pair, (void*)E->d_events,
                     (void*)E->d_updates})
@@ -1090,6 +1094,12 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     ENS_HIP(E, dmalloc(&E->d_id, n));
     ENS_HIP(E, dmalloc(&E->d_type, n));
     ENS_HIP(E, dmalloc(&E->d_map, n));
+    E->padded = p->padded_storage != 0;
+    if (E->padded) {
+        std::vector<int> ones(n, 1);
+        ENS_HIP(E, dmalloc(&E->d_ones, n));
+        ENS_HIP(E, stream_copy(E->stream, E->d_ones, ones.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    }
     ENS_HIP(E, dmalloc(&E->d_cscratch, 3 * (size_t)n));
     ENS_HIP(E, hipHostMalloc((void**)&E->h_map, sizeof(int) * n, hipHostMallocDefault));
     for (int i = 0; i < n; i++) E->h_map[i] = -2;

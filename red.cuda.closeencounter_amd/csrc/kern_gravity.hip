@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
         const int k = in ? ii[s] : sw.sink0;
         const d4 q = r[k];
         xi[s] = q.x; yi[s] = q.y; zi[s] = q.z;
-        live[s] = in && id[k] > 0;
+        live[s] = in && sw.gate[k] > 0;
         Ri[s] = EV ? radius[k] : 0.0;
         ax[s] = ay[s] = az[s] = 0.0;
     }
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
         bool an[G];
         if (j + G <= jhi) {
 #pragma unroll
-            for (int k = 0; k < G; ++k) { qn[k] = r[j + k]; an[k] = id[j + k] >= 0; }
+            for (int k = 0; k < G; ++k) { qn[k] = r[j + k]; an[k] = sw.gate[j + k] >= 0; }
         }
         for (; j + G <= jhi; j += G) {
             d4 q[G];
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
             for (int k = 0; k < G; ++k) { q[k] = qn[k]; act[k] = an[k]; }
             if (j + 2 * G <= jhi) {
 #pragma unroll
-                for (int k = 0; k < G; ++k) { qn[k] = r[j + G + k]; an[k] = id[j + G + k] >= 0; }
+                for (int k = 0; k < G; ++k) { qn[k] = r[j + G + k]; an[k] = sw.gate[j + G + k] >= 0; }
             }
 #pragma unroll
             for (int k = 0; k < G; ++k)
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
                                            ay[s], az[s], P, E, ctl, r, v, mass, radius, id);
         }
         for (; j < jhi; ++j) {
-            const bool act = id[j] >= 0;
+            const bool act = sw.gate[j] >= 0;
             const d4 q = r[j];
 #pragma unroll
             for (int s = 0; s < SPT; s++)
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_grav_fp64(const d4* __restrict__ r, con
             __syncthreads();
             for (int k = threadIdx.x; k < nj; k += nt) {
                 tile[k] = r[j0 + k];
-                tact[k] = id[j0 + k] >= 0;
+                tact[k] = sw.gate[j0 + k] >= 0;
             }
             __syncthreads();
 #pragma unroll 2
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
         const bool sink_in = sk < ns;
         const int i = i0 + (sink_in ? sk : 0);
         const d4 p = r[i];
-        const bool live = sink_in && id[i] > 0;
+        const bool live = sink_in && sw.gate[i] > 0;
         const double Ri = EV ? radius[i] : 0.0;
         const double scr = EV ? pair_screen(P, Ri, mass[i], p) : 0.0;
         // Sources reach the lanes through buffer loads: the lane's byte offset is fixed for the
@@ -355,13 +355,13 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
             off_r[m] = (unsigned)(slot + m * SLOTS) * (unsigned)sizeof(d4);
             off_i[m] = (unsigned)(slot + m * SLOTS) * (unsigned)sizeof(int);
         }
-        // raw ids are kept: comparing them here would wait for the loads
+        // raw gate values are kept: comparing them here would wait for the loads
         auto fetch = [&](int c, d4* qd, int* idd) {
             const int sb = c * SQ_J;
             const __amdgpu_buffer_rsrc_t rs_r = __builtin_amdgcn_make_buffer_rsrc(
                 (void*)(r + sw.src0 + sb), 0, (nsrc - sb) * (int)sizeof(d4), BUF_RSRC_DW3);
             const __amdgpu_buffer_rsrc_t rs_i = __builtin_amdgcn_make_buffer_rsrc(
-                (void*)(id + sw.src0 + sb), 0, (nsrc - sb) * (int)sizeof(int), BUF_RSRC_DW3);
+                (void*)(sw.gate + sw.src0 + sb), 0, (nsrc - sb) * (int)sizeof(int), BUF_RSRC_DW3);
 #pragma unroll
             for (int m = 0; m < M; ++m) {
                 const double2 lo =
@@ -527,7 +527,7 @@ __global__ __launch_bounds__(64 * (NP + 1)) void k_grav_fp64_seq(const d4* __res
         const int i = i0 + lane / 3;
         const int comp = lane % 3;
         double* o = reinterpret_cast<double*>(a + i);
-        o[comp] = id[i] > 0 ? sum : 0.0;
+        o[comp] = sw.gate[i] > 0 ? sum : 0.0;
         if (comp == 0) o[3] = 0.0;
     }
 }
@@ -598,10 +598,10 @@ __global__ __launch_bounds__(256) void k_grav_fp64_astro(const d4* __restrict__ 
     double ax = -mu / (r2 * sqrt(r2)) * qi.x;
     double ay = -mu / (r2 * sqrt(r2)) * qi.y;
     double az = -mu / (r2 * sqrt(r2)) * qi.z;
-    if (id[i] > 0) {
+    if (sw.gate[i] > 0) {
         const double Ri = EV ? radius[i] : 0.0;
         for (int j = sw.src0; j < sw.src1; ++j) {
-            if (i == j || id[j] < 0) continue;
+            if (i == j || sw.gate[j] < 0) continue;
             const d4 q = r[j];
             const double dx = q.x - qi.x;
             const double dy = q.y - qi.y;
@@ -656,7 +656,7 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
         const int k = in ? ii[s] : sw.sink0;
         const d4 q = r[k];
         xi[s] = q.x; yi[s] = q.y; zi[s] = q.z;
-        live[s] = in && id[k] > 0;
+        live[s] = in && sw.gate[k] > 0;
         Ri[s] = EV ? radius[k] : 0.0;
         ax[s] = ay[s] = az[s] = 0.0;
         cx[s] = cy[s] = cz[s] = 0.0;
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
     const int jsplit = min(sw.src1, max(sw.src0, n_fp64_src));
     if (p == 0) {
         for (int j = sw.src0; j < jsplit; ++j) {
-            const bool act = id[j] >= 0;
+            const bool act = sw.gate[j] >= 0;
             const d4 q = r[j];
 #pragma unroll
             for (int s = 0; s < SPT; s++)
@@ -732,7 +732,7 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
             if (EV && live[s] && mn[s] < cut2[s]) {
                 // exact FP64 re-evaluation of the reference predicate for this chunk
                 for (int j = j0; j < j1; ++j) {
-                    if (id[j] < 0 || j == ii[s]) continue;
+                    if (sw.gate[j] < 0 || j == ii[s]) continue;
                     const d4 q = r[j];
                     const double dx = q.x - xi[s];
                     const double dy = q.y - yi[s];
@@ -761,7 +761,7 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const d4* __restrict__ par
         x += q.x; y += q.y; z += q.z;
     }
     const int i = sw.sink0 + k;
-    const bool live = id[i] > 0;
+    const bool live = sw.gate[i] > 0;
     a[i] = d4{live ? x : 0.0, live ? y : 0.0, live ? z : 0.0, 0.0};
 }
 
@@ -800,7 +800,7 @@ __global__ __launch_bounds__(256) void k_split_terms(const d4* __restrict__ r, c
     const int i1 = min(sw.sink1, i0 + SPLIT_SG);
     if (jj >= nsrc) return;
     const int j = sw.src0 + jj;
-    const bool act = id[j] >= 0;
+    const bool act = sw.gate[j] >= 0;
     const d4 q = r[j];
     const double Rj = EV ? radius[j] : 0.0;
     for (int i = i0; i < i1; ++i) {
@@ -816,7 +816,7 @@ __global__ __launch_bounds__(256) void k_split_terms(const d4* __restrict__ r, c
         row[jj] = sc * dx;
         row[nsrc + jj] = sc * dy;
         row[2 * nsrc + jj] = sc * dz;
-        if (EV && use && id[i] > 0) pair_test(P, E, ctl, i, j, d, radius[i], Rj, r, v, mass, id);
+        if (EV && use && sw.gate[i] > 0) pair_test(P, E, ctl, i, j, d, radius[i], Rj, r, v, mass, id);
     }
 }
 
@@ -843,7 +843,7 @@ __global__ __launch_bounds__(256) void k_split_sum(const double* __restrict__ T,
         double sum = 0.0;
         for (int k = 0; k < (int)blockDim.x; ++k) sum += part[k];
         double* o = reinterpret_cast<double*>(a + i);
-        o[c] = id[i] > 0 ? sum : 0.0;
+        o[c] = sw.gate[i] > 0 ? sum : 0.0;
         if (c == 0) o[3] = 0.0;
     }
 }
@@ -892,7 +892,7 @@ __global__ __launch_bounds__(64) void k_split_chain(const double* __restrict__ T
     if (lane == 0) {
         const int i = sw.sink0 + row / 3, c = row % 3;
         double* o = reinterpret_cast<double*>(a + i);
-        o[c] = id[i] > 0 ? sum : 0.0;
+        o[c] = sw.gate[i] > 0 ? sum : 0.0;
         if (c == 0) o[3] = 0.0;
     }
 }
@@ -930,7 +930,7 @@ __global__ __launch_bounds__(256) void k_split_sum_fin(const double* __restrict_
     for (int k = 0; k < nseg; ++k) acc += seg[(size_t)row * nseg + k];
     const int i = sw.sink0 + row / 3, c = row % 3;
     double* o = reinterpret_cast<double*>(a + i);
-    o[c] = id[i] > 0 ? acc : 0.0;
+    o[c] = sw.gate[i] > 0 ? acc : 0.0;
     if (c == 0) o[3] = 0.0;
 }
 
@@ -1184,11 +1184,11 @@ __device__ __forceinline__ void fused_terms(const FusedSrc* q, const int* j, con
 
 // Stages sources [j0, j0 + nj) of a sweep in LDS (one global round trip for the tile).
 __device__ __forceinline__ void fused_stage_sources(FusedLds& L, int j0, int nj, const d4* r, const double* radius,
-                                                    const int* id)
+                                                    const int* gate)
 {
     for (int k = threadIdx.x; k < nj; k += FUSED_BLOCK) {
         const d4 q = r[j0 + k];
-        L.src[k] = FusedSrc{q.x, q.y, q.z, q.w, radius[j0 + k], id[j0 + k] >= 0 ? 1 : 0, 0};
+        L.src[k] = FusedSrc{q.x, q.y, q.z, q.w, radius[j0 + k], gate[j0 + k] >= 0 ? 1 : 0, 0};
     }
 }
 
@@ -1212,12 +1212,12 @@ __device__ void fused_sweep(const Sweep& sw, const d4* r, const d4* v, d4* a, co
             if (tid < S) {
                 const int i = sw.sink0 + g0 + tid;
                 const d4 p = r[i];
-                L.sink[tid] = FusedSink{p.x, p.y, p.z, EV ? radius[i] : 0.0, id[i] > 0 ? 1 : 0, 0};
+                L.sink[tid] = FusedSink{p.x, p.y, p.z, EV ? radius[i] : 0.0, sw.gate[i] > 0 ? 1 : 0, 0};
             }
             double sum = 0.0;   // lane tid < 3 S: row (sink g0 + tid / 3, component tid % 3)
             for (int j0 = 0; j0 < nsrc; j0 += J) {
                 const int nj = min(J, nsrc - j0);
-                fused_stage_sources(L, sw.src0 + j0, nj, r, radius, id);
+                fused_stage_sources(L, sw.src0 + j0, nj, r, radius, sw.gate);
                 __syncthreads();
                 constexpr int TU = 2;   // terms per work-item and pass
                 for (int k0 = 0; k0 < S * nj; k0 += TU * FUSED_BLOCK) {
@@ -1306,12 +1306,12 @@ __device__ void fused_sweep(const Sweep& sw, const d4* r, const d4* v, d4* a, co
         const bool in = i < sw.sink1;
         const int ic = in ? i : sw.sink0;
         const d4 p = r[ic];
-        const bool live = in && id[ic] > 0;
+        const bool live = in && sw.gate[ic] > 0;
         const double Ri = EV ? radius[ic] : 0.0;
         double ax = 0.0, ay = 0.0, az = 0.0;
         for (int j0 = sw.src0; j0 < sw.src1; j0 += FUSED_TILE) {
             const int nj = min(FUSED_TILE, sw.src1 - j0);
-            fused_stage_sources(L, j0, nj, r, radius, id);
+            fused_stage_sources(L, j0, nj, r, radius, sw.gate);
             __syncthreads();
             if (in) {
                 // four sources at a time, their terms added in ascending j

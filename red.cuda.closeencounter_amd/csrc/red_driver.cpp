@@ -86,7 +86,8 @@ void usage()
 {
     std::cout << "Usage: red <parameterlist>\n"
                  "     -id_dev | --id_active_device             : the id of the device which will execute the code\n"
-                 "     -ups    | --use-padded-storage           : accepted; storage is never padded here\n"
+                 "     -ups    | --use-padded-storage           : the padded kernel's rules: inactive bodies keep\n"
+                 "                                                pulling and being pulled until the next rebuild\n"
                  "     -n_tpb  | --n_thread-per-block           : threads per block of the force sweeps\n"
                  "     -iDir   | --inputDir <directory>         : the directory containing the input files\n"
                  "     -p      | --parameter <filename>         : the file containing the parameters of the simulation\n"
@@ -518,6 +519,7 @@ int main(int argc, const char** argv, const char** env)
         rp.encounter_hill_factor = P.hill;
         rp.dt_day = P.dt;
         rp.precision = opt.mixed ? RED_PREC_MIXED : RED_PREC_FP64;
+        rp.padded_storage = opt.ups ? 1 : 0;
         rp.device = opt.id_dev;
         rp.block = opt.n_tpb >= 64 ? opt.n_tpb : 0;
         int rc = red_ctx_create(&rb, &rp, &S.ctx);

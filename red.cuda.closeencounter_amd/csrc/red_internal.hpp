@@ -44,6 +44,10 @@ struct Ctl {
 // One force sweep: sinks [sink0, sink1) against sources [src0, src1).
 struct Sweep {
     int sink0, sink1, src0, src1;
+    // activity of sinks (> 0) and sources (>= 0): the body ids, or under padded storage an
+    // array of ones (pp_disk.cu:144-215 tests no id: inactive bodies keep pulling and being
+    // pulled until the next compaction); event records always carry the ids
+    const int* gate;
 };
 
 // Pair-test parameters shared by every force kernel.

@@ -59,7 +59,7 @@ class RedParams(ctypes.Structure):
                 ("use_graph", ctypes.c_int), ("kahan_pairs", ctypes.c_int), ("device", ctypes.c_int),
                 ("block", ctypes.c_int), ("sinks_per_thread", ctypes.c_int), ("strategy", ctypes.c_int),
                 ("fp64_split", ctypes.c_int), ("source_parts", ctypes.c_int), ("host_threads", ctypes.c_int),
-                ("reserved", ctypes.c_int * 1)]
+                ("padded_storage", ctypes.c_int)]
 
 
 class RedBodies(ctypes.Structure):
@@ -216,7 +216,8 @@ class Disk:
                  error_check_for_tp=False, close_encounter=False, inner_steps=True, frame_center=1,
                  hit_centrum=0.0, ejection=0.0, radii_enhance_factor=0.0, encounter_hill_factor=0.0,
                  precision=PREC_FP64, event_capacity=0, use_graph=False, kahan_pairs=False, device=0,
-                 block=0, sinks_per_thread=0, strategy=0, fp64_split=False, source_parts=0, r=None, v=None):
+                 block=0, sinks_per_thread=0, strategy=0, fp64_split=False, source_parts=0, padded_storage=False,
+                 r=None, v=None):
         self.L = load_library()
         self.ctx = ctypes.c_void_p()
         prm = RedParams()
@@ -242,6 +243,7 @@ class Disk:
         prm.strategy = strategy
         prm.fp64_split = int(fp64_split)
         prm.source_parts = source_parts
+        prm.padded_storage = int(padded_storage)
         self._keep = dict(
             id=np.ascontiguousarray(bodies.ids, dtype=np.int32),
             type=np.ascontiguousarray(bodies.types, dtype=np.int32),
@@ -492,7 +494,7 @@ class Disk:
 def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_check_for_tp=False,
             use_graph=True, device=0, block=0, frame_center=1, precision=PREC_FP64, radii_enhance_factor=0.0,
             encounter_hill_factor=0.0, close_encounter=False, ejection=0.0, hit_centrum=0.0, inner_steps=True,
-            event_capacity=0, host_threads=0):
+            event_capacity=0, host_threads=0, padded_storage=False):
     """Ensemble parameters; every option is passed on, so that the library can refuse the ones
     an ensemble does not support instead of their being dropped here."""
     prm = RedParams()
@@ -514,6 +516,7 @@ def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_ch
     prm.device = device
     prm.block = block
     prm.host_threads = host_threads
+    prm.padded_storage = int(padded_storage)
     return prm
 
 

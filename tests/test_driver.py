@@ -251,7 +251,7 @@ def test_driver_restart_from_snapshot(tmp_path, native_mod):
     assert np.allclose(res.v, full.v, rtol=0, atol=1e-13)
 
 
-def _oracle_loop(oracle_mod, body_path, prm, dt_day):
+def _oracle_loop(oracle_mod, body_path, prm, dt_day, padded=False):
     """The same run through the oracle and the Python restatement of the driver loop:
     snapshots (time, state) and the events handed to the event file, in order."""
     from rednative import mainloop
@@ -259,7 +259,7 @@ def _oracle_loop(oracle_mod, body_path, prm, dt_day):
     o = oracle_mod.Oracle(bi, integrator=prm["integrator"], adaptive=prm["adaptive"], tolerance=prm["tolerance"],
                           dt_day=dt_day, hit_centrum=prm["hit_centrum"], ejection=prm["ejection"],
                           radii_enhance_factor=prm["radii_enhance_factor"],
-                          encounter_hill_factor=prm["encounter_hill_factor"])
+                          encounter_hill_factor=prm["encounter_hill_factor"], padded=padded)
     o.t = prm["start_time"]
     snaps, events = [], []
     mainloop.run_main_loop(o, prm, lambda e: snaps.append((e.t, e.state())), lambda evs: events.extend(evs))
@@ -405,3 +405,37 @@ def test_driver_reference_rkf78_input(tmp_path, oracle_mod, native_mod):
         want = files.format_result_line(s0["id"][i], row[1], s0["type"][i], t0 / GAUSS, s0["p"][i], 0, 0.0,
                                         s0["r"][i], s0["v"][i] * GAUSS).split()
         assert row == want, (i, row, want)
+
+
+@pytest.mark.gpu
+def test_driver_padded_storage_matches_oracle(tmp_path, oracle_mod):
+    """-ups: the outer bodies, beyond the ejection distance, are ejected at once and keep
+    pulling and moving under the padded kernel's rules until the rebuild after the first step
+    (no collisions: two merged bodies parked at one point would meet, a 0/0); every event line
+    (all 50 columns) and the last snapshot equal the oracle's padded run, and differ from the
+    run without -ups."""
+    b = generate.disk_allsi(150, seed=17, a_range=(1.0, 1.05), m_range=(1e-10, 1e-8))
+    files.write_body_file(str(tmp_path / "disk.txt"), b)
+    (tmp_path / "p.txt").write_text("integrator = rk4\nlength = 0.2\noutput_interval = 0.1\n"
+                                    "ejection = 1.04\ndt = 0.5\n")
+    print("case driver padded storage", flush=True)
+    out = {}
+    for mode, extra in (("ups", ["-ups"]), ("plain", [])):
+        r = run_red(["-iDir", str(tmp_path), "-p", "p.txt", "-ic", "disk.txt", "-result", f"result_{mode}",
+                     "-event", f"event_{mode}", "-info", f"info_{mode}", "-log", f"log_{mode}"] + extra)
+        assert r.returncode == 0, r.stderr
+        out[mode] = ((tmp_path / f"result_{mode}.txt").read_text(), (tmp_path / f"event_{mode}.txt").read_text())
+    assert out["ups"][0] != out["plain"][0]
+    prm = files.read_parameters(str(tmp_path / "p.txt"))
+    _, snaps, events = _oracle_loop(oracle_mod, str(tmp_path / "disk.txt"), prm, prm["dt"], padded=True)
+    lines = [ln.split() for ln in out["ups"][1].splitlines() if ln.strip()]
+    assert len(lines) == len(events) > 0
+    assert sum(1 for e in events if e["name"] == 2) > 0   # ejections
+    for k, (line, e) in enumerate(zip(lines, events)):
+        assert line == files.format_event_line(e).split(), k
+    t, s = snaps[-1]
+    keep = s["id"] > 0
+    rows = [ln.split() for ln in out["ups"][0].splitlines() if ln.strip()][-int(keep.sum()):]
+    for row, i, rr in zip(rows, s["id"][keep], s["r"][keep]):
+        assert int(row[0]) == i
+        assert [float(x) for x in row[10:13]] == [float(f"{x:.16e}") for x in rr]

@@ -135,6 +135,8 @@ EVENT_CASES = {
                       radii_enhance_factor=2500.0),
     "rkf78_merge": dict(integrator=4, adaptive=True, tolerance=1e-10, dt_day=0.5, hit_centrum=0.3, ejection=6.0,
                         radii_enhance_factor=2500.0),
+    "rkf78_padded_hill": dict(integrator=4, adaptive=True, tolerance=1e-10, dt_day=0.5, ejection=6.0,
+                              radii_enhance_factor=1.0, encounter_hill_factor=3.0, padded_storage=True),
     "ck_hill": dict(integrator=3, adaptive=True, tolerance=1e-10, dt_day=0.5, radii_enhance_factor=1.0,
                     encounter_hill_factor=10.0),
     "rk4_ce_mode": dict(integrator=2, adaptive=False, dt_day=0.5, close_encounter=True, radii_enhance_factor=500.0),

@@ -193,3 +193,61 @@ def test_compaction_by_hand(oracle_mod):
     # nothing left to remove: a second pass keeps everything
     assert o.check_for_rebuild_vectors(0)
     assert list(o.state()["id"]) == COMPACTED_IDS
+
+
+# ---- padded storage (-ups, pp_disk.cu:144-215) -------------------------------------------
+
+def _padded_disk(x_pos, x_radius=1.0e-6, x_mass=3.0e-6, x_id=-3):
+    """Star, protoplanet P, inactive protoplanet X (its id negated, as after an ejection: mass
+    and radius kept), test particle T."""
+    types = np.array([0, 3, 3, 6], dtype=np.int32)
+    r = np.array([[0.0, 0, 0], [1.0, 0, 0], x_pos, [0.0, -1.0, 0]])
+    return files.BodySet([1, 0, 0, 2, 0, 0, 1], np.array([1, 2, x_id, 4], dtype=np.int32), ["s", "p", "x", "t"],
+                         types, np.zeros(4), np.array([1.0, 1.0e-6, x_mass, 0.0]),
+                         np.array([0.00465, 1.0e-6, x_radius, 0.0]), np.zeros(4), np.zeros(4),
+                         np.zeros(4, dtype=np.int32), np.zeros(4), r, np.zeros((4, 3)))
+
+
+def _pull(src_m, src_r, at):
+    d = np.asarray(src_r, float) - np.asarray(at, float)
+    return src_m * d / np.linalg.norm(d) ** 3
+
+
+def test_padded_inactive_body_keeps_pulling_and_being_pulled(oracle_mod):
+    """Under padded storage an inactive body's stored mass still attracts every sink and the
+    body itself is accelerated by the other sources; without it the body neither pulls nor
+    moves.  So padded - plain is X's pull on S, P and T, and X's own acceleration is the pull
+    of S and P (a dropped source or sink term, or a zeroed mass, fails)."""
+    b = _padded_disk([0.0, 1.5, 0.0])
+    a_plain = oracle_mod.Oracle(b).accel(0.0, b.r, b.v)
+    a_pad = oracle_mod.Oracle(b, padded=True).accel(0.0, b.r, b.v)
+    assert np.array_equal(a_plain[2], np.zeros(3))
+    for i in (0, 1, 3):
+        np.testing.assert_allclose(a_pad[i] - a_plain[i], _pull(3.0e-6, b.r[2], b.r[i]), rtol=1e-9, atol=1e-21)
+    np.testing.assert_allclose(a_pad[2], _pull(1.0, b.r[0], b.r[2]) + _pull(1.0e-6, b.r[1], b.r[2]), rtol=1e-14)
+
+
+def test_padded_inactive_body_meets_pair_tests(oracle_mod):
+    """The padded kernel tests pairs without ids: X 0.5e-6 AU from P (within 1 x (R_P + R_X))
+    is recorded as a collision, the heavier X the survivor with its negated id; the plain
+    kernel skips the inactive X."""
+    b = _padded_disk([1.0, 0.5e-6, 0.0])
+    o = oracle_mod.Oracle(b, radii_enhance_factor=1.0)
+    o.accel(0.0, b.r, b.v)
+    assert o.raw_events() == []
+    o = oracle_mod.Oracle(b, radii_enhance_factor=1.0, padded=True)
+    o.accel(0.0, b.r, b.v)
+    ev = o.raw_events()
+    assert [(e["name"], e["id1"], e["id2"], e["sink"], e["source"]) for e in ev] == [(EV_COLLISION, -3, 2, 2, 1)]
+    assert ev[0]["d"] == 0.5e-6
+
+
+def test_padded_merged_body_adds_nothing(oracle_mod):
+    """A merged body (massless, zero radius, parked at (1e20, 0, 0)) changes no active sink's
+    acceleration in either mode, bit for bit: zero-mass terms add exact zeros.  This is why
+    the massless padding particles of the padded layout need not exist here."""
+    b = _padded_disk([1.0e20, 0.0, 0.0], x_radius=0.0, x_mass=0.0)
+    a_plain = oracle_mod.Oracle(b).accel(0.0, b.r, b.v)
+    a_pad = oracle_mod.Oracle(b, padded=True).accel(0.0, b.r, b.v)
+    for i in (0, 1, 3):
+        assert np.array_equal(a_pad[i], a_plain[i]), i
