@@ -1137,6 +1137,7 @@ struct FusedLds {
     FusedSrc src[FUSED_TILE];
     FusedSink sink[FUSED_TILE_SINKS];
     int flag;
+    int eject[2];   // small disks: the ejection predicate of the next step, by step parity
 };
 
 // U pair terms of one work-item in the reference's operation order (as fp64_pair), from
@@ -1352,8 +1353,9 @@ __device__ void fused_sweep(const Sweep& sw, const d4* r, const d4* v, d4* a, co
 }
 
 // The three sweeps of one force evaluation (as eval_forces launches them).
-__device__ void fused_forces(const FusedArgs& A, const PairTest& P, int stage, const d4* r, const d4* v, d4* a,
-                             FusedLds& L)
+// (inlined: a call would copy the kernel's argument block to scratch to pass it by reference)
+__device__ __forceinline__ void fused_forces(const FusedArgs& A, const PairTest& P, int stage, const d4* r,
+                                             const d4* v, d4* a, FusedLds& L)
 {
     EventSink E;
     E.buf = A.ev_buf;
@@ -1374,9 +1376,170 @@ __device__ void fused_forces(const FusedArgs& A, const PairTest& P, int stage, c
     }
 }
 
-// One body of a stage combination (k_stage's work for body i).
+// ---- small disks: every massive source in one LDS tile ---------------------------------
+// The pass that forms a stage state also stages it (position, mass, radius, activity) as the
+// next force evaluation's sources, so an evaluation needs no staging pass and no barrier of
+// its own before the pair terms; the SI sweep's serial sums (wave 0) run beside the other
+// sinks' sums (waves 1-3).  Terms and sums are the same operations in the same order as the
+// tiled sweeps above, so the results are identical.
+
+struct SmallPlan {
+    int nsi, nm, n;
+    bool few;       // few SI sinks: their pair terms are formed by the whole work-group
+};
+
+// FusedSrc.pad carries the sink activity (gate > 0) of a staged massive body.
+__device__ __forceinline__ void small_stage_src(FusedLds& L, int i, const d4& q, const double* radius,
+                                                const int* gate)
+{
+    const int g = gate[i];
+    L.src[i] = FusedSrc{q.x, q.y, q.z, q.w, radius[i], g >= 0 ? 1 : 0, g > 0 ? 1 : 0};
+}
+
+// One sink against staged sources [0, nsrc): four at a time, added in ascending j.
+template <bool OV, bool EV>
+__device__ __forceinline__ void small_sink(const FusedArgs& A, const PairTest& P, const EventSink& E, int i,
+                                           int nsrc, const d4* r, const d4* v, d4* a, const FusedLds& L, int nm)
+{
+    // a massive sink's record is staged among the sources
+    const bool staged = i < nm;
+    const d4 p = staged ? d4{L.src[i].x, L.src[i].y, L.src[i].z, L.src[i].m} : r[i];
+    const bool live = staged ? L.src[i].pad != 0 : A.sw[0].gate[i] > 0;
+    const double Ri = EV ? (staged ? L.src[i].R : A.radius[i]) : 0.0;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    constexpr int TU = 4;
+    for (int k0 = 0; k0 < nsrc; k0 += TU) {
+        FusedSrc q[TU];
+        int jg[TU], ig[TU];
+        double px[TU], py[TU], pz[TU], tx[TU], ty[TU], tz[TU], d[TU];
+        bool val[TU], use[TU];
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            val[u] = k0 + u < nsrc;
+            q[u] = L.src[val[u] ? k0 + u : k0];
+            jg[u] = k0 + u;
+            ig[u] = i;
+            px[u] = p.x; py[u] = p.y; pz[u] = p.z;
+        }
+        fused_terms<TU, OV>(q, jg, ig, px, py, pz, val, tx, ty, tz, d, use);
+#pragma unroll
+        for (int u = 0; u < TU; ++u) {
+            if (val[u]) {
+                ax += tx[u];
+                ay += ty[u];
+                az += tz[u];
+            }
+        }
+        if (EV && live) {
+#pragma unroll
+            for (int u = 0; u < TU; ++u)
+                if (use[u]) pair_test(P, E, A.ctl, i, jg[u], d[u], Ri, q[u].R, r, v, A.mass, A.id);
+        }
+    }
+    a[i] = d4{live ? ax : 0.0, live ? ay : 0.0, live ? az : 0.0, 0.0};
+}
+
+// One force evaluation of a small disk whose massive sources [0, nm) are staged in L.src;
+// ends with a barrier.
+template <bool EV>
+__device__ __forceinline__ void small_forces(const FusedArgs& A, const PairTest& P, const SmallPlan& SP, int stage,
+                                             const d4* r, const d4* v, d4* a, FusedLds& L)
+{
+    EventSink E;
+    E.buf = A.ev_buf;
+    E.cap = A.ev_cap;
+    E.stage = stage;
+    E.c_stage = A.c_stage[stage];
+    const int tid = threadIdx.x;
+    int first = 0, lane0 = 0, stride = FUSED_BLOCK;   // sinks [first, n) one per work-item
+    if (SP.few) {
+        // the SI sweep's pair terms, S sinks x nm sources, by every work-item
+        const int S = SP.nsi, nj = SP.nm;
+        constexpr int TU = 2;
+        for (int k0 = 0; k0 < S * nj; k0 += TU * FUSED_BLOCK) {
+            FusedSrc q[TU];
+            int jg[TU], ig[TU], kk[TU];
+            double px[TU], py[TU], pz[TU], tx[TU], ty[TU], tz[TU], d[TU];
+            bool val[TU], use[TU];
+#pragma unroll
+            for (int u = 0; u < TU; ++u) {
+                kk[u] = k0 + u * FUSED_BLOCK + tid;
+                val[u] = kk[u] < S * nj;
+                const int kc = val[u] ? kk[u] : 0;
+                ig[u] = kc / nj;
+                jg[u] = kc - ig[u] * nj;
+                q[u] = L.src[jg[u]];
+                px[u] = L.src[ig[u]].x; py[u] = L.src[ig[u]].y; pz[u] = L.src[ig[u]].z;
+            }
+            fused_terms<TU, true>(q, jg, ig, px, py, pz, val, tx, ty, tz, d, use);
+#pragma unroll
+            for (int u = 0; u < TU; ++u) {
+                if (val[u]) {
+                    double* t3 = L.terms + 3 * kk[u];
+                    t3[0] = tx[u]; t3[1] = ty[u]; t3[2] = tz[u];
+                }
+            }
+            if (EV) {
+#pragma unroll
+                for (int u = 0; u < TU; ++u)
+                    if (use[u] && L.src[ig[u]].pad)
+                        pair_test(P, E, A.ctl, ig[u], jg[u], d[u], L.src[ig[u]].R, q[u].R, r, v, A.mass, A.id);
+            }
+        }
+        __syncthreads();
+        first = S; lane0 = 64; stride = FUSED_BLOCK - 64;
+        if (tid < 3 * S) {
+            // row (sink tid / 3, component tid % 3) in ascending j, the next 16 terms read
+            // while the current 16 are added
+            const int sk = tid / 3;
+            const double* t3 = L.terms + 3 * (sk * nj) + (tid - 3 * sk);
+            constexpr int GQ = 16;
+            const int ng = nj / GQ;
+            double sum = 0.0, x[GQ], y[GQ];
+            if (ng > 0) {
+#pragma unroll
+                for (int k = 0; k < GQ; ++k) x[k] = t3[3 * k];
+            }
+            for (int g = 0; g < ng; g += 2) {
+                if (g + 1 < ng) {
+#pragma unroll
+                    for (int k = 0; k < GQ; ++k) y[k] = t3[3 * ((g + 1) * GQ + k)];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int k = 0; k < GQ; ++k) sum += x[k];
+                __builtin_amdgcn_sched_barrier(0);
+                if (g + 1 < ng) {
+                    if (g + 2 < ng) {
+#pragma unroll
+                        for (int k = 0; k < GQ; ++k) x[k] = t3[3 * ((g + 2) * GQ + k)];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int k = 0; k < GQ; ++k) sum += y[k];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            for (int jj = ng * GQ; jj < nj; ++jj) sum += t3[3 * jj];
+            double* o = reinterpret_cast<double*>(a + sk);
+            const int comp = tid - 3 * sk;
+            o[comp] = L.src[sk].pad ? sum : 0.0;
+            if (comp == 0) o[3] = 0.0;
+        }
+    }
+    if (tid >= lane0) {
+        for (int i = first + tid - lane0; i < SP.n; i += stride) {
+            if (i < SP.nsi) small_sink<true, EV>(A, P, E, i, SP.nm, r, v, a, L, SP.nm);        // SI x massive
+            else if (i < SP.nm) small_sink<false, EV>(A, P, E, i, SP.nsi, r, v, a, L, SP.nm);  // NSI x SI
+            else small_sink<false, EV>(A, P, E, i, SP.nm, r, v, a, L, SP.nm);                  // NI x massive
+        }
+    }
+    __syncthreads();
+}
+
+// One body of a stage combination (k_stage's work for body i); returns its position record.
 template <int FORM>
-__device__ __forceinline__ void fused_stage_body(const StageArgs& S, int i, double dt)
+__device__ __forceinline__ d4 fused_stage_body(const StageArgs& S, int i, double dt)
 {
     const d4 yr = S.yr[i];
     const d4 yv = S.yv[i];
@@ -1386,6 +1549,27 @@ __device__ __forceinline__ void fused_stage_body(const StageArgs& S, int i, doub
     ovv.w = 0.0;
     S.outr[i] = orr;
     S.outv[i] = ovv;
+    return orr;
+}
+
+// A stage combination of a small disk that also stages the massive bodies' new states as
+// the next evaluation's sources; with eject, the ejection predicate of the new states (the
+// next step's check at its start) is raised in *eject.
+__device__ __forceinline__ void small_stage(int form, const StageArgs& S, int n, int nm, double dt, FusedLds& L,
+                                            const FusedArgs& A, int* eject)
+{
+    for (int i = threadIdx.x; i < n; i += FUSED_BLOCK) {
+        d4 q;
+        switch (form) {
+        case SF_AXPY: q = fused_stage_body<SF_AXPY>(S, i, dt); break;
+        case SF_DTSUM: q = fused_stage_body<SF_DTSUM>(S, i, dt); break;
+        case SF_ACCUM: q = fused_stage_body<SF_ACCUM>(S, i, dt); break;
+        case SF_RK4NP1: q = fused_stage_body<SF_RK4NP1>(S, i, dt); break;
+        default: q = fused_stage_body<SF_RKF78NP1>(S, i, dt); break;
+        }
+        if (i < nm) small_stage_src(L, i, q, A.radius, A.sw[0].gate);
+        if (eject && A.id[i] > 0 && A.type[i] != 0 && ejection_name(q, A.ej, A.hc) != RED_EV_NONE) *eject = 1;
+    }
 }
 
 __device__ void fused_stage(int form, const StageArgs& S, int n, double dt)
@@ -1421,6 +1605,74 @@ __global__ __launch_bounds__(FUSED_BLOCK) void k_fused_fixed(FusedArgs A, PairTe
     double t = A.t, ps = A.ps;
     int cur = A.cur, steps = 0, reason = 0;
     const bool scan = 0.0 < A.ej || 0.0 < A.hc;
+    SmallPlan SP;
+    SP.nsi = A.sw[0].sink1;
+    SP.nm = A.sw[0].src1;
+    SP.n = A.n;
+    SP.few = SP.nsi > 0 && SP.nsi <= FUSED_TILE_SINKS && SP.nm > 64 && SP.nsi * SP.nm <= FUSED_TILE;
+    // the sweeps as the small path assumes them (barycentric bounds of number_of_bodies.cpp)
+    const bool small = SP.nm <= FUSED_TILE && A.sw[0].sink0 == 0 && A.sw[0].src0 == 0 &&
+                       A.sw[1].sink0 == SP.nsi && A.sw[1].sink1 == SP.nm && A.sw[1].src0 == 0 &&
+                       A.sw[1].src1 == SP.nsi && A.sw[2].sink0 == SP.nm && A.sw[2].sink1 == SP.n &&
+                       A.sw[2].src0 == 0 && A.sw[2].src1 == SP.nm;
+    if (small) {
+        if (tid == 0) {
+            L.eject[0] = 0;
+            L.eject[1] = 0;
+            A.ctl->t = t;
+            A.ctl->dt = A.dt;
+            A.ctl->attempt = 0;
+            A.ctl->err_bits = 0ull;
+            A.ctl->nan_flag = 0;
+        }
+        __syncthreads();
+        // y_n staged as the first evaluation's sources, and the first step's ejection check
+        for (int i = tid; i < A.n; i += FUSED_BLOCK) {
+            const d4 q = A.r[cur][i];
+            if (i < SP.nm) small_stage_src(L, i, q, A.radius, A.sw[0].gate);
+            if (scan && A.id[i] > 0 && A.type[i] != 0 && ejection_name(q, A.ej, A.hc) != RED_EV_NONE) L.eject[0] = 1;
+        }
+        __syncthreads();
+        for (;;) {
+            if (!(t <= A.t_stop)) { reason = RED_FUSED_STOP_TIME; break; }
+            if (steps >= A.max_steps) { reason = RED_FUSED_STOP_STEPS; break; }
+            if (A.interval <= fabs(ps)) { reason = RED_FUSED_STOP_OUTPUT; break; }
+            if (L.eject[steps & 1] != 0) { reason = RED_FUSED_STOP_EJECTION; break; }
+            const StageArgs* rows = rows_l + cur * A.n_main;
+            int nev = 0;
+            // evaluation k - 1 (f0 from y_n, then each stage state), then stage state k
+            for (int k = 1; k <= A.n_main; ++k) {
+                const d4* fr = k == 1 ? A.r[cur] : rows[k - 2].outr;
+                const d4* fv = k == 1 ? A.v[cur] : rows[k - 2].outv;
+                if (P.enabled) small_forces<true>(A, P, SP, k - 1, fr, fv, A.a[k - 1], L);
+                else small_forces<false>(A, P, SP, k - 1, fr, fv, A.a[k - 1], L);
+                if (k == 1 && A.clear_events_after_f0 && tid == 0)
+                    __hip_atomic_store(&A.ctl->n_events, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const StageArgs& S = rows[k - 1];
+                const bool last = k == A.n_main;
+                if (last && tid == 0) {
+                    // every pair test of the step is behind the last barrier (with one force
+                    // evaluation, f0's records were cleared); the count reaches every
+                    // work-item through LDS behind this pass's barrier
+                    L.flag = (A.clear_events_after_f0 && A.n_main == 1)
+                                 ? 0
+                                 : __hip_atomic_load(&A.ctl->n_events, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    L.eject[steps & 1] = 0;   // read at this step's start, behind barriers since
+                    A.ctl->t = t + A.dt;      // the next step's event times
+                }
+                small_stage(A.form[k - 1], S, A.n, SP.nm, A.dt, L, A, last && scan ? &L.eject[(steps + 1) & 1] : nullptr);
+                __syncthreads();
+                if (last) nev = L.flag;
+            }
+            t = t + A.dt;
+            ps = ps + fabs(A.dt);
+            ++steps;
+            cur = 1 - cur;
+            if (nev > 0) { reason = RED_FUSED_STOP_EVENTS; break; }
+        }
+        if (tid == 0) *A.out = FusedOut{t, ps, steps, cur, reason, 0};
+        return;
+    }
     for (;;) {
         if (!(t <= A.t_stop)) { reason = RED_FUSED_STOP_TIME; break; }
         if (steps >= A.max_steps) { reason = RED_FUSED_STOP_STEPS; break; }
