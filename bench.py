@@ -476,7 +476,9 @@ def ensemble_section(native, days, sample, use_graph=True):
     prm = dict(stop_time=t_stop, output_interval=1e300, inner_steps=True, **rules)
     t1 = time.perf_counter()
     steps_one = 0
-    for k in range(sample):
+    # a sample spread evenly over the ensemble (disk sizes are drawn in random order)
+    picks = sorted({round(i * (len(disks) - 1) / max(sample - 1, 1)) for i in range(sample)})
+    for k in picks:
         d = native.Disk(disks[k], use_graph=use_graph, **kw)
         mainloop.run_main_loop(d, prm, lambda e: None)
         d.sync()
@@ -489,7 +491,9 @@ def ensemble_section(native, days, sample, use_graph=True):
                         f"RKF7(8) tol 1e-11, collisions 1x(R_i+R_j) merged, close encounters at 3 mutual Hill radii, "
                         f"ejection 100 AU, {days} days each",
             "ensemble_disk_steps_per_s": rate_ens, "per_disk_disk_steps_per_s": rate_one,
-            "per_disk_sample": sample, "speedup": rate_ens / rate_one, "supersteps": info["supersteps"],
+            "ensemble_bodies": sum(d.n for d in disks),
+            "per_disk_sample": len(picks), "per_disk_sample_bodies": sum(disks[k].n for k in picks),
+            "speedup": rate_ens / rate_one, "supersteps": info["supersteps"],
             "ms_per_superstep": 1e3 * t_ens / max(info["supersteps"], 1), "seconds": t_ens, "events": ev,
             "host_event_seconds": info["event_seconds"], "host_rebuild_seconds": info["rebuild_seconds"],
             "autotuned_block": block, "autotuned_jtile": jtile}

@@ -1006,10 +1006,12 @@ int sweep_parts(const Sweep& sw, const GravConfig& C, size_t partial_elems)
     if (sequential) return 1;
     int parts = C.source_parts;
     if (parts <= 0) {
-        // aim for about eight wavefronts per SIMD (256 CUs x 4 SIMDs)
+        // aim for about 24 wavefronts per SIMD (256 CUs x 4 SIMDs), three rounds of resident
+        // waves: measured on the 65533 x 65539 test-particle sweep (mixed, 2 sinks per
+        // work-item), 16 parts 1.70 ms, 32 parts 1.60, 48 parts 1.57, 64 parts 1.57, 96 1.58
         const int spt = C.spt > 0 ? C.spt : 1;
         const int waves = (nsink + 64 * spt - 1) / (64 * spt);
-        parts = (8 * 1024 + waves - 1) / waves;
+        parts = (24 * 1024 + waves - 1) / waves;
         parts = std::min(parts, std::max(1, nsrc / 512));
     }
     parts = std::max(1, std::min(parts, 64));
