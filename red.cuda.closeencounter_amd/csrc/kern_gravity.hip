@@ -162,14 +162,19 @@ __device__ __forceinline__ void part_range(int j0, int j1, int parts, int p, int
     hi = min(j1, lo + len);
 }
 
-// Writes a sink's sum: straight to a (masked by activity) or to its part slot.
+// Writes a sink's sum: straight to a (masked by activity) or to its part slot.  Part sums
+// are three planes (x, y, z) of FP64 per part, 24 bytes per sink and part.
 __device__ __forceinline__ void store_acc(d4* a, d4* partial, int parts, int p, const Sweep& sw, int i, bool live,
                                           double x, double y, double z)
 {
     if (parts == 1) {
         a[i] = d4{live ? x : 0.0, live ? y : 0.0, live ? z : 0.0, 0.0};
     } else {
-        partial[(size_t)p * (sw.sink1 - sw.sink0) + (i - sw.sink0)] = d4{x, y, z, 0.0};
+        const size_t nsink = (size_t)(sw.sink1 - sw.sink0);
+        double* plane = reinterpret_cast<double*>(partial) + 3 * (size_t)p * nsink + (i - sw.sink0);
+        plane[0] = x;
+        plane[nsink] = y;
+        plane[2 * nsink] = z;
     }
 }
 
@@ -756,9 +761,10 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const d4* __restrict__ par
     const int nsink = sw.sink1 - sw.sink0;
     if (k >= nsink) return;
     double x = 0.0, y = 0.0, z = 0.0;
+    const double* planes = reinterpret_cast<const double*>(partial);
     for (int p = 0; p < parts; p++) {
-        const d4 q = partial[(size_t)p * nsink + k];
-        x += q.x; y += q.y; z += q.z;
+        const double* q = planes + 3 * (size_t)p * nsink + k;
+        x += q[0]; y += q[nsink]; z += q[2 * (size_t)nsink];
     }
     const int i = sw.sink0 + k;
     const bool live = sw.gate[i] > 0;
