@@ -43,34 +43,47 @@ def pair_counts(counts):
     return si + nsi + ni, dict(si=si, nsi=nsi, ni=ni)
 
 
-def run_path(native, bodies, precision, steps, warmup, kahan=False, tol=1e-10):
-    d = native.Disk(bodies, integrator=3, adaptive=True, tolerance=tol, dt_day=1.0, precision=precision,
-                    radii_enhance_factor=1.0, ejection=100.0, kahan_pairs=kahan, use_graph=False)
-    for _ in range(warmup):
-        d.step()
-        d.check_for_ejection_hit_centrum()
-        d.check_for_collision()
-        d.clear_event_counter()
-    E0, _ = d.energy()
-    d.sync()
-    d.set_profiling(True)
-    d.profile(reset=True)
+def _driver_steps(d, steps):
+    """`steps` iterations of the driver loop (ejection check, step, collision check)."""
     tries = 0
     sim_t = 0.0
-    t0 = time.perf_counter()
     for _ in range(steps):
         sim_t += d.step()
         tries += d.last_step.tries
         d.check_for_ejection_hit_centrum()
         d.check_for_collision()
         d.clear_event_counter()
+    return sim_t, tries
+
+
+def run_path(native, bodies, precision, steps, warmup, kahan=False, tol=1e-10, use_graph=True):
+    """The timed steps replay each attempt from its captured HIP graph with no timing events
+    (events around every launch cost 1-2 % of a step, and graphs cannot carry them); the
+    per-launch-class durations come from a second pass over the same number of steps, on a
+    disk stepped without graphs with HIP events on the context streams around every launch."""
+    kw = dict(integrator=3, adaptive=True, tolerance=tol, dt_day=1.0, precision=precision,
+              radii_enhance_factor=1.0, ejection=100.0, kahan_pairs=kahan)
+    d = native.Disk(bodies, use_graph=use_graph, **kw)
+    _driver_steps(d, warmup)
+    E0, _ = d.energy()
+    d.sync()
+    t0 = time.perf_counter()
+    sim_t, tries = _driver_steps(d, steps)
     d.sync()
     t1 = time.perf_counter()
-    prof = d.profile(reset=True)
-    d.set_profiling(False)
     E1, _ = d.energy()
-    return dict(seconds=t1 - t0, tries=tries, sim_t=sim_t, E0=E0, E1=E1, prof=prof,
-                counts=d.counts(), t_end=d.t)
+    res = dict(seconds=t1 - t0, tries=tries, sim_t=sim_t, E0=E0, E1=E1, counts=d.counts(), t_end=d.t)
+    d.close()
+    p = native.Disk(bodies, use_graph=False, **kw)
+    _driver_steps(p, 1)
+    p.sync()
+    p.set_profiling(True)
+    p.profile(reset=True)
+    _driver_steps(p, steps)
+    p.sync()
+    res["prof"] = p.profile(reset=True)
+    p.close()
+    return res
 
 
 def reference_workload_section(native, reps=20):
@@ -556,7 +569,7 @@ def load_drift(path):
         return None
 
 
-def bound_tolerance_section(native, bodies, steps_mixed, steps_fp64, drift_path, tol=1e-11):
+def bound_tolerance_section(native, bodies, steps_mixed, steps_fp64, drift_path, tol=1e-11, use_graph=True):
     """Both paths on the same disk at the tolerance whose energy drift meets dE/E <= 1e-9
     over 1000 yr (Cash-Karp's truncation drift is the same in both precisions; at 1e-10 it
     extrapolates to 3.4e-9): simulated years per hour, their ratio, and the mixed path's
@@ -564,7 +577,7 @@ def bound_tolerance_section(native, bodies, steps_mixed, steps_fp64, drift_path,
     GAUSS_YR = GAUSS * 365.25
     res = {"tolerance": tol}
     for name, prec, steps in (("mixed", native.PREC_MIXED, steps_mixed), ("fp64", native.PREC_FP64, steps_fp64)):
-        r = run_path(native, bodies, prec, steps, 2, tol=tol)
+        r = run_path(native, bodies, prec, steps, 2, tol=tol, use_graph=use_graph)
         yrs = r["sim_t"] / GAUSS_YR
         res[name] = {"sim_yr_per_hr": yrs / (r["seconds"] / 3600.0), "ms_per_step": 1e3 * r["seconds"] / steps,
                      "steps": steps, "attempts": r["tries"], "sim_days": r["sim_t"] / GAUSS,
@@ -623,7 +636,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-days", type=float, default=100.0)
     ap.add_argument("--no-graph", action="store_true",
-                    help="proposer, ensemble and small-disk sections without HIP graphs (for kernel traces)")
+                    help="every section without HIP graphs (for kernel traces and counters)")
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the mixed-vs-FP64 accuracy and reference-workload sections (for kernel traces "
                          "of the main sections alone)")
@@ -678,9 +691,10 @@ def main():
     bodies = generate.to_internal(generate.disk_mixed(n_pl=65536, n_tp=65533, seed=303))
     pairs_eval, parts = pair_counts(bodies.counts)
 
-    mixed = run_path(native, bodies, native.PREC_MIXED, args.steps, args.warmup, kahan=args.kahan)
+    graphs = not args.no_graph
+    mixed = run_path(native, bodies, native.PREC_MIXED, args.steps, args.warmup, kahan=args.kahan, use_graph=graphs)
     fp64 = None if args.no_fp64 else run_path(native, bodies, native.PREC_FP64, args.fp64_steps,
-                                                   max(1, args.warmup // 2))
+                                                   max(1, args.warmup // 2), use_graph=graphs)
 
     def summarize(r, steps):
         evals = steps + 5 * r["tries"]            # f0 once per step, 5 stages per attempt
@@ -705,11 +719,12 @@ def main():
     acc = None if args.no_fp64 or args.no_aux else accuracy_section(native, bodies)
     refw = None if args.no_fp64 or args.no_aux else reference_workload_section(native)
     cpu = None if args.no_cpu else cpu_baseline(bodies)
-    prop = None if args.no_proposer else proposer_section(native, args.proposer_years)
-    ens = None if args.no_ensemble else ensemble_section(native, args.ensemble_days, 16)
-    small = None if args.no_aux else small_disk_section(native, with_cpu=not args.no_cpu)
+    prop = None if args.no_proposer else proposer_section(native, args.proposer_years, graphs)
+    ens = None if args.no_ensemble else ensemble_section(native, args.ensemble_days, 16, graphs)
+    small = None if args.no_aux else small_disk_section(native, with_cpu=not args.no_cpu, use_graph=graphs)
     bound = None if args.no_bound or args.no_fp64 else bound_tolerance_section(
-        native, bodies, max(10, args.steps // 3), max(4, args.fp64_steps // 3), args.drift_summary)
+        native, bodies, max(10, args.steps // 3), max(4, args.fp64_steps // 3), args.drift_summary,
+        use_graph=graphs)
 
     line = {
         "metric": "pair-interactions/s & sim-yr/hr at N=131072, mixed vs FP64, ΔE/E≤1e-9",
@@ -745,7 +760,10 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "traffic_source": None if not traffic_src else traffic_src.get("source"),
                      "launch_ms": avg_ni_s * 1e3, "pairs_per_launch": ni_pairs,
-                     "flop_per_pair": FLOP_PER_PAIR},
+                     "flop_per_pair": FLOP_PER_PAIR,
+                     "duration_source": "HIP events on the context stream around every launch, in a "
+                                        "second pass over the same number of steps without graphs "
+                                        "(the timed pass replays graphs with no events)"},
         "roofline_hbm": hbm_rooflines(mixed, bodies, args.hbm_summary),
         "at_bound_tolerance": bound,
         "profile_ms": {k: v[0] for k, v in mixed["prof"].items() if v[1]},
