@@ -592,6 +592,53 @@ def bound_tolerance_section(native, bodies, steps_mixed, steps_fp64, drift_path,
     return res
 
 
+def ensemble_span_section(native, years, chunk_days=100.0, use_graph=True):
+    """The 512-disk workload of ensemble_section over its full span: advanced in chunks, each
+    followed by draining every disk's event records into a reusable buffer (as a caller
+    writing the event file would), so host memory stays bounded.  Reports the throughput of
+    the advance calls alone and with the drains."""
+    from rednative import generate
+    disks = [generate.to_internal(d) for d in generate.ensemble(n_disks=512, seed=404)]
+    kw = dict(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, ejection=100.0,
+              radii_enhance_factor=1.0, encounter_hill_factor=3.0)
+    E = native.Ensemble(disks, use_graph=use_graph, **kw)
+    block, jtile, _ = E.autotune(reps=3)
+    buf = (native.RedEvent * (1 << 16))()
+    t_adv = t_drain = 0.0
+    supersteps = 0
+    records = 0
+    t_stop_days = 365.25 * years
+    day = 0.0
+    while day < t_stop_days:
+        day = min(t_stop_days, day + chunk_days)
+        t0 = time.perf_counter()
+        info = E.advance(day * GAUSS)
+        t1 = time.perf_counter()
+        for k in range(len(disks)):
+            while True:
+                got, left = E.drain_events(k, buf)
+                records += got
+                if got >= left:
+                    break
+        t2 = time.perf_counter()
+        t_adv += t1 - t0
+        t_drain += t2 - t1
+        supersteps += info["supersteps"]
+        print(f"ensemble span: {day / 365.25:.2f} yr, {supersteps} supersteps, {records} records, "
+              f"{t_adv:.1f} s advancing, {t_drain:.1f} s draining", file=sys.stderr, flush=True)
+    cnt = [E.counters(k) for k in range(len(disks))]
+    steps = sum(c["passed"] for c in cnt)
+    ev = {k: sum(c[k] for c in cnt) for k in ("n_collision", "n_close_encounter", "n_ejection", "n_hit_centrum")}
+    E.close()
+    return {"workload": f"512 disks, N log-uniform in [64, 4096] (seed 404), each star + 2 giants + planetesimals, "
+                        f"RKF7(8) tol 1e-11, collisions 1x(R_i+R_j) merged, close encounters at 3 mutual Hill "
+                        f"radii, ejection 100 AU, {years} yr each, event records drained every {chunk_days} days",
+            "disk_steps": steps, "supersteps": supersteps, "seconds_advance": t_adv, "seconds_drain": t_drain,
+            "ensemble_disk_steps_per_s": steps / t_adv, "ensemble_disk_steps_per_s_with_drain": steps / (t_adv + t_drain),
+            "ms_per_superstep": 1e3 * t_adv / max(supersteps, 1), "records_drained": records, "events": ev,
+            "autotuned_block": block, "autotuned_jtile": jtile}
+
+
 def copy_calibration(mib):
     """One device-to-device copy of `mib` MiB (the runtime's copy kernel): the reference
     point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -640,7 +687,9 @@ def main():
     ap.add_argument("--no-aux", action="store_true",
                     help="skip the mixed-vs-FP64 accuracy and reference-workload sections (for kernel traces "
                          "of the main sections alone)")
-    ap.add_argument("--only", choices=["proposer", "ensemble", "drift", "smalldisk"], default=None,
+    ap.add_argument("--ensemble-years", type=float, default=100.0,
+                    help="--only ensemble-span: years per disk (the ensemble workload's full span is 100)")
+    ap.add_argument("--only", choices=["proposer", "ensemble", "ensemble-span", "drift", "smalldisk"], default=None,
                     help="run one auxiliary section alone and print its JSON (drift: the 16384-body "
                          "disk's energy drift over --drift-years, not part of the default run)")
     ap.add_argument("--fused-only", action="store_true",
@@ -675,6 +724,10 @@ def main():
         return
     if args.only == "ensemble":
         print(json.dumps({"ensemble": ensemble_section(native, args.ensemble_days, 16, not args.no_graph)}))
+        return
+    if args.only == "ensemble-span":
+        print(json.dumps({"ensemble_span": ensemble_span_section(native, args.ensemble_years,
+                                                                 use_graph=not args.no_graph)}))
         return
     if args.only == "smalldisk":
         print(json.dumps({"small_disk": small_disk_section(native, with_cpu=not args.no_cpu,

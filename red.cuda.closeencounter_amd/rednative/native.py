@@ -603,6 +603,13 @@ class Ensemble:
                 "n_close_encounter", "n_event_last_clear")
         return dict(zip(keys, (int(x) for x in c)))
 
+    def drain_events(self, disk, buf):
+        """Moves the disk's handled records (oldest first) into `buf`, a RedEvent array, as
+        many as fit; returns (copied, left before the call)."""
+        n = ctypes.c_int()
+        self._chk(self.L.red_ensemble_disk_events(self.h, disk, buf, len(buf), ctypes.byref(n)))
+        return min(n.value, len(buf)), n.value
+
     def events(self, disk):
         """Records of the event passes the disk handled since the last call, in order."""
         n = ctypes.c_int()
