@@ -39,6 +39,7 @@
 #include "host_tools.hpp"
 #include "red_internal.hpp"
 #include "scheme.hpp"
+#include "stage_forms.hpp"
 
 using namespace red;
 
@@ -153,22 +154,74 @@ __device__ void ens_pair_test(const EnsPair& P, int disk, int i, int j, double d
     for (int c = 0; c < 6; c++) { e->oe1[c] = 0.0; e->oe2[c] = 0.0; e->oes[c] = 0.0; }
 }
 
-// stage 0 runs only for disks starting a new step (bit 1), the other stages for every
-// disk in the attempt (bit 0).  Inactive sources add +0, inactive sinks get a = 0, as in the
-// single-disk sweeps.
+// The stage combinations that follow a force evaluation, formed per sink in its epilogue:
+// each needs only its own body's records, so the launch that finishes a sink's acceleration
+// forms its next stage state (and y_(n+1) after the last main stage), with k_stage's
+// arithmetic (combine_get), instead of a stage launch over all bodies.
+struct EnsEpi {
+    const StageArgs* rows;   // device: [0] y_(n+1), [r] stage state r
+    int next, next_form;     // stage state formed after this evaluation (0: none)
+    int np1, np1_form;       // y_(n+1) formed after it (np1 != 0)
+    const d4* cur_a;         // this evaluation's accelerations, whose values are in registers
+};
+
+// One row for body i; the term that is this evaluation's acceleration comes from `cur`.
+template <int FORM>
+__device__ __forceinline__ void epi_row_f(const StageArgs& S, int i, double dt, const d4* cur_a, d4 cur)
+{
+    const d4 yr = S.yr[i];
+    const d4 yv = S.yv[i];
+    d4 orr = combine_get<FORM>(S, yr, dt, [&](int k) { return S.t[k].fv[i]; });
+    d4 ovv = combine_get<FORM>(S, yv, dt, [&](int k) { return S.t[k].fa == cur_a ? cur : S.t[k].fa[i]; });
+    orr.w = yr.w;   // mass rides along unchanged
+    ovv.w = 0.0;
+    S.outr[i] = orr;
+    S.outv[i] = ovv;
+}
+
+__device__ __forceinline__ void epi_row(int form, const StageArgs& S, int i, double dt, const d4* cur_a, d4 cur)
+{
+    switch (form) {
+    case SF_AXPY: epi_row_f<SF_AXPY>(S, i, dt, cur_a, cur); break;
+    case SF_DTSUM: epi_row_f<SF_DTSUM>(S, i, dt, cur_a, cur); break;
+    case SF_ACCUM: epi_row_f<SF_ACCUM>(S, i, dt, cur_a, cur); break;
+    case SF_RK4NP1: epi_row_f<SF_RK4NP1>(S, i, dt, cur_a, cur); break;
+    default: epi_row_f<SF_RKF78NP1>(S, i, dt, cur_a, cur); break;
+    }
+}
+
+__device__ __forceinline__ void ens_epilogue(const EnsEpi& X, int i, double dt, d4 cur)
+{
+    if (X.next > 0) epi_row(X.next_form, X.rows[X.next], i, dt, X.cur_a, cur);
+    if (X.np1) epi_row(X.np1_form, X.rows[0], i, dt, X.cur_a, cur);
+}
+
+// Disks in the attempt (bit 0); stage 0's forces only for disks starting a new step (bit 1):
+// a retried disk's tiles form only the epilogue from the stored f0.  Inactive sources add +0,
+// inactive sinks get a = 0, as in the single-disk sweeps.
 template <bool EV>
 __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ tiles, const d4* __restrict__ r,
                                                    const d4* __restrict__ v, d4* __restrict__ a,
                                                    const double* __restrict__ mass, const double* __restrict__ radius,
                                                    const int* __restrict__ id, const int* __restrict__ flags,
-                                                   int need_bit, int jtile, EnsPair P)
+                                                   int need_bit, int jtile, EnsPair P, EnsEpi X)
 {
     __shared__ d4 src[MAX_JTILE];
     __shared__ int sact[MAX_JTILE];
     __shared__ double term[2][3][MAX_JTILE];
     const EnsTile T = tiles[blockIdx.x];
-    if (!(flags[T.disk] & need_bit)) return;
-    if (T.kind == 0) {
+    const int fl = flags[T.disk];
+    if (!(fl & 1)) return;
+    const bool epi = X.next > 0 || X.np1 != 0;
+    // each work-item carries at most one sink to the epilogue at the end (a tile holds at
+    // most blockDim.x sinks)
+    int ei = -1;
+    d4 ecur{0.0, 0.0, 0.0, 0.0};
+    if (!(fl & need_bit)) {
+        // forces not due (a retry's f0): the epilogue from the stored accelerations
+        const int i = T.sink0 + threadIdx.x;
+        if (i < T.sink1) { ei = i; ecur = a[i]; }
+    } else if (T.kind == 0) {
         const int i = T.sink0 + threadIdx.x;
         const bool in = i < T.sink1;
         const int ic = in ? i : T.sink0;
@@ -195,7 +248,12 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
                 if (EV && use && live && d < scr) ens_pair_test(P, T.disk, i, j0 + k, d, Ri, r, v, mass, radius, id);
             }
         }
-        if (in) a[i] = d4{live ? ax : 0.0, live ? ay : 0.0, live ? az : 0.0, 0.0};
+        const d4 res = d4{live ? ax : 0.0, live ? ay : 0.0, live ? az : 0.0, 0.0};
+        if (in) {
+            a[i] = res;
+            ei = i;
+            ecur = res;
+        }
     } else {
         // One sink against many sources (a disk's star): its three sums are serial chains in
         // ascending j, the kernel's critical path.  With more than one wave, waves 1.. form
@@ -328,7 +386,16 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
         }
         if (c < 3) reinterpret_cast<double*>(a + i)[c] = live ? acc : 0.0;
         if (c == 3) reinterpret_cast<double*>(a + i)[3] = 0.0;
+        if (epi && threadIdx.x < 64) {
+            // the three sums of lanes 0-2 gathered in lane 0 for the sink's epilogue
+            const double sx = __shfl(acc, 0), sy = __shfl(acc, 1), sz = __shfl(acc, 2);
+            if (threadIdx.x == 0) {
+                ei = i;
+                ecur = d4{live ? sx : 0.0, live ? sy : 0.0, live ? sz : 0.0, 0.0};
+            }
+        }
     }
+    if (epi && ei >= 0) ens_epilogue(X, ei, P.disk_dt[T.disk], ecur);
 }
 
 // The ejection / hit-centrum predicate on each attempt's y_(n+1): one flag per disk in the
@@ -448,6 +515,7 @@ struct red_ensemble {
     int* d_nan = nullptr;
     int* d_probe = nullptr;
     int* d_map = nullptr;                    // slot compaction map (k_ens_compact)
+    StageArgs* d_rows = nullptr;             // the stage combinations' arguments ([0] y_(n+1))
     int* d_ones = nullptr;                   // padded storage: every body gates as active
     bool padded = false;
     double* d_cscratch = nullptr;            // its gather scratch: mass, radius, id, type
@@ -493,6 +561,7 @@ struct red_ensemble {
     int* h_map = nullptr;                    // pinned; -2 outside the slots being compacted
     std::vector<EnsDiskPair> h_pair;         // host copy of d_pair
     int tiles_cap = 0;                       // d_tiles holds tiles_cap entries, empty ones at the end
+    int tiles_block = 0;                     // the launch shape the tiles and the graph were made for
     int host_threads = 1;
     d4* h_state = nullptr;                   // pinned copies of yr, yv, outr, outv (n each)
 
@@ -607,7 +676,7 @@ StageArgs ens_args(red_ensemble* E, const Row& row, d4* outr, d4* outv)
     return A;
 }
 
-hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v)
+hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v, bool epilogue = true)
 {
     if (E->n_tiles == 0) return hipSuccess;
     EnsPair P;
@@ -625,15 +694,35 @@ hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v)
     P.count = E->d_ev_count;
     P.gate = E->padded ? E->d_ones : E->d_id;
     const int need = stage == 0 ? 2 : 1;
+    // the stage combinations formed in the launch's epilogue: stage state stage + 1 (main
+    // stages, and the error stages of schemes that form them from stage sums), y_(n+1) after
+    // the last main stage
+    EnsEpi X;
+    X.rows = E->d_rows;
+    X.cur_a = E->acc[stage];
+    X.next = 0; X.next_form = 0; X.np1 = 0; X.np1_form = 0;
+    if (epilogue) {
+        const Scheme& S = E->S;
+        const bool adaptive = E->prm.adaptive && S.err_form >= 0;
+        const int nx = stage + 1;
+        if (nx < (int)S.rows.size() && (nx < S.n_main || (adaptive && !S.extra_on_yout && nx < S.n_stage))) {
+            X.next = nx;
+            X.next_form = S.rows[nx].form;
+        }
+        if (stage == S.n_main - 1) {
+            X.np1 = 1;
+            X.np1_form = S.np1.form;
+        }
+    }
     // RKF7(8) clears the records at every attempt start (int_rungekutta8.cu:730-733), so f0's
     // would be dropped unread: its launch runs without pair tests
     const bool ev = E->pair_tests && !(stage == 0 && E->S.clear_events_each_attempt);
     if (ev)
         hipLaunchKernelGGL(k_ens_force<true>, dim3(E->n_tiles), dim3(E->block), 0, E->stream, E->d_tiles, r, v,
-                           E->acc[stage], E->d_mass, E->d_radius, E->d_id, E->d_flags, need, E->jtile, P);
+                           E->acc[stage], E->d_mass, E->d_radius, E->d_id, E->d_flags, need, E->jtile, P, X);
     else
         hipLaunchKernelGGL(k_ens_force<false>, dim3(E->n_tiles), dim3(E->block), 0, E->stream, E->d_tiles, r, v,
-                           E->acc[stage], E->d_mass, E->d_radius, E->d_id, E->d_flags, need, E->jtile, P);
+                           E->acc[stage], E->d_mass, E->d_radius, E->d_id, E->d_flags, need, E->jtile, P, X);
     return hipGetLastError();
 }
 
@@ -647,20 +736,13 @@ hipError_t ens_attempt(red_ensemble* E)
     hipLaunchKernelGGL(k_ens_reset, dim3((E->nd + 255) / 256), dim3(256), 0, E->stream, E->d_err, E->d_nan, E->d_probe,
                        E->nd, E->d_ev_count);
     RED_HIP_TRY(hipGetLastError());
+    // each force launch forms the next stage state (and y_(n+1)) in its epilogue
     RED_HIP_TRY(ens_force(E, 0, E->yr, E->yv));
-    for (int r = 1; r < S.n_main; r++) {
-        RED_HIP_TRY(launch_stage(S.rows[r].form, ens_args(E, S.rows[r], E->sr[r], E->sv[r]), E->stream));
-        RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r]));
-    }
-    RED_HIP_TRY(launch_stage(S.np1.form, ens_args(E, S.np1, E->outr, E->outv), E->stream));
+    for (int r = 1; r < S.n_main; r++) RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r]));
     if (E->prm.adaptive && S.err_form >= 0) {
         for (int r = S.n_main; r < S.n_stage; r++) {
-            if (S.extra_on_yout) {
-                RED_HIP_TRY(ens_force(E, r, E->outr, E->outv));
-            } else {
-                RED_HIP_TRY(launch_stage(S.rows[r].form, ens_args(E, S.rows[r], E->sr[r], E->sv[r]), E->stream));
-                RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r]));
-            }
+            if (S.extra_on_yout) RED_HIP_TRY(ens_force(E, r, E->outr, E->outv));
+            else RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r]));
         }
         RED_HIP_TRY(launch_err(S.err_form, ens_args(E, S.err, nullptr, nullptr), E->stream));
     }
@@ -702,6 +784,11 @@ hipError_t run_attempt(red_ensemble* E)
 int upload_tiles(red_ensemble* E)
 {
     std::vector<EnsTile> tiles = build_tiles(E, E->block);
+    if (E->tiles_block != E->block) {
+        // tiles hold at most `block` sinks, one per work-item of a launch of that shape
+        destroy_graph(E);
+        E->tiles_block = E->block;
+    }
     if ((int)tiles.size() > E->tiles_cap) {
         SetupLock lk;
         (void)hipStreamSynchronize(E->stream);
@@ -1008,7 +1095,7 @@ int red_ensemble_destroy(red_ensemble* E)
     destroy_graph(E);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     for (void* p : {(void*)E->yr, (void*)E->yv, (void*)E->outr, (void*)E->outv, (void*)E->disk_of, (void*)E->err_on,
-                    (void*)E->d_in, (void*)E->d_out, (void*)E->d_map, (void*)E->d_ones, (void*)E->d_cscratch, (void*)E->d_tiles,
+                    (void*)E->d_in, (void*)E->d_out, (void*)E->d_map, (void*)E->d_ones, (void*)E->d_cscratch, (void*)E->d_rows, (void*)E->d_tiles,
                     (void*)E->d_mass,
                     (void*)E->d_radius, (void*)E->d_id, (void*)E->d_type, (void*)E->d_pair, (void*)E->d_events,
                     (void*)E->d_updates})
@@ -1147,6 +1234,17 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     E->log.assign(E->nd, {});
     E->compact_due.assign(E->nd, 0);
     if (E->h_err_on.empty()) E->h_err_on.assign(n, 0);
+    {
+        // the stage combinations' arguments, read by the force launches' epilogues
+        std::vector<StageArgs> rows(std::max(E->S.n_stage, 1));
+        memset(rows.data(), 0, sizeof(StageArgs) * rows.size());
+        rows[0] = ens_args(E, E->S.np1, E->outr, E->outv);
+        for (int r = 1; r < (int)E->S.rows.size() && r < E->S.n_stage; r++)
+            rows[r] = ens_args(E, E->S.rows[r], E->sr[r], E->sv[r]);
+        ENS_HIP(E, dmalloc(&E->d_rows, rows.size()));
+        ENS_HIP(E, stream_copy(E->stream, E->d_rows, rows.data(), sizeof(StageArgs) * rows.size(),
+                               hipMemcpyHostToDevice));
+    }
     E->h_pair.assign(E->nd, EnsDiskPair{});
     std::vector<int> all_disks(E->nd);
     for (int d = 0; d < E->nd; d++) all_disks[d] = d;
@@ -1404,9 +1502,9 @@ int red_ensemble_autotune(red_ensemble* E, int reps, int* best_block, int* best_
             // stage 0 recomputes f0 of the current state: identical values, so a retry
             // that reuses f0 is unaffected; the records it may append are cleared with the
             // next attempt's first launch
-            ENS_HIP(E, ens_force(E, 0, E->yr, E->yv));   // warm-up
+            ENS_HIP(E, ens_force(E, 0, E->yr, E->yv, false));   // warm-up
             ENS_HIP(E, hipEventRecord(a, E->stream));
-            for (int k = 0; k < reps; k++) ENS_HIP(E, ens_force(E, 0, E->yr, E->yv));
+            for (int k = 0; k < reps; k++) ENS_HIP(E, ens_force(E, 0, E->yr, E->yv, false));
             ENS_HIP(E, hipEventRecord(b, E->stream));
             ENS_HIP(E, hipEventSynchronize(b));
             float t = 0.0f;
@@ -1422,6 +1520,7 @@ int red_ensemble_autotune(red_ensemble* E, int reps, int* best_block, int* best_
     E->jtile = bj;
     int rc = upload_tiles(E);
     if (rc) return rc;
+    destroy_graph(E);   // captured launches hold the launch shape and the source tile
     if (best_block) *best_block = bb;
     if (best_jtile) *best_jtile = bj;
     return RED_OK;

@@ -65,6 +65,45 @@ def test_ensemble_autotune_keeps_results(native_mod):
         assert_bitwise(E1.state(k)[0], E2.state(k)[0], f"disk {k}")
 
 
+@pytest.mark.parametrize("integ,adaptive,tol,dt", [(0, False, 0.0, 0.1), (1, False, 0.0, 0.2), (2, True, 1e-10, 0.5),
+                                                    (3, True, 1e-10, 0.5), (4, False, 0.0, 0.5), (3, False, 0.0, 0.5)])
+def test_ensemble_every_scheme_matches_single_disk(native_mod, integ, adaptive, tol, dt):
+    """Every integrator and step control: the force launches form the stage states, y_(n+1)
+    and the error stages in their epilogues (retried attempts from the stored f0), and every
+    member equals its disk stepped alone, bit for bit."""
+    disks = small_ensemble(n=5, seed=408)
+    kw = dict(integrator=integ, adaptive=adaptive, tolerance=tol, dt_day=dt)
+    print(f"case ensemble scheme {kw}", flush=True)
+    E = native_mod.Ensemble(disks, **kw)
+    t_stop = 3.0 * 0.01720209895
+    info = E.advance(t_stop)
+    assert info["active_disks"] == 0
+    for k, b in enumerate(disks):
+        alone = run_alone(native_mod, b, t_stop, **kw)
+        r, v = E.state(k)
+        s = alone.state()
+        assert_bitwise(r, s["r"], f"disk {k} r")
+        assert_bitwise(v, s["v"], f"disk {k} v")
+        c = alone.counters()
+        di = E.disk_info(k)
+        assert (di["tried"], di["failed"], di["passed"]) == (c["tried"], c["failed"], c["passed"])
+
+
+def test_ensemble_autotune_between_advances(native_mod):
+    """Autotuning after the first advance (a graph already captured for the old launch shape)
+    leaves the results as they are."""
+    disks = small_ensemble(n=6, seed=409)
+    kw = dict(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001)
+    E1 = native_mod.Ensemble(disks, block=64, **kw)
+    E2 = native_mod.Ensemble(disks, block=64, **kw)
+    t1, t2 = 1.0 * 0.01720209895, 2.0 * 0.01720209895
+    E1.advance(t1); E2.advance(t1)
+    E2.autotune(reps=2)
+    E1.advance(t2); E2.advance(t2)
+    for k in range(len(disks)):
+        assert_bitwise(E1.state(k)[0], E2.state(k)[0], f"disk {k}")
+
+
 def test_ensemble_fixed_step_rk4(native_mod):
     disks = small_ensemble(n=6, seed=406)
     kw = dict(integrator=2, adaptive=False, dt_day=0.5)
