@@ -276,7 +276,8 @@ int red_copy_to_host(red_ctx* ctx, void* host, const void* dev, unsigned long lo
  * share one set of device buffers; every stage of an attempt is one launch over all disks,
  * the attempt is captured once in a HIP graph (params->use_graph) and replayed, and each
  * disk runs the reference's step control (integrator.cu:211-263, int_rungekutta8.cu:704-797)
- * on its own time and trial step, so it takes exactly the steps it would take alone.
+ * on its own time and trial step, on the device, so it takes exactly the steps it would take
+ * alone.
  * Every disk also runs the main loop's event checks in the reference's order
  * (main.cu:201-255) with the single-disk code: the ejection / hit-centrum check before a step,
  * pair tests inside the force launches, the collision or close-encounter pass after a step
@@ -312,6 +313,17 @@ int red_ensemble_disk_info(red_ensemble* ens, int disk, int out[3], double* t, d
 /* Picks the force-launch shape (threads per block x source tile, each in {64,128,256}) by
  * timing `reps` force evaluations per shape; ms[9] receives the timings (block-major). */
 int red_ensemble_autotune(red_ensemble* ens, int reps, int* best_block, int* best_jtile, double ms[9]);
+/* Supersteps per graph launch, 1..64 (default 8).  Each disk's controller runs on the device
+ * (the reference's, int_rungekutta8.cu:704-797 and siblings, with step_pow below), so a
+ * batch of supersteps needs the host once: a disk whose step needs the host (a merge, an
+ * ejection scan, a compaction that moves bodies) waits out the rest of its batch. */
+int red_ensemble_set_batch(red_ensemble* ens, int supersteps);
+
+/* The step controllers' factor x^y (0 < y <= 1/2) rounded to nearest, as the host controller
+ * (on_device = 0) or the ensembles' device controller (on_device = 1, on the given device)
+ * computes it, for n pairs of host arrays.  The reference calls its C runtime's pow here
+ * (int_rungekutta8.cu:777, int_rungekutta4.cu:253), whose last bit depends on the platform. */
+int red_step_pow(const double* x, const double* y, double* out, int n, int on_device, int device);
 
 /* ---- learned step-size proposer -----------------------------------------------------
  * Not in the reference, whose only step control is the elementary controller

@@ -101,6 +101,8 @@ def lib():
         L.orc_create_sp_events.argtypes = [ctypes.POINTER(OrcEvent), I, ctypes.POINTER(OrcEvent)]
         L.orc_max_error.restype = D
         L.orc_max_error.argtypes = [dp, dp, I, D, D]
+        L.orc_step_factor.restype = D
+        L.orc_step_factor.argtypes = [D, D]
         L.orc_handle_collision_pair.restype = I
         L.orc_handle_collision_pair.argtypes = [P, ctypes.POINTER(OrcEvent), I]
         _lib = L
@@ -283,3 +285,8 @@ def max_error(err_r, err_v, n_var, dt, lam):
     er = np.ascontiguousarray(err_r, dtype=np.float64)
     ev = np.ascontiguousarray(err_v, dtype=np.float64)
     return lib().orc_max_error(_dp(er), _dp(ev), int(n_var), float(dt), float(lam))
+
+
+def step_factor(x, y):
+    """The controllers' factor x^y as the oracle computes it (powq, rounded to a double)."""
+    return lib().orc_step_factor(float(x), float(y))

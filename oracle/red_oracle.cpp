@@ -36,6 +36,9 @@
 // padded = 1 restates the padded-storage kernel (-ups, pp_disk.cu:144-215): no id test on
 // sinks, sources or pairs.  Its padding particles (massless, id 0, pp_disk.cu:1550-1571)
 // add exact zeros to every sum and never meet a pair test, so none are created here.
+// The step controllers' factor pow(tol / err, 1/(order+1)) is taken correctly rounded
+// (step_factor below, through libquadmath's powq) instead of from the platform's libm, whose
+// last bit differs between C runtimes (the reference is built with MSVC's and nvcc's).
 // Additions the reference does not have (marked NEW): total energy and angular momentum,
 // an OpenMP sink loop whose results are bit-identical to the serial one, and an optional
 // mutual-Hill-radius close-encounter criterion.
@@ -50,8 +53,14 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#include <quadmath.h>
 
 namespace {
+
+// x^y rounded to nearest: powq carries 113 bits, so rounding its result to a double is the
+// correct rounding unless x^y lies within about 2^-110 of a midpoint between two doubles
+// (int_rungekutta8.cu:777, int_rungekutta4.cu:253 call the C runtime's pow here).
+double step_factor(double x, double y) { return (double)powq((__float128)x, (__float128)y); }
 
 const double Gauss  = 1.720209895e-2;            // red_constants.h:21
 const double Gauss2 = 2.959122082855911025e-4;   // red_constants.h:22
@@ -808,7 +817,7 @@ struct RK4 : Integrator {  // int_rungekutta4.cu:27-273
                 }
                 max_err = get_max_error(n_var, LAMBDA);
                 last_err = max_err;
-                dt_try *= 0.9 * pow(tolerance / max_err, 1.0 / 4.0);
+                dt_try *= 0.9 * step_factor(tolerance / max_err, 1.0 / 4.0);
                 if ((int)ppd->events.size() > 0) {
                     // the reference leaves the loop before iter++ (int_rungekutta4.cu:255-264)
                     if (dt_try < dt_did) dt_try = dt_did;
@@ -883,7 +892,7 @@ struct RK5 : Integrator {
                 }
                 max_err = get_max_error(n_var, 1.0);
                 last_err = max_err;
-                dt_try *= 0.9 * pow(tolerance / max_err, 1.0 / 5.0);
+                dt_try *= 0.9 * step_factor(tolerance / max_err, 1.0 / 5.0);
                 if ((int)ppd->events.size() > 0) {
                     // the reference leaves the loop before iter++ (int_rungekutta4.cu:255-264)
                     if (dt_try < dt_did) dt_try = dt_did;
@@ -976,7 +985,7 @@ struct RKF78 : Integrator {  // int_rungekutta8.cu:27-47,463-797
                 }
                 max_err = get_max_error(n_var, LAMBDA);
                 last_err = max_err;
-                dt_try *= 0.9 * pow(tolerance / max_err, 1.0 / (order + 1));
+                dt_try *= 0.9 * step_factor(tolerance / max_err, 1.0 / (order + 1));
             }
             iter++;
         } while (adaptive && max_err > tolerance);
@@ -1230,6 +1239,8 @@ int orc_create_sp_events(const orc_event* in, int n, orc_event* out)
 }
 
 // integrator.cu:211-256 on caller-given error vectors (n_var components each).
+double orc_step_factor(double x, double y) { return step_factor(x, y); }
+
 double orc_max_error(const double* err_r, const double* err_v, int n_var, double dt, double lambda)
 {
     return max_error(err_r, err_v, n_var, dt, lambda);

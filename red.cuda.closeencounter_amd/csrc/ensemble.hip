@@ -2,13 +2,14 @@
 //
 // All disks live in one set of buffers (bodies of disk d occupy [b0_d, b1_d)).  Every
 // stage of an attempt is ONE launch over all disks: the force launch walks a list of
-// tiles (sink range x source range of one disk), the stage-sum and error launches walk
-// all bodies with a per-body disk index, and the whole attempt is captured once in a HIP
-// graph and replayed.  Each disk keeps its own time, trial step and attempt count; the
-// host reads one error maximum per disk per attempt and runs the reference's controller
-// for every disk with the same arithmetic as a single disk, so an ensemble member
-// follows exactly the steps it would take alone (int_rungekutta8.cu:704-797 and
-// siblings).
+// tiles (sink range x source range of one disk) and forms the stage sums in its epilogue,
+// the error launch walks all bodies with a per-body disk index.  Each disk keeps its own
+// time, trial step and attempt count in device arrays, and a work-item per disk runs the
+// reference's controller after every attempt with the single-disk host controller's
+// arithmetic (step_pow.hpp), so an ensemble member follows exactly the steps it would take
+// alone (int_rungekutta8.cu:704-797 and siblings).  A batch of supersteps (one attempt of
+// every disk still stepping, each) is captured once in a HIP graph and replayed; the host
+// sees the disks once per batch.
 //
 // Events: the force launches run the pair tests of every disk (collision at
 // f (R_i + R_j), close encounter at f mutual Hill radii; pp_disk.cu:265-291) into one
@@ -19,6 +20,8 @@
 // close-encounter pass after an accepted step, the compaction after 8 events (within the
 // disk's slot of the buffers), and at the start of the next step the ejection scan, done
 // exactly on the host whenever the predicate found a body or the state was changed since.
+// The device holds a disk for the host where one of these changes its state (a merge, a
+// scan, a compaction that moves bodies); the other steps' passes run after the batch.
 //
 // Force tiles (FP64, the reference's operation order and ascending-j summation):
 //   kind 0: up to `block` sinks, one work-item each; sources staged through LDS in
@@ -40,6 +43,7 @@
 #include "red_internal.hpp"
 #include "scheme.hpp"
 #include "stage_forms.hpp"
+#include "step_pow.hpp"
 
 using namespace red;
 
@@ -68,9 +72,12 @@ struct EnsPair {
     const double* disk_dt;         // trial step per disk
     double c_stage;
     int stage;
-    red_event* buf;                // records of the attempt, all disks
+    int superstep;                 // the record's attempt field until the host stamps it
+    red_event* buf;                // records of the batch, all disks
     int cap;
     int* count;
+    int* rec;                      // records of this attempt per disk, and of them collisions
+    int* rec_col;
     const int* gate;               // activity: the ids, or ones under padded storage
 };
 
@@ -133,13 +140,15 @@ __device__ void ens_pair_test(const EnsPair& P, int disk, int i, int j, double d
     } else {
         name = RED_EV_CLOSE_ENCOUNTER;
     }
+    atomicAdd(&P.rec[disk], 1);
+    if (name == RED_EV_COLLISION) atomicAdd(&P.rec_col[disk], 1);
     const int k = atomicAdd(P.count, 1);
     if (k >= P.cap) return;   // counted but dropped: the host reports an overflow
     red_event* e = P.buf + k;
     e->name = name;
     e->id1 = id[surv]; e->idx1 = surv - D.b0;
     e->id2 = id[merg]; e->idx2 = merg - D.b0;
-    e->attempt = 0; e->stage = P.stage;
+    e->attempt = P.superstep; e->stage = P.stage;
     e->sink = il; e->source = jl; e->pad = disk;
     e->t = P.disk_t[disk] + P.c_stage * P.disk_dt[disk];
     e->d = d;
@@ -468,12 +477,162 @@ __global__ __launch_bounds__(256) void k_ens_compact_back(const int* __restrict_
     B.mass[i] = B.s_mass[i]; B.radius[i] = B.s_radius[i]; B.id[i] = B.s_id[i]; B.type[i] = B.s_type[i];
 }
 
-// Per-disk error maxima, NaN flags and ejection flags, and the attempt's record count.
-__global__ void k_ens_reset(unsigned long long* err, int* nan, int* probe, int n, int* ev_count)
+// ---- per-disk step control on the device -------------------------------------------------
+// Each disk's controller state lives in device arrays (SoA), so a batch of supersteps runs
+// without the host: every superstep starts with k_ens_begin (which disks attempt, from their
+// phase) and ends with k_ens_control (the reference's controller on the attempt's error
+// maximum), one work-item per disk.  The host mirrors the arrays between batches.
+struct EnsWords {
+    double t_stop;
+    int k_limit;       // supersteps of this batch that may run
+    int stop;          // set once the record list is half full: the batch's later supersteps idle
+    int nan_disk;      // lowest disk with a non-finite error estimate (n_disks: none)
+    int limit_disk;    // lowest disk that reached 1000 attempts in one step (n_disks: none)
+    int ev_count;      // pair records of the batch
+    int pad;
+};
+
+struct EnsCtl {
+    double *t, *dt_try, *dt_did, *last_err;
+    int *iter, *tried, *failed, *passed, *phase, *accepted_prev, *need_scan, *hold_mode;
+    EnsWords* w;
+    int* steplog;      // [superstep][disk]: 0 no attempt, 1 rejected, 2 + attempt index: accepted
+};
+
+// Per-attempt values of every disk (device only).
+struct EnsScratch {
+    int* flags;                // bit 0 attempts, bit 1 new step (forces of f0 due), bit 2 y <- y_(n+1)
+    unsigned long long* err;   // error maximum (bits of a non-negative double)
+    int *nan, *probe, *rec, *rec_col;
+};
+
+enum { PH_NEW = 0, PH_RETRY = 1, PH_DONE = 2, PH_HELD = 3 };
+constexpr int MAX_BATCH = 64;
+
+// Which disks attempt in superstep s: the loop condition t <= t_stop at a new step
+// (main.cu:176), a step that starts with an ejection scan waits for the host (held), a
+// rejected attempt is retried.  Clears the attempt's per-disk values.
+__global__ __launch_bounds__(256) void k_ens_begin(EnsCtl C, EnsScratch X, int s, int nd, int scan_on)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d < n) { err[d] = 0ull; nan[d] = 0; probe[d] = 0; }
-    if (d == 0) *ev_count = 0;
+    if (d >= nd) return;
+    int f = 0;
+    if (!C.w->stop && s < C.w->k_limit) {
+        const int ph = C.phase[d];
+        if (ph == PH_NEW) {
+            if (!(C.t[d] <= C.w->t_stop)) C.phase[d] = PH_DONE;
+            else if (scan_on && C.need_scan[d]) C.phase[d] = PH_HELD;
+            else { f = 1 | 2; C.iter[d] = 0; }
+        } else if (ph == PH_RETRY) {
+            f = 1;
+        }
+        if (f & 1) {
+            C.dt_did[d] = C.dt_try[d];
+            if (C.accepted_prev[d]) { f |= 4; C.accepted_prev[d] = 0; }
+        }
+    }
+    X.flags[d] = f;
+    X.err[d] = 0ull; X.nan[d] = 0; X.probe[d] = 0; X.rec[d] = 0; X.rec_col[d] = 0;
+    C.steplog[s * nd + d] = 0;
+}
+
+// The controller after superstep s's attempt (integrator.cu:255, int_rungekutta8.cu:776-797,
+// int_rungekutta4.cu:246-273 and the fixed-step loops), with the host controller's
+// arithmetic (tools::controller_update: the same operations and step_pow).  An accepted step
+// is held for the host when its records need a merge (collisions), or may lead to a
+// compaction that moves bodies (hold_mode, set by the host: bit 0 the slot holds inactive
+// bodies, so any record may complete the 8 events; bit 1 the 8 are already reached).
+template <bool ADAPTIVE>
+__global__ __launch_bounds__(256) void k_ens_control(EnsCtl C, EnsScratch X, int s, int nd, double lambda,
+                                                     double tol, double exponent, int accept_on_event,
+                                                     int ce_mode, int half_cap)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d == 0 && C.w->ev_count > half_cap) C.w->stop = 1;
+    if (d >= nd || !(X.flags[d] & 1)) return;
+    bool accept = true, counted = true;
+    if (ADAPTIVE) {
+        if (X.nan[d]) {
+            atomicMin(&C.w->nan_disk, d);
+            C.phase[d] = PH_HELD;
+            return;
+        }
+        const double m = __longlong_as_double((long long)X.err[d]);
+        const double max_err = fabs(C.dt_try[d] * lambda * m);
+        C.dt_try[d] = C.dt_try[d] * (0.9 * step_pow(tol / max_err, exponent));
+        C.last_err[d] = max_err;
+        accept = !(max_err > tol);
+        // RK4 and Cash-Karp accept an attempt that recorded pair events and leave the loop
+        // before counting it (int_rungekutta4.cu:255-264)
+        if (accept_on_event && X.rec[d] > 0) {
+            if (C.dt_try[d] < C.dt_did[d]) C.dt_try[d] = C.dt_did[d];
+            accept = true;
+            counted = false;
+        }
+    }
+    const int stamp = C.iter[d];
+    if (counted) C.iter[d] = stamp + 1;
+    if (accept) {
+        const int it = C.iter[d];
+        C.tried[d] += it;
+        C.failed[d] += it - 1;
+        C.passed[d] += 1;
+        C.t[d] = C.t[d] + C.dt_did[d];
+        C.accepted_prev[d] = 1;
+        C.need_scan[d] = X.probe[d] ? 1 : 0;
+        C.steplog[s * nd + d] = 2 + stamp;
+        const int hm = C.hold_mode[d];
+        const bool hold = (!ce_mode && X.rec_col[d] > 0) || (hm & 2) || ((hm & 1) && X.rec[d] > 0);
+        C.phase[d] = hold ? PH_HELD : PH_NEW;
+    } else {
+        C.phase[d] = PH_RETRY;
+        C.steplog[s * nd + d] = 1;
+        if (C.iter[d] >= 1000) {
+            atomicMin(&C.w->limit_disk, d);
+            C.phase[d] = PH_HELD;
+        }
+    }
+}
+
+// step_pow over arrays, as the device controller evaluates it
+__global__ __launch_bounds__(256) void k_step_pow(const double* __restrict__ x, const double* __restrict__ y,
+                                                  double* __restrict__ out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = step_pow(x[i], y[i]);
+}
+
+// The controller state's device layout (the host twin has the same): per-disk arrays, the
+// batch words, then the step log.  Returns the bytes of the part the host uploads.
+size_t ctl_layout(int nd, size_t off[4])
+{
+    const size_t n = (size_t)std::max(nd, 1);
+    off[0] = 0;                                              // 4 double arrays
+    off[1] = off[0] + 4 * n * sizeof(double);                // 8 int arrays
+    off[2] = (off[1] + 8 * n * sizeof(int) + 15) / 16 * 16;  // batch words
+    off[3] = off[2] + sizeof(EnsWords);                      // step log
+    return off[3];
+}
+
+size_t carve_ctl(char* base, int nd, EnsCtl* C)
+{
+    const size_t n = (size_t)std::max(nd, 1);
+    size_t off[4];
+    const size_t up = ctl_layout(nd, off);
+    double* dp = (double*)(base + off[0]);
+    C->t = dp; C->dt_try = dp + n; C->dt_did = dp + 2 * n; C->last_err = dp + 3 * n;
+    int* ip = (int*)(base + off[1]);
+    C->iter = ip; C->tried = ip + n; C->failed = ip + 2 * n; C->passed = ip + 3 * n;
+    C->phase = ip + 4 * n; C->accepted_prev = ip + 5 * n; C->need_scan = ip + 6 * n; C->hold_mode = ip + 7 * n;
+    C->w = (EnsWords*)(base + off[2]);
+    C->steplog = (int*)(base + off[3]);
+    return up;
+}
+
+size_t ctl_bytes(int nd)
+{
+    size_t off[4];
+    return ctl_layout(nd, off) + sizeof(int) * (size_t)MAX_BATCH * (size_t)std::max(nd, 1);
 }
 
 template <typename T>
@@ -504,17 +663,22 @@ struct red_ensemble {
     unsigned char* err_on = nullptr;
     double *d_mass = nullptr, *d_radius = nullptr;
     int *d_id = nullptr, *d_type = nullptr;
-    // per-disk control, one block each way per attempt: in = dt, t, flags; out = error
-    // maxima, NaN flags, ejection flags, the record count
-    char* d_in = nullptr;
-    char* d_out = nullptr;
-    size_t in_bytes = 0, out_bytes = 0;
-    double *d_dt = nullptr, *d_t = nullptr;
+    // per-disk controller state on the device and its pinned host twin (same layout): one
+    // block each way per batch of supersteps
+    char* d_ctl = nullptr;
+    char* h_ctl = nullptr;
+    size_t ctl_up = 0, ctl_all = 0;          // bytes uploaded / downloaded
+    EnsCtl dc{}, hc{};
+    // per-attempt values (device only)
+    char* d_scr = nullptr;
+    EnsScratch scr{};
+    double *d_dt = nullptr, *d_t = nullptr;  // the attempt's step and step-start time (in d_ctl)
     int* d_flags = nullptr;
     unsigned long long* d_err = nullptr;
     int* d_nan = nullptr;
-    int* d_probe = nullptr;
     int* d_map = nullptr;                    // slot compaction map (k_ens_compact)
+    int batch = 8;                           // supersteps per graph launch
+    std::vector<int> inactive;               // inactive bodies in each disk's slot (until compaction)
     StageArgs* d_rows = nullptr;             // the stage combinations' arguments ([0] y_(n+1))
     int* d_ones = nullptr;                   // padded storage: every body gates as active
     bool padded = false;
@@ -522,38 +686,32 @@ struct red_ensemble {
     EnsDiskPair* d_pair = nullptr;
     red_event* d_events = nullptr;
     int ev_cap = 0;
-    int* d_ev_count = nullptr;
+    int* d_ev_count = nullptr;               // in d_ctl's batch words
     BodyUpdate* d_updates = nullptr;
     int updates_cap = 0;
     EnsTile* d_tiles = nullptr;
     int n_tiles = 0;
     int block = 256, jtile = 256;
     bool pair_tests = false, scan_on = false;
-    // host per-disk controller state
+    // host mirror of the per-disk controller state (the device's between batches)
     std::vector<double> t, dt_try, dt_did, last_err;
-    std::vector<int> iter, tried, failed, passed, phase;   // phase 0: new step, 1: retry, 2: done
+    std::vector<int> iter, tried, failed, passed, phase;   // PH_NEW, PH_RETRY, PH_DONE, PH_HELD
     std::vector<int> accepted_prev;
-    std::vector<char> need_scan;             // the next step starts with an exact ejection scan
+    std::vector<int> need_scan;              // the next step starts with an exact ejection scan
+    std::vector<int> hold_mode;
     std::vector<events::Counters> evc;
-    // pair records of each disk's last attempt, in the pinned read-back h_events: an attempt
-    // that recorded pair events is accepted (int_rungekutta4.cu:255-264, and every attempt of
-    // a fixed step) or its records are cleared with the next attempt (RKF7(8),
-    // int_rungekutta8.cu:730-733), so a step's records are those of its last attempt
+    // pair records of one step of a disk, in the pinned read-back h_events: an attempt that
+    // recorded pair events is accepted (int_rungekutta4.cu:255-264, and every attempt of a
+    // fixed step) or its records are cleared with the next attempt (RKF7(8),
+    // int_rungekutta8.cu:730-733), so a step's records are those of its accepted attempt
     std::vector<std::vector<const red_event*>> pending;
     std::vector<int> pending_attempt;        // the attempt counter when they were recorded
     std::vector<std::vector<red_event>> log;       // handled records (sp lists) not yet read
-    red_event* h_events = nullptr;         // pinned staging of an attempt's pair records
+    red_event* h_events = nullptr;         // pinned staging of a batch's pair records
     int h_events_cap = 0;
-    char* h_in = nullptr;                  // pinned staging of the control blocks
-    char* h_out = nullptr;
-    double* h_dt = nullptr;
-    double* h_t = nullptr;
-    int* h_flags = nullptr;
-    unsigned long long* h_err = nullptr;
-    int* h_nan = nullptr;
-    int* h_probe = nullptr;
-    int* h_ev_count = nullptr;
+    int* h_flags = nullptr;                // pinned: flags for the closing y <- y_(n+1) copy
     hipGraphExec_t graph = nullptr;
+    int graph_batch = 0;                     // supersteps in the captured graph
     bool use_graph = true;
     std::vector<char> compact_due;           // the disk's slot is compacted after this attempt
     double event_s = 0.0, rebuild_s = 0.0;   // host time of the current advance call
@@ -676,7 +834,7 @@ StageArgs ens_args(red_ensemble* E, const Row& row, d4* outr, d4* outv)
     return A;
 }
 
-hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v, bool epilogue = true)
+hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v, int superstep, bool epilogue = true)
 {
     if (E->n_tiles == 0) return hipSuccess;
     EnsPair P;
@@ -692,6 +850,9 @@ hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v, bool 
     P.buf = E->d_events;
     P.cap = E->ev_cap;
     P.count = E->d_ev_count;
+    P.superstep = superstep;
+    P.rec = E->scr.rec;
+    P.rec_col = E->scr.rec_col;
     P.gate = E->padded ? E->d_ones : E->d_id;
     const int need = stage == 0 ? 2 : 1;
     // the stage combinations formed in the launch's epilogue: stage state stage + 1 (main
@@ -726,31 +887,46 @@ hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v, bool 
     return hipGetLastError();
 }
 
-// One attempt for every disk (the body of the captured graph).
-hipError_t ens_attempt(red_ensemble* E)
+// Superstep s of a batch: which disks attempt, their attempt, their controllers.
+hipError_t ens_superstep(red_ensemble* E, int s)
 {
     const Scheme& S = E->S;
-    const dim3 gb((E->n + 255) / 256), bb(256);
+    const dim3 gb((E->n + 255) / 256), bb(256), gd((E->nd + 255) / 256);
+    hipLaunchKernelGGL(k_ens_begin, gd, bb, 0, E->stream, E->dc, E->scr, s, E->nd, E->scan_on ? 1 : 0);
+    RED_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_ens_accept, gb, bb, 0, E->stream, E->yr, E->yv, E->outr, E->outv, E->disk_of, E->d_flags, E->n);
     RED_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_ens_reset, dim3((E->nd + 255) / 256), dim3(256), 0, E->stream, E->d_err, E->d_nan, E->d_probe,
-                       E->nd, E->d_ev_count);
-    RED_HIP_TRY(hipGetLastError());
     // each force launch forms the next stage state (and y_(n+1)) in its epilogue
-    RED_HIP_TRY(ens_force(E, 0, E->yr, E->yv));
-    for (int r = 1; r < S.n_main; r++) RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r]));
-    if (E->prm.adaptive && S.err_form >= 0) {
+    RED_HIP_TRY(ens_force(E, 0, E->yr, E->yv, s));
+    for (int r = 1; r < S.n_main; r++) RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r], s));
+    const bool adaptive = E->prm.adaptive && S.err_form >= 0;
+    if (adaptive) {
         for (int r = S.n_main; r < S.n_stage; r++) {
-            if (S.extra_on_yout) RED_HIP_TRY(ens_force(E, r, E->outr, E->outv));
-            else RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r]));
+            if (S.extra_on_yout) RED_HIP_TRY(ens_force(E, r, E->outr, E->outv, s));
+            else RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r], s));
         }
         RED_HIP_TRY(launch_err(S.err_form, ens_args(E, S.err, nullptr, nullptr), E->stream));
     }
     if (E->scan_on) {
         hipLaunchKernelGGL(k_ens_probe, gb, bb, 0, E->stream, E->outr, E->d_id, E->d_type, E->disk_of, E->d_flags,
-                           E->n, E->prm.ejection, E->prm.hit_centrum, E->d_probe);
+                           E->n, E->prm.ejection, E->prm.hit_centrum, E->scr.probe);
         RED_HIP_TRY(hipGetLastError());
     }
+    const int half = E->ev_cap / 2;
+    const int ce = E->prm.close_encounter ? 1 : 0, aoe = S.accept_on_event ? 1 : 0;
+    if (adaptive)
+        hipLaunchKernelGGL(k_ens_control<true>, gd, bb, 0, E->stream, E->dc, E->scr, s, E->nd, S.lambda,
+                           E->prm.tolerance, S.exponent, aoe, ce, half);
+    else
+        hipLaunchKernelGGL(k_ens_control<false>, gd, bb, 0, E->stream, E->dc, E->scr, s, E->nd, S.lambda,
+                           E->prm.tolerance, S.exponent, aoe, ce, half);
+    return hipGetLastError();
+}
+
+// A batch: `batch` supersteps (the body of the captured graph).
+hipError_t ens_batch(red_ensemble* E)
+{
+    for (int s = 0; s < E->batch; s++) RED_HIP_TRY(ens_superstep(E, s));
     return hipSuccess;
 }
 
@@ -761,20 +937,22 @@ void destroy_graph(red_ensemble* E)
     E->graph = nullptr;
 }
 
-hipError_t run_attempt(red_ensemble* E)
+hipError_t run_batch(red_ensemble* E)
 {
-    if (!E->use_graph) return ens_attempt(E);
+    if (!E->use_graph) return ens_batch(E);
+    if (E->graph && E->graph_batch != E->batch) destroy_graph(E);
     if (!E->graph) {
         SetupLock lk;
         hipGraph_t g = nullptr;
         RED_HIP_TRY(hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = ens_attempt(E);
+        hipError_t e = ens_batch(E);
         hipError_t e2 = hipStreamEndCapture(E->stream, &g);
         if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return e; }
         RED_HIP_TRY(e2);
         e = hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         RED_HIP_TRY(e);
+        E->graph_batch = E->batch;
     }
     return hipGraphLaunch(E->graph, E->stream);
 }
@@ -1029,22 +1207,82 @@ void for_disks(int threads, const std::vector<int>& list, F f)
     for (auto& th : pool) th.join();
 }
 
-// The checks after the accepted steps of the disks in `list`: their event passes, then every
-// merge's device update in one launch (each disk's state is y_(n+1)), then the rebuilds.
-int after_steps(red_ensemble* E, const std::vector<int>& list)
+// Inactive bodies in disk d's slot (merged, ejected or hit the centre, until a compaction).
+int count_inactive(const red_ensemble* E, int d)
 {
-    if (list.empty()) return RED_OK;
+    int k = 0;
+    for (int i = E->b0[d]; i < E->b1[d]; i++) k += !(0 < E->h_id[i] && 7 > E->h_type[i]);
+    return k;
+}
+
+// Whether the host must see disk d after its next accepted step (k_ens_control): without
+// inactive bodies a compaction keeps the slot as it is, so only merges need the host; with
+// them, any record may complete the 8 events of a compaction, or the 8 are already there.
+int hold_mode_of(const red_ensemble* E, int d)
+{
+    if (E->prm.close_encounter || E->inactive[d] == 0) return 0;
+    return E->evc[d].event[events::CNT_LAST_CLEAR] >= 8 ? 3 : 1;
+}
+
+// The checks after every accepted step of a batch (main.cu:225-255), member by member in
+// step order on up to host_threads threads: each step's records are those its accepted
+// attempt recorded (the superstep is in the record's attempt field).  A step the device did
+// not hold changes nothing on the device (no merge; a compaction, if due, keeps every body in
+// place); a held disk's last step may merge, then compact.  Merges go to the device in one
+// launch, the compactions follow.
+int after_batch(red_ensemble* E, int ne)
+{
+    const int nd = E->nd, K = E->batch;
+    const int* log = E->hc.steplog;
     const auto t_ev = std::chrono::steady_clock::now();
-    size_t records = 0;
-    for (int d : list) records += E->pending[d].size();
-    std::vector<std::vector<BodyUpdate>> ups(E->nd);
-    std::vector<int> rcs(E->nd, RED_OK);
-    std::vector<std::string> msgs(E->nd);
-    std::vector<char> comp(E->nd, 0);
-    for_disks(records >= 1024 ? E->host_threads : 1, list, [&](int d) {
-        bool c = false;
-        rcs[d] = after_step_host(E, d, ups[d], &c, msgs[d]);
-        comp[d] = c;
+    std::vector<int> list;
+    for (int d = 0; d < nd; d++)
+        for (int s = 0; s < K; s++)
+            if (log[s * nd + d] >= 2) { list.push_back(d); break; }
+    if (list.empty()) return RED_OK;
+    // records to their disks, each disk's in superstep order
+    std::vector<std::vector<const red_event*>> rec(nd);
+    for (int q = 0; q < ne; q++) {
+        const red_event& e = E->h_events[q];
+        if (E->S.clear_events_each_attempt && e.stage == 0) continue;   // f0's records, cleared with the attempt
+        rec[e.pad].push_back(&e);
+    }
+    std::vector<std::vector<BodyUpdate>> ups(nd);
+    std::vector<int> rcs(nd, RED_OK);
+    std::vector<std::string> msgs(nd);
+    std::vector<char> comp(nd, 0);
+    for_disks(ne >= 1024 ? E->host_threads : 1, list, [&](int d) {
+        std::vector<const red_event*>& R = rec[d];
+        std::stable_sort(R.begin(), R.end(), [](const red_event* a, const red_event* b) { return a->attempt < b->attempt; });
+        int last = -1;
+        for (int s = 0; s < K; s++)
+            if (log[s * nd + d] >= 2) last = s;
+        size_t q = 0;
+        for (int s = 0; s <= last; s++) {
+            const size_t q0 = q;
+            while (q < R.size() && R[q]->attempt == s) q++;
+            if (log[s * nd + d] < 2) continue;   // a rejected attempt's records are dropped
+            E->pending[d].assign(R.begin() + q0, R.begin() + q);
+            E->pending_attempt[d] = log[s * nd + d] - 2;
+            std::vector<BodyUpdate> u;
+            bool c = false;
+            const int rc = after_step_host(E, d, u, &c, msgs[d]);
+            if (rc) { rcs[d] = rc; return; }
+            const bool held = s == last && E->phase[d] == PH_HELD;
+            if (!u.empty() && !held) { rcs[d] = RED_ERR_ARG; msgs[d] = "a merge in a step the device did not hold"; return; }
+            if (!u.empty()) ups[d] = std::move(u);
+            if (c) {
+                if (count_inactive(E, d) == 0) {
+                    E->evc[d].set(events::CNT_LAST_CLEAR, 0);   // the rebuild keeps every body in place
+                } else if (held) {
+                    comp[d] = 1;
+                } else {
+                    rcs[d] = RED_ERR_ARG;
+                    msgs[d] = "a compaction in a step the device did not hold";
+                    return;
+                }
+            }
+        }
     });
     std::vector<BodyUpdate> all;
     std::vector<int> merged;
@@ -1053,11 +1291,11 @@ int after_steps(red_ensemble* E, const std::vector<int>& list)
         if (!ups[d].empty()) {
             merged.push_back(d);
             for (BodyUpdate u : ups[d]) { u.idx += E->b0[d]; all.push_back(u); }
+            // the step's ejection predicate holds for the state unless a merge moved a body; a
+            // rebuild only drops inactive bodies, which no scan visits
+            if (E->scan_on) E->need_scan[d] = 1;
         }
         if (comp[d]) E->compact_due[d] = 1;
-        // the attempt's ejection predicate holds for the state unless a merge moved a body; a
-        // rebuild only drops inactive bodies, which no scan visits
-        if (E->scan_on) E->need_scan[d] = E->h_probe[d] || !ups[d].empty();
     }
     if (!all.empty()) {
         if ((int)all.size() > E->updates_cap) {
@@ -1075,13 +1313,42 @@ int after_steps(red_ensemble* E, const std::vector<int>& list)
         ENS_HIP(E, hipStreamSynchronize(E->stream));
         const int rc = upload_pairs(E, merged);
         if (rc) return rc;
+        for (int d : merged) E->inactive[d] = count_inactive(E, d);
     }
     E->event_s += seconds_since(t_ev);
     const auto t_rb = std::chrono::steady_clock::now();
+    std::vector<int> compacted;
+    for (int d = 0; d < nd; d++)
+        if (E->compact_due[d]) compacted.push_back(d);
     const int rc = compact_all(E);
     if (rc) return rc;
+    for (int d : compacted) E->inactive[d] = count_inactive(E, d);
     E->rebuild_s += seconds_since(t_rb);
     return RED_OK;
+}
+
+// The host mirror of the controller state into the pinned twin of the device block, and back.
+void pack_ctl(red_ensemble* E)
+{
+    const int nd = E->nd;
+    const EnsCtl& H = E->hc;
+    for (int d = 0; d < nd; d++) {
+        H.t[d] = E->t[d]; H.dt_try[d] = E->dt_try[d]; H.dt_did[d] = E->dt_did[d]; H.last_err[d] = E->last_err[d];
+        H.iter[d] = E->iter[d]; H.tried[d] = E->tried[d]; H.failed[d] = E->failed[d]; H.passed[d] = E->passed[d];
+        H.phase[d] = E->phase[d]; H.accepted_prev[d] = E->accepted_prev[d]; H.need_scan[d] = E->need_scan[d];
+        H.hold_mode[d] = E->hold_mode[d];
+    }
+}
+
+void unpack_ctl(red_ensemble* E)
+{
+    const int nd = E->nd;
+    const EnsCtl& H = E->hc;
+    for (int d = 0; d < nd; d++) {
+        E->t[d] = H.t[d]; E->dt_try[d] = H.dt_try[d]; E->dt_did[d] = H.dt_did[d]; E->last_err[d] = H.last_err[d];
+        E->iter[d] = H.iter[d]; E->tried[d] = H.tried[d]; E->failed[d] = H.failed[d]; E->passed[d] = H.passed[d];
+        E->phase[d] = H.phase[d]; E->accepted_prev[d] = H.accepted_prev[d]; E->need_scan[d] = H.need_scan[d];
+    }
 }
 
 }  // namespace
@@ -1095,7 +1362,7 @@ int red_ensemble_destroy(red_ensemble* E)
     destroy_graph(E);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
     for (void* p : {(void*)E->yr, (void*)E->yv, (void*)E->outr, (void*)E->outv, (void*)E->disk_of, (void*)E->err_on,
-                    (void*)E->d_in, (void*)E->d_out, (void*)E->d_map, (void*)E->d_ones, (void*)E->d_cscratch, (void*)E->d_rows, (void*)E->d_tiles,
+                    (void*)E->d_ctl, (void*)E->d_scr, (void*)E->d_map, (void*)E->d_ones, (void*)E->d_cscratch, (void*)E->d_rows, (void*)E->d_tiles,
                     (void*)E->d_mass,
                     (void*)E->d_radius, (void*)E->d_id, (void*)E->d_type, (void*)E->d_pair, (void*)E->d_events,
                     (void*)E->d_updates})
@@ -1103,7 +1370,7 @@ int red_ensemble_destroy(red_ensemble* E)
     for (auto p : E->sr) (void)hipFree(p);
     for (auto p : E->sv) (void)hipFree(p);
     for (auto p : E->acc) (void)hipFree(p);
-    for (void* p : {(void*)E->h_in, (void*)E->h_out, (void*)E->h_events, (void*)E->h_state, (void*)E->h_map})
+    for (void* p : {(void*)E->h_ctl, (void*)E->h_flags, (void*)E->h_events, (void*)E->h_state, (void*)E->h_map})
         if (p) (void)hipHostFree(p);
     if (E->stream) (void)hipStreamDestroy(E->stream);
     delete E;
@@ -1191,24 +1458,28 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     ENS_HIP(E, hipHostMalloc((void**)&E->h_map, sizeof(int) * n, hipHostMallocDefault));
     for (int i = 0; i < n; i++) E->h_map[i] = -2;
     const size_t nd = (size_t)std::max(E->nd, 1);
-    E->in_bytes = nd * (sizeof(double) * 2 + sizeof(int));
-    E->out_bytes = nd * (sizeof(unsigned long long) + 2 * sizeof(int)) + sizeof(int);
-    ENS_HIP(E, dmalloc(&E->d_in, E->in_bytes));
-    ENS_HIP(E, dmalloc(&E->d_out, E->out_bytes));
-    ENS_HIP(E, hipHostMalloc((void**)&E->h_in, E->in_bytes, hipHostMallocDefault));
-    ENS_HIP(E, hipHostMalloc((void**)&E->h_out, E->out_bytes, hipHostMallocDefault));
-    E->d_dt = (double*)E->d_in; E->d_t = E->d_dt + nd; E->d_flags = (int*)(E->d_t + nd);
-    E->h_dt = (double*)E->h_in; E->h_t = E->h_dt + nd; E->h_flags = (int*)(E->h_t + nd);
-    E->d_err = (unsigned long long*)E->d_out; E->d_nan = (int*)(E->d_err + nd);
-    E->d_probe = E->d_nan + nd; E->d_ev_count = E->d_probe + nd;
-    E->h_err = (unsigned long long*)E->h_out; E->h_nan = (int*)(E->h_err + nd);
-    E->h_probe = E->h_nan + nd; E->h_ev_count = E->h_probe + nd;
-    memset(E->h_in, 0, E->in_bytes);
-    memset(E->h_out, 0, E->out_bytes);
+    // the controller state (device block and its pinned twin) and the per-attempt values
+    E->ctl_all = ctl_bytes(E->nd);
+    ENS_HIP(E, dmalloc(&E->d_ctl, E->ctl_all));
+    ENS_HIP(E, hipHostMalloc((void**)&E->h_ctl, E->ctl_all, hipHostMallocDefault));
+    memset(E->h_ctl, 0, E->ctl_all);
+    E->ctl_up = carve_ctl(E->d_ctl, E->nd, &E->dc);
+    (void)carve_ctl(E->h_ctl, E->nd, &E->hc);
+    const size_t scr_bytes = nd * (sizeof(unsigned long long) + 5 * sizeof(int));
+    ENS_HIP(E, dmalloc(&E->d_scr, scr_bytes));
+    E->scr.err = (unsigned long long*)E->d_scr;
+    E->scr.flags = (int*)(E->scr.err + nd);
+    E->scr.nan = E->scr.flags + nd; E->scr.probe = E->scr.nan + nd;
+    E->scr.rec = E->scr.probe + nd; E->scr.rec_col = E->scr.rec + nd;
+    E->d_t = E->dc.t; E->d_dt = E->dc.dt_did;
+    E->d_flags = E->scr.flags; E->d_err = E->scr.err; E->d_nan = E->scr.nan;
+    E->d_ev_count = &E->dc.w->ev_count;
+    ENS_HIP(E, hipHostMalloc((void**)&E->h_flags, sizeof(int) * nd, hipHostMallocDefault));
+    ENS_HIP(E, hipMemsetAsync(E->d_scr, 0, scr_bytes, E->stream));
+    ENS_HIP(E, stream_copy(E->stream, E->d_ctl, E->h_ctl, E->ctl_all, hipMemcpyHostToDevice));
     ENS_HIP(E, dmalloc(&E->d_pair, E->nd));
     E->ev_cap = p->event_capacity > 0 ? p->event_capacity : std::max(4096, E->n);   // as a single disk's
     ENS_HIP(E, dmalloc(&E->d_events, E->ev_cap));
-    ENS_HIP(E, hipMemsetAsync(E->d_out, 0, E->out_bytes, E->stream));
     if (E->n > 0) {
         ENS_HIP(E, stream_copy(E->stream, E->yr, r.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->yv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
@@ -1226,8 +1497,11 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     E->t.assign(E->nd, 0.0); E->dt_try.assign(E->nd, dt0); E->dt_did.assign(E->nd, 0.0);
     E->last_err.assign(E->nd, 0.0);
     E->iter.assign(E->nd, 0); E->tried.assign(E->nd, 0); E->failed.assign(E->nd, 0); E->passed.assign(E->nd, 0);
-    E->phase.assign(E->nd, 0); E->accepted_prev.assign(E->nd, 0);
+    E->phase.assign(E->nd, PH_NEW); E->accepted_prev.assign(E->nd, 0);
     E->need_scan.assign(E->nd, 1);
+    E->hold_mode.assign(E->nd, 0);
+    E->inactive.assign(E->nd, 0);
+    for (int d = 0; d < E->nd; d++) E->inactive[d] = count_inactive(E, d);
     E->evc.assign(E->nd, events::Counters());
     E->pending.assign(E->nd, {});
     E->pending_attempt.assign(E->nd, 0);
@@ -1268,28 +1542,28 @@ int red_ensemble_set_time(red_ensemble* E, double t)
 
 // Integrates every disk with the reference loop `while (t <= t_stop) { ejection check; step;
 // collision check; rebuild check }` (main.cu:176-255) until all disks are done or
-// max_attempts attempts have been launched.
+// max_attempts supersteps have run.  The device runs batches of supersteps with each disk's
+// controller (k_ens_begin / k_ens_control); between batches the host runs what the device
+// held a disk for (ejection scans, merges, compactions) and every step's event passes.
 int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_ensemble_info* info)
 {
     if (!E) return RED_ERR_ARG;
-    const Scheme& S = E->S;
-    const bool adaptive = E->prm.adaptive && S.err_form >= 0;
-    int attempts = 0, active = 0;
+    int attempts = 0;
     const auto t_begin = std::chrono::steady_clock::now();
     E->event_s = 0.0;
     E->rebuild_s = 0.0;
+    const int nd = E->nd;
     // a disk done against an earlier t_stop continues (a run to t1, then to t2, steps as one
     // run to t2); a disk left in a retry by max_attempts resumes it
-    for (int d = 0; d < E->nd; d++)
-        if (E->phase[d] == 2) E->phase[d] = 0;
+    for (int d = 0; d < nd; d++)
+        if (E->phase[d] == PH_DONE) E->phase[d] = PH_NEW;
     for (;;) {
-        active = 0;
-        // the loop iteration's ejection / hit-centrum check, before the step; many disks'
-        // scans read one host copy of the state buffers
+        // the loop iteration's ejection / hit-centrum check, before the step, for the disks at
+        // a step boundary that take a step; many disks' scans read one host copy of the state
         int scans = 0;
-        for (int d = 0; d < E->nd; d++) {
-            if (E->phase[d] == 0 && !(E->t[d] <= t_stop)) E->phase[d] = 2;
-            scans += E->phase[d] == 0 && E->scan_on && E->need_scan[d];
+        for (int d = 0; d < nd; d++) {
+            const bool boundary = E->phase[d] == PH_NEW || E->phase[d] == PH_HELD;
+            scans += boundary && E->t[d] <= t_stop && E->scan_on && E->need_scan[d];
         }
         if (scans > 0) {
             const auto t_scan = std::chrono::steady_clock::now();
@@ -1298,36 +1572,46 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
                 const int rc = fetch_states(E);
                 if (rc) return rc;
             }
-            for (int d = 0; d < E->nd; d++) {
-                if (!(E->phase[d] == 0 && E->scan_on && E->need_scan[d])) continue;
+            for (int d = 0; d < nd; d++) {
+                const bool boundary = E->phase[d] == PH_NEW || E->phase[d] == PH_HELD;
+                if (!(boundary && E->t[d] <= t_stop && E->scan_on && E->need_scan[d])) continue;
                 const int rc = ejection_check(E, d, bulk);
                 if (rc) return rc;
                 E->need_scan[d] = 0;
+                E->inactive[d] = count_inactive(E, d);
             }
             E->event_s += seconds_since(t_scan);
         }
-        for (int d = 0; d < E->nd; d++) {
-            int f = 0;
-            if (E->phase[d] == 0) {
-                f |= 1 | 2;
-                E->iter[d] = 0;
-            } else if (E->phase[d] == 1) {
-                f |= 1;
-            }
-            if (E->accepted_prev[d]) f |= 4;
-            if (f & 1) { active++; E->dt_did[d] = E->dt_try[d]; }
-            E->h_flags[d] = f;
-            E->h_dt[d] = E->dt_try[d];
-            E->h_t[d] = E->t[d];
+        int active = 0;
+        for (int d = 0; d < nd; d++) {
+            if (E->phase[d] == PH_HELD) E->phase[d] = PH_NEW;
+            active += (E->phase[d] == PH_NEW && E->t[d] <= t_stop) || E->phase[d] == PH_RETRY;
+            E->hold_mode[d] = hold_mode_of(E, d);
         }
         if (active == 0 || attempts >= max_attempts) break;
-        ENS_HIP(E, hipMemcpyAsync(E->d_in, E->h_in, E->in_bytes, hipMemcpyHostToDevice, E->stream));
-        ENS_HIP(E, run_attempt(E));
-        ENS_HIP(E, hipMemcpyAsync(E->h_out, E->d_out, E->out_bytes, hipMemcpyDeviceToHost, E->stream));
+        pack_ctl(E);
+        EnsWords& w = *E->hc.w;
+        w.t_stop = t_stop;
+        w.k_limit = std::min(E->batch, max_attempts - attempts);
+        w.stop = 0;
+        w.nan_disk = nd;
+        w.limit_disk = nd;
+        w.ev_count = 0;
+        w.pad = 0;
+        ENS_HIP(E, hipMemcpyAsync(E->d_ctl, E->h_ctl, E->ctl_up, hipMemcpyHostToDevice, E->stream));
+        ENS_HIP(E, run_batch(E));
+        ENS_HIP(E, hipMemcpyAsync(E->h_ctl, E->d_ctl, E->ctl_all, hipMemcpyDeviceToHost, E->stream));
         ENS_HIP(E, hipStreamSynchronize(E->stream));
-        attempts++;
-        // the attempt's pair records, to their disks
-        const int ne = E->pair_tests ? *E->h_ev_count : 0;
+        unpack_ctl(E);
+        for (int s = 0; s < E->batch; s++) {
+            bool any = false;
+            for (int d = 0; d < nd && !any; d++) any = E->hc.steplog[s * nd + d] != 0;
+            attempts += any;
+        }
+        if (w.nan_disk < nd) return E->fail(RED_ERR_NAN, "non-finite error estimate in disk " + std::to_string(w.nan_disk));
+        if (w.limit_disk < nd) return E->fail(RED_ERR_STEP_LIMIT, "disk " + std::to_string(w.limit_disk) + " did not converge");
+        // the batch's pair records, to their disks
+        const int ne = E->pair_tests ? w.ev_count : 0;
         if (ne > E->ev_cap)
             return E->fail(RED_ERR_EVENT_OVERFLOW, "pair events exceed the event capacity (" + std::to_string(ne) +
                                                        " > " + std::to_string(E->ev_cap) + ")");
@@ -1340,76 +1624,62 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         }
         if (ne > 0)
             ENS_HIP(E, stream_copy(E->stream, E->h_events, E->d_events, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
-        for (int d = 0; d < E->nd; d++) {
-            if (!(E->h_flags[d] & 1)) continue;
-            // records still pending here would point into the read-back just overwritten
-            if (!E->pending[d].empty() && !S.clear_events_each_attempt)
-                return E->fail(RED_ERR_ARG, "disk " + std::to_string(d) + ": pair records of a rejected attempt");
-            E->pending[d].clear();
-            E->pending_attempt[d] = E->iter[d];
-        }
-        for (int q = 0; q < ne; q++) {
-            const red_event& e = E->h_events[q];
-            if (S.clear_events_each_attempt && e.stage == 0) continue;   // f0's records, cleared with the attempt
-            E->pending[e.pad].push_back(&e);
-        }
-        std::vector<int> accepted;
-        for (int d = 0; d < E->nd; d++) {
-            E->accepted_prev[d] = 0;
-            if (!(E->h_flags[d] & 1)) continue;
-            bool accept = true, counted = true;
-            if (adaptive) {
-                if (E->h_nan[d]) return E->fail(RED_ERR_NAN, "non-finite error estimate in disk " + std::to_string(d));
-                double m;
-                memcpy(&m, &E->h_err[d], 8);
-                const double max_err = tools::controller_update(&E->dt_try[d], S.lambda, m, E->prm.tolerance, S.exponent);
-                E->last_err[d] = max_err;
-                accept = !(max_err > E->prm.tolerance);
-                // RK4 and Cash-Karp accept an attempt that recorded pair events and leave the
-                // loop before counting it (int_rungekutta4.cu:255-264)
-                if (S.accept_on_event && !E->pending[d].empty()) {
-                    if (E->dt_try[d] < E->dt_did[d]) E->dt_try[d] = E->dt_did[d];
-                    accept = true;
-                    counted = false;
-                }
-            }
-            if (counted) E->iter[d]++;
-            if (accept) {
-                E->tried[d] += E->iter[d];
-                E->failed[d] += E->iter[d] - 1;
-                E->passed[d]++;
-                E->t[d] += E->dt_did[d];
-                E->phase[d] = 0;
-                E->accepted_prev[d] = 1;
-                accepted.push_back(d);
-            } else {
-                E->phase[d] = 1;
-                if (E->iter[d] >= 1000) return E->fail(RED_ERR_STEP_LIMIT, "disk " + std::to_string(d) + " did not converge");
-            }
-        }
-        const int rc = after_steps(E, accepted);
+        const int rc = after_batch(E, ne);
         if (rc) return rc;
     }
     // apply the last accepted results so that the state buffers hold y
     bool pending = false;
-    for (int d = 0; d < E->nd; d++) { E->h_flags[d] = E->accepted_prev[d] ? 4 : 0; pending |= E->accepted_prev[d] != 0; }
+    for (int d = 0; d < nd; d++) { E->h_flags[d] = E->accepted_prev[d] ? 4 : 0; pending |= E->accepted_prev[d] != 0; }
     if (pending) {
-        ENS_HIP(E, hipMemcpyAsync(E->d_flags, E->h_flags, sizeof(int) * E->nd, hipMemcpyHostToDevice, E->stream));
+        ENS_HIP(E, hipMemcpyAsync(E->d_flags, E->h_flags, sizeof(int) * nd, hipMemcpyHostToDevice, E->stream));
         hipLaunchKernelGGL(k_ens_accept, dim3((E->n + 255) / 256), dim3(256), 0, E->stream, E->yr, E->yv, E->outr,
                            E->outv, E->disk_of, E->d_flags, E->n);
         ENS_HIP(E, hipGetLastError());
         ENS_HIP(E, hipStreamSynchronize(E->stream));
-        for (int d = 0; d < E->nd; d++) E->accepted_prev[d] = 0;
+        for (int d = 0; d < nd; d++) E->accepted_prev[d] = 0;
     }
     if (info) {
         info->supersteps = attempts;
         int still = 0;
-        for (int d = 0; d < E->nd; d++) still += E->phase[d] == 1 || (E->phase[d] == 0 && E->t[d] <= t_stop);
+        for (int d = 0; d < nd; d++) still += E->phase[d] == PH_RETRY || (E->phase[d] == PH_NEW && E->t[d] <= t_stop);
         info->active_disks = still;
         info->seconds = seconds_since(t_begin);
         info->event_seconds = E->event_s;
         info->rebuild_seconds = E->rebuild_s;
     }
+    return RED_OK;
+}
+
+int red_step_pow(const double* x, const double* y, double* out, int n, int on_device, int device)
+{
+    if (n < 0 || (n > 0 && (!x || !y || !out))) return RED_ERR_ARG;
+    if (!on_device) {
+        for (int i = 0; i < n; i++) out[i] = step_pow(x[i], y[i]);
+        return RED_OK;
+    }
+    if (n == 0) return RED_OK;
+    if (hipSetDevice(device) != hipSuccess) return RED_ERR_HIP;
+    double* buf = nullptr;
+    const size_t b = sizeof(double) * (size_t)n;
+    if (hipMalloc((void**)&buf, 3 * b) != hipSuccess) return RED_ERR_HIP;
+    hipError_t e = hipMemcpy(buf, x, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(buf + n, y, b, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_step_pow, dim3((n + 255) / 256), dim3(256), 0, 0, buf, buf + n, buf + 2 * (size_t)n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, buf + 2 * (size_t)n, b, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    return e == hipSuccess ? RED_OK : RED_ERR_HIP;
+}
+
+// Supersteps per graph launch (1..64): each batch is one captured graph of that many
+// supersteps and one control transfer each way.
+int red_ensemble_set_batch(red_ensemble* E, int supersteps)
+{
+    if (!E || supersteps < 1 || supersteps > MAX_BATCH) return RED_ERR_ARG;
+    if (supersteps != E->batch) destroy_graph(E);
+    E->batch = supersteps;
     return RED_OK;
 }
 
@@ -1502,9 +1772,9 @@ int red_ensemble_autotune(red_ensemble* E, int reps, int* best_block, int* best_
             // stage 0 recomputes f0 of the current state: identical values, so a retry
             // that reuses f0 is unaffected; the records it may append are cleared with the
             // next attempt's first launch
-            ENS_HIP(E, ens_force(E, 0, E->yr, E->yv, false));   // warm-up
+            ENS_HIP(E, ens_force(E, 0, E->yr, E->yv, 0, false));   // warm-up
             ENS_HIP(E, hipEventRecord(a, E->stream));
-            for (int k = 0; k < reps; k++) ENS_HIP(E, ens_force(E, 0, E->yr, E->yv, false));
+            for (int k = 0; k < reps; k++) ENS_HIP(E, ens_force(E, 0, E->yr, E->yv, 0, false));
             ENS_HIP(E, hipEventRecord(b, E->stream));
             ENS_HIP(E, hipEventSynchronize(b));
             float t = 0.0f;

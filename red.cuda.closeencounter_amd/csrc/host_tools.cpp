@@ -1,6 +1,7 @@
 // host_tools.cpp -- host-side arithmetic of the event pipeline and step control.
 // Built with g++ -ffp-contract=off (see host_tools.hpp for why not with the HIP compiler).
 #include "host_tools.hpp"
+#include "step_pow.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -149,14 +150,14 @@ int calculate_phase(double mu, const double* oe, double* r, double* v)
 double controller_update(double* dt, double lambda, double errmax, double tol, double exponent)
 {
     const double max_err = std::fabs(*dt * lambda * errmax);
-    *dt *= 0.9 * std::pow(tol / max_err, exponent);
+    *dt *= 0.9 * step_pow(tol / max_err, exponent);
     return max_err;
 }
 
 double tolerance_step(double dt, double err, double tol, double exponent)
 {
     if (!(err > 0.0)) return HUGE_VAL;
-    return dt * std::pow(tol / err, exponent);
+    return dt * step_pow(tol / err, exponent);
 }
 
 }  // namespace tools

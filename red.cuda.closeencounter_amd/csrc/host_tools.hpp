@@ -39,7 +39,8 @@ void calculate_orbital_element(double mu, const double* r, const double* v, doub
 int kepler_equation_solver(double ecc, double mean, double eps, double* E);             // tools.cpp:292-315
 int calculate_phase(double mu, const double* oe, double* r, double* v);                 // tools.cpp:317-359
 // Controller update of the adaptive schemes (integrator.cu:255, int_rungekutta8.cu:776-777):
-// returns |dt * lambda * errmax| and multiplies dt by 0.9 (tol / err)^exponent.
+// returns |dt * lambda * errmax| and multiplies dt by 0.9 (tol / err)^exponent, the power
+// correctly rounded (step_pow.hpp, the ensemble's device controller computes the same bits).
 double controller_update(double* dt, double lambda, double errmax, double tol, double exponent);
 // Tolerance-matching step of an attempt: dt (tol / err)^exponent (+inf when err <= 0).
 double tolerance_step(double dt, double err, double tol, double exponent);

@@ -128,6 +128,8 @@ SIGNATURES = {
     "red_ensemble_disk_events": (_I, [_P, _I, ctypes.POINTER(RedEvent), _I, _IP]),
     "red_ensemble_disk_info": (_I, [_P, _I, _IP, _DP, _DP]),
     "red_ensemble_autotune": (_I, [_P, _I, _IP, _IP, _DP]),
+    "red_ensemble_set_batch": (_I, [_P, _I]),
+    "red_step_pow": (_I, [_P, _P, _P, _I, _I, _I]),
     "red_mlp_param_count": (_I, []),
     "red_mlp_create": (_I, [_I, _P, ctypes.POINTER(_P)]),
     "red_mlp_destroy": (_I, [_P]),
@@ -523,7 +525,7 @@ def _params(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, error_ch
 class Ensemble:
     """Independent disks integrated together through red_ensemble_* (internal units)."""
 
-    def __init__(self, disks, **kw):
+    def __init__(self, disks, batch=None, **kw):
         self.L = load_library()
         self.h = ctypes.c_void_p()
         self.sizes = [d.n for d in disks]
@@ -550,6 +552,8 @@ class Ensemble:
             msg = self.L.red_ensemble_last_error(self.h).decode() if self.h else ""
             self.close()
             raise RedError(rc, msg or "red_ensemble_create failed")
+        if batch is not None:
+            self.set_batch(batch)
 
     def close(self):
         if getattr(self, "h", None) and self.h:
@@ -627,6 +631,21 @@ class Ensemble:
         b = ctypes.c_int(); j = ctypes.c_int(); ms = (ctypes.c_double * 9)()
         self._chk(self.L.red_ensemble_autotune(self.h, reps, ctypes.byref(b), ctypes.byref(j), ms))
         return b.value, j.value, np.array(list(ms)).reshape(3, 3)
+
+    def set_batch(self, supersteps):
+        self._chk(self.L.red_ensemble_set_batch(self.h, int(supersteps)))
+
+
+def step_pow(x, y, on_device=False, device=0):
+    """The step controllers' factor x^y rounded to nearest (red_step_pow), elementwise."""
+    L = load_library()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.broadcast_to(y, x.shape), dtype=np.float64)
+    out = np.empty_like(x)
+    rc = L.red_step_pow(_ptr(x), _ptr(y), _ptr(out), int(x.size), 1 if on_device else 0, int(device))
+    if rc:
+        raise RedError(rc, "red_step_pow failed")
+    return out
 
 
 class Mlp:

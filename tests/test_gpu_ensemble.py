@@ -250,3 +250,35 @@ def test_ensemble_config4_slice(native_mod, oracle_mod):
         so = o.state()
         assert_bitwise(r, so["r"], f"disk {k} vs oracle r")
         assert_bitwise(v, so["v"], f"disk {k} vs oracle v")
+
+
+@pytest.mark.parametrize("case", ["rkf78_merge", "ck_hill"])
+def test_ensemble_batch_sizes_agree(native_mod, case):
+    """The device runs each disk's controller for a batch of supersteps and holds a disk for
+    the host where its step needs one (merges, ejection scans, compactions that move bodies):
+    batches of 1, 3 and 16 supersteps give every member the same state, bodies, counters and
+    event records."""
+    from test_gpu_events import planted_disk
+    kw = EVENT_CASES[case]
+    gen = small_ensemble(n=3, seed=411)
+    for b in gen:
+        b.radius[0] = 1e-7
+    disks = [planted_disk(seed=41), planted_disk(seed=44)] + gen
+    t_stop = 20.0 * 0.01720209895
+    runs = []
+    for batch in (1, 3, 16):
+        print(f"case ensemble batch {case} batch={batch}", flush=True)
+        E = native_mod.Ensemble(disks, event_capacity=200000, batch=batch, **kw)
+        info = E.advance(t_stop)
+        assert info["active_disks"] == 0
+        runs.append([(E.state(k), E.bodies(k), E.counters(k), E.disk_info(k), E.events(k)) for k in range(len(disks))])
+        E.close()
+    from test_gpu_events import events_equal
+    for other in runs[1:]:
+        for k in range(len(disks)):
+            (r0, v0), b0, c0, i0, e0 = runs[0][k]
+            (r1, v1), b1, c1, i1, e1 = other[k]
+            assert_bitwise(r0, r1, f"disk {k} r")
+            assert_bitwise(v0, v1, f"disk {k} v")
+            assert np.array_equal(b0["id"], b1["id"]) and c0 == c1 and i0 == i1, k
+            events_equal(e0, e1, f"disk {k} events")
