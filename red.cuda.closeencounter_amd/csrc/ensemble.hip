@@ -485,11 +485,14 @@ __global__ __launch_bounds__(256) void k_ens_compact_back(const int* __restrict_
 struct EnsWords {
     double t_stop;
     int k_limit;       // supersteps of this batch that may run
-    int stop;          // set once the record list is half full: the batch's later supersteps idle
+    int stop;          // set once the kept records fill half the record capacity: the batch's
+                       // later supersteps idle
     int nan_disk;      // lowest disk with a non-finite error estimate (n_disks: none)
     int limit_disk;    // lowest disk that reached 1000 attempts in one step (n_disks: none)
-    int ev_count;      // pair records of the batch
-    int pad;
+    int ev_count;      // raw pair records of the current attempt
+    int raw_max;       // the most raw records one attempt of the batch appended
+    int kept_count;    // records kept in the batch's list (one per pair and step)
+    int lost;          // records that found no room in the pair table or the kept list
 };
 
 struct EnsCtl {
@@ -548,7 +551,13 @@ __global__ __launch_bounds__(256) void k_ens_control(EnsCtl C, EnsScratch X, int
                                                      int ce_mode, int half_cap)
 {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d == 0 && C.w->ev_count > half_cap) C.w->stop = 1;
+    if (d == 0) {
+        // the attempt's raw records are in the kept list now (k_dedup_emit): the raw list
+        // starts empty for the next attempt
+        C.w->raw_max = max(C.w->raw_max, C.w->ev_count);
+        C.w->ev_count = 0;
+        if (C.w->kept_count > half_cap) C.w->stop = 1;
+    }
     if (d >= nd || !(X.flags[d] & 1)) return;
     bool accept = true, counted = true;
     if (ADAPTIVE) {
@@ -591,6 +600,119 @@ __global__ __launch_bounds__(256) void k_ens_control(EnsCtl C, EnsScratch X, int
             atomicMin(&C.w->limit_disk, d);
             C.phase[d] = PH_HELD;
         }
+    }
+}
+
+// ---- the attempt's pair records, deduplicated on the device ---------------------------------
+// The host keeps, of each step's records, one per (survivor, merger) pair: the one of
+// smallest t, the first of equal times in canonical order (stage, sink, source), placed where
+// the pair first occurs in that order (create_sp_events, pp_disk.cu:1111-1149).  Within one
+// attempt a pair's records differ only in their stage, so three passes over the attempt's
+// raw records through a pair table (keyed by superstep, survivor and merger) find each pair's
+// smallest time, the smallest stage at that time (the kept record) and its smallest stage
+// (where it first occurs); the kept records go to the batch's list with that first stage in
+// the attempt field (superstep | first << 16), and the host only orders them by (first stage,
+// sink, source).  In close-encounter mode every record is kept (no dedupe there).
+struct EnsDedup {
+    unsigned long long* key;    // pair table, 2^bits slots; ~0: empty
+    unsigned long long* tmin;   // smallest time of the pair's records (order-preserving bits)
+    unsigned int* first;        // smallest stage of the pair's records
+    unsigned int* kept;         // smallest stage among its records at tmin
+    int bits;
+    const red_event* raw;       // the attempt's raw records
+    int raw_cap;
+    red_event* out;             // the batch's kept records
+    int out_cap;
+    const EnsDiskPair* dp;      // each disk's first body
+};
+
+constexpr unsigned long long EMPTY_KEY = ~0ull;
+
+__device__ __forceinline__ unsigned long long time_key(double t)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(t);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// superstep (6 bits) | survivor (29 bits) | merger (29 bits), global body indices; never
+// ~0, as survivor and merger differ
+__device__ __forceinline__ unsigned long long pair_key(const EnsDedup& D, const red_event& e, int s)
+{
+    const unsigned long long b0 = (unsigned long long)D.dp[e.pad].b0;
+    return ((unsigned long long)s << 58) | ((b0 + (unsigned long long)e.idx1) << 29) | (b0 + (unsigned long long)e.idx2);
+}
+
+__device__ __forceinline__ unsigned int key_hash(unsigned long long k, int bits)
+{
+    return (unsigned int)((k * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+}
+
+// The pair's slot, claimed if new; -1 when the table is full (every probe sequence ends).
+__device__ int pair_slot(const EnsDedup& D, unsigned long long k)
+{
+    const unsigned int mask = (1u << D.bits) - 1u;
+    const unsigned int h = key_hash(k, D.bits);
+    for (unsigned int p = 0; p <= mask; p++) {
+        const unsigned int i = (h + p) & mask;
+        const unsigned long long old = atomicCAS(&D.key[i], EMPTY_KEY, k);
+        if (old == EMPTY_KEY || old == k) return (int)i;
+    }
+    return -1;
+}
+
+// The pair's slot after k_dedup_insert (a read-only probe); -1 if it is not there.
+__device__ int pair_find(const EnsDedup& D, unsigned long long k)
+{
+    const unsigned int mask = (1u << D.bits) - 1u;
+    const unsigned int h = key_hash(k, D.bits);
+    for (unsigned int p = 0; p <= mask; p++) {
+        const unsigned int i = (h + p) & mask;
+        const unsigned long long q = D.key[i];
+        if (q == k) return (int)i;
+        if (q == EMPTY_KEY) return -1;
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(256) void k_dedup_insert(EnsDedup D, EnsWords* w, int s)
+{
+    const int n = min(w->ev_count, D.raw_cap);   // records past the capacity were not written
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const red_event& e = D.raw[q];
+        const int i = pair_slot(D, pair_key(D, e, s));
+        if (i < 0) { atomicAdd(&w->lost, 1); continue; }
+        atomicMin(&D.tmin[i], time_key(e.t));
+        atomicMin(&D.first[i], (unsigned int)e.stage);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dedup_mark(EnsDedup D, EnsWords* w, int s)
+{
+    const int n = min(w->ev_count, D.raw_cap);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const red_event& e = D.raw[q];
+        const int i = pair_find(D, pair_key(D, e, s));
+        if (i >= 0 && time_key(e.t) == D.tmin[i]) atomicMin(&D.kept[i], (unsigned int)e.stage);
+    }
+}
+
+template <bool ALL>
+__global__ __launch_bounds__(256) void k_dedup_emit(EnsDedup D, EnsWords* w, int s)
+{
+    const int n = min(w->ev_count, D.raw_cap);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const red_event& e = D.raw[q];
+        unsigned int first = (unsigned int)e.stage;
+        if (!ALL) {
+            const int i = pair_find(D, pair_key(D, e, s));
+            if (i < 0 || D.kept[i] != (unsigned int)e.stage) continue;
+            first = D.first[i];
+        }
+        const int k = atomicAdd(&w->kept_count, 1);
+        if (k >= D.out_cap) { atomicAdd(&w->lost, 1); continue; }
+        red_event* o = D.out + k;
+        *o = e;
+        o->attempt = s | (int)(first << 16);
     }
 }
 
@@ -678,6 +800,13 @@ struct red_ensemble {
     int* d_nan = nullptr;
     int* d_map = nullptr;                    // slot compaction map (k_ens_compact)
     int batch = 8;                           // supersteps per graph launch
+    // the attempts' records deduplicated on the device into the batch's kept list
+    EnsDedup dedup{};
+    char* d_table = nullptr;                 // the pair table, cleared before every batch
+    size_t table_bytes = 0;
+    red_event* d_kept = nullptr;
+    int kept_cap = 0;
+    int dedup_blocks = 1;
     std::vector<int> inactive;               // inactive bodies in each disk's slot (until compaction)
     StageArgs* d_rows = nullptr;             // the stage combinations' arguments ([0] y_(n+1))
     int* d_ones = nullptr;                   // padded storage: every body gates as active
@@ -910,6 +1039,22 @@ hipError_t ens_superstep(red_ensemble* E, int s)
     if (E->scan_on) {
         hipLaunchKernelGGL(k_ens_probe, gb, bb, 0, E->stream, E->outr, E->d_id, E->d_type, E->disk_of, E->d_flags,
                            E->n, E->prm.ejection, E->prm.hit_centrum, E->scr.probe);
+        RED_HIP_TRY(hipGetLastError());
+    }
+    if (E->pair_tests) {
+        // the attempt's records, one per pair (all of them in close-encounter mode), into the
+        // batch's list
+        const dim3 gq(E->dedup_blocks);
+        EnsWords* w = E->dc.w;
+        if (!E->prm.close_encounter) {
+            hipLaunchKernelGGL(k_dedup_insert, gq, bb, 0, E->stream, E->dedup, w, s);
+            RED_HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(k_dedup_mark, gq, bb, 0, E->stream, E->dedup, w, s);
+            RED_HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(k_dedup_emit<false>, gq, bb, 0, E->stream, E->dedup, w, s);
+        } else {
+            hipLaunchKernelGGL(k_dedup_emit<true>, gq, bb, 0, E->stream, E->dedup, w, s);
+        }
         RED_HIP_TRY(hipGetLastError());
     }
     const int half = E->ev_cap / 2;
@@ -1175,9 +1320,11 @@ int after_step_host(red_ensemble* E, int d, std::vector<BodyUpdate>& ups, bool* 
         return RED_OK;
     }
     if (E->pair_tests && !raw.empty()) {
-        events::canonical_sort(raw);
+        // one record per pair already (k_dedup_emit): put them where each pair first occurs
+        events::kept_order(raw);
         const size_t base = log.size();
-        events::create_sp_events(raw, log);
+        log.resize(base + raw.size());
+        for (size_t k = 0; k < raw.size(); k++) log[base + k] = *raw[k];
         stamp(base);
         events::BodyView B = E->view(d);
         const int rc = events::collision_sp_pass(B, log.data() + base, log.size() - base, E->evc[d], ups, msg);
@@ -1253,14 +1400,16 @@ int after_batch(red_ensemble* E, int ne)
     std::vector<char> comp(nd, 0);
     for_disks(ne >= 1024 ? E->host_threads : 1, list, [&](int d) {
         std::vector<const red_event*>& R = rec[d];
-        std::stable_sort(R.begin(), R.end(), [](const red_event* a, const red_event* b) { return a->attempt < b->attempt; });
+        std::stable_sort(R.begin(), R.end(), [](const red_event* a, const red_event* b) {
+            return (a->attempt & 0xffff) < (b->attempt & 0xffff);
+        });
         int last = -1;
         for (int s = 0; s < K; s++)
             if (log[s * nd + d] >= 2) last = s;
         size_t q = 0;
         for (int s = 0; s <= last; s++) {
             const size_t q0 = q;
-            while (q < R.size() && R[q]->attempt == s) q++;
+            while (q < R.size() && (R[q]->attempt & 0xffff) == s) q++;
             if (log[s * nd + d] < 2) continue;   // a rejected attempt's records are dropped
             E->pending[d].assign(R.begin() + q0, R.begin() + q);
             E->pending_attempt[d] = log[s * nd + d] - 2;
@@ -1365,6 +1514,7 @@ int red_ensemble_destroy(red_ensemble* E)
                     (void*)E->d_ctl, (void*)E->d_scr, (void*)E->d_map, (void*)E->d_ones, (void*)E->d_cscratch, (void*)E->d_rows, (void*)E->d_tiles,
                     (void*)E->d_mass,
                     (void*)E->d_radius, (void*)E->d_id, (void*)E->d_type, (void*)E->d_pair, (void*)E->d_events,
+                    (void*)E->d_kept, (void*)E->d_table,
                     (void*)E->d_updates})
         (void)hipFree(p);
     for (auto p : E->sr) (void)hipFree(p);
@@ -1431,6 +1581,7 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
         }
     }
     *out = E;
+    if (E->n >= (1 << 29)) return E->fail(RED_ERR_ARG, "more than 2^29 - 1 bodies in an ensemble");
     ENS_HIP(E, hipSetDevice(p->device));
     ENS_HIP(E, hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
     const int n = std::max(E->n, 1);
@@ -1480,6 +1631,28 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     ENS_HIP(E, dmalloc(&E->d_pair, E->nd));
     E->ev_cap = p->event_capacity > 0 ? p->event_capacity : std::max(4096, E->n);   // as a single disk's
     ENS_HIP(E, dmalloc(&E->d_events, E->ev_cap));
+    if (E->pair_tests) {
+        // a batch keeps at most half the capacity before its last attempt, which adds at most
+        // the capacity; the pair table holds twice that many keys
+        E->kept_cap = E->ev_cap + E->ev_cap / 2 + 1;
+        ENS_HIP(E, dmalloc(&E->d_kept, E->kept_cap));
+        int bits = 10;
+        while ((1ull << bits) < 2ull * (unsigned long long)E->kept_cap) bits++;
+        if (bits > 31) return E->fail(RED_ERR_ARG, "event capacity too large for the pair table");
+        const size_t slots = (size_t)1 << bits;
+        E->table_bytes = slots * (2 * sizeof(unsigned long long) + 2 * sizeof(unsigned int));
+        ENS_HIP(E, dmalloc(&E->d_table, E->table_bytes));
+        EnsDedup& D = E->dedup;
+        D.key = (unsigned long long*)E->d_table;
+        D.tmin = D.key + slots;
+        D.first = (unsigned int*)(D.tmin + slots);
+        D.kept = D.first + slots;
+        D.bits = bits;
+        D.raw = E->d_events; D.raw_cap = E->ev_cap;
+        D.out = E->d_kept; D.out_cap = E->kept_cap;
+        D.dp = E->d_pair;
+        E->dedup_blocks = std::max(1, std::min(2048, (E->ev_cap + 255) / 256));
+    }
     if (E->n > 0) {
         ENS_HIP(E, stream_copy(E->stream, E->yr, r.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
         ENS_HIP(E, stream_copy(E->stream, E->yv, v.data(), sizeof(d4) * E->n, hipMemcpyHostToDevice));
@@ -1597,8 +1770,11 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         w.nan_disk = nd;
         w.limit_disk = nd;
         w.ev_count = 0;
-        w.pad = 0;
+        w.raw_max = 0;
+        w.kept_count = 0;
+        w.lost = 0;
         ENS_HIP(E, hipMemcpyAsync(E->d_ctl, E->h_ctl, E->ctl_up, hipMemcpyHostToDevice, E->stream));
+        if (E->pair_tests) ENS_HIP(E, hipMemsetAsync(E->d_table, 0xff, E->table_bytes, E->stream));
         ENS_HIP(E, run_batch(E));
         ENS_HIP(E, hipMemcpyAsync(E->h_ctl, E->d_ctl, E->ctl_all, hipMemcpyDeviceToHost, E->stream));
         ENS_HIP(E, hipStreamSynchronize(E->stream));
@@ -1610,11 +1786,13 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         }
         if (w.nan_disk < nd) return E->fail(RED_ERR_NAN, "non-finite error estimate in disk " + std::to_string(w.nan_disk));
         if (w.limit_disk < nd) return E->fail(RED_ERR_STEP_LIMIT, "disk " + std::to_string(w.limit_disk) + " did not converge");
-        // the batch's pair records, to their disks
-        const int ne = E->pair_tests ? w.ev_count : 0;
-        if (ne > E->ev_cap)
-            return E->fail(RED_ERR_EVENT_OVERFLOW, "pair events exceed the event capacity (" + std::to_string(ne) +
-                                                       " > " + std::to_string(E->ev_cap) + ")");
+        // the batch's kept pair records, to their disks
+        if (E->pair_tests && w.raw_max > E->ev_cap)
+            return E->fail(RED_ERR_EVENT_OVERFLOW, "pair events exceed the event capacity (" +
+                                                       std::to_string(w.raw_max) + " > " + std::to_string(E->ev_cap) + ")");
+        if (E->pair_tests && w.lost > 0)
+            return E->fail(RED_ERR_EVENT_OVERFLOW, std::to_string(w.lost) + " pair records found no room in the batch's list");
+        const int ne = E->pair_tests ? w.kept_count : 0;
         if (ne > E->h_events_cap) {
             SetupLock lk;
             if (E->h_events) (void)hipHostFree(E->h_events);
@@ -1623,7 +1801,7 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
             ENS_HIP(E, hipHostMalloc((void**)&E->h_events, sizeof(red_event) * E->h_events_cap, hipHostMallocDefault));
         }
         if (ne > 0)
-            ENS_HIP(E, stream_copy(E->stream, E->h_events, E->d_events, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
+            ENS_HIP(E, stream_copy(E->stream, E->h_events, E->d_kept, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
         const int rc = after_batch(E, ne);
         if (rc) return rc;
     }

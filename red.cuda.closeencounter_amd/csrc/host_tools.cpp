@@ -249,6 +249,28 @@ void canonical_sort(std::vector<red_event>& ev)
     ev.swap(out);
 }
 
+// Records kept one per pair by the device, the stage where their pair first occurs in the
+// attempt field's upper half: ordered by (that stage, sink, source), which is the order of
+// the pairs' first records in the canonical order of one attempt.
+void kept_order(std::vector<const red_event*>& ev)
+{
+    const size_t n = ev.size();
+    if (n < 2) return;
+    std::vector<PackedKey> k(n);
+    for (size_t i = 0; i < n; i++) {
+        const red_event& e = *ev[i];
+        k[i] = PackedKey{(uint64_t)((uint32_t)e.attempt >> 16), (biased(e.sink) << 32) | biased(e.source), (uint32_t)i};
+    }
+    std::sort(k.begin(), k.end(), [](const PackedKey& a, const PackedKey& b) {
+        if (a.hi != b.hi) return a.hi < b.hi;
+        if (a.lo != b.lo) return a.lo < b.lo;
+        return a.idx < b.idx;
+    });
+    std::vector<const red_event*> out(n);
+    for (size_t i = 0; i < n; i++) out[i] = ev[k[i].idx];
+    ev.swap(out);
+}
+
 // The reference compares every record with every later one (O(E^2)); its result is, for each
 // (id1, id2) in the order of its first record, the record of smallest t, the first of equal
 // times (a later record replaces the kept one only when its t is smaller).  Sorting the

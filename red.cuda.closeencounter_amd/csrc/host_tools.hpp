@@ -82,6 +82,9 @@ bool event_key_less(const red_event& a, const red_event& b);
 void canonical_sort(std::vector<red_event>& ev);
 // The same order for a list of records held elsewhere (records stay where they are).
 void canonical_sort(std::vector<const red_event*>& ev);
+// Records the device kept one per pair (ensembles), the stage of each pair's first record in
+// the attempt field's upper 16 bits: the order create_sp_events gives them.
+void kept_order(std::vector<const red_event*>& ev);
 // pp_disk.cu:1111-1149: one record per (id1, id2) in first-occurrence order, the earliest t
 // kept (the first of equal times).
 void create_sp_events(const std::vector<red_event>& raw, std::vector<red_event>& sp);
