@@ -12,6 +12,8 @@
 //  * broadcast sweep: a work-item owns SPT sinks, the j loop is wave-uniform so every
 //    source record {x, y, z, m} arrives through one 32-byte scalar load into SGPRs and
 //    feeds all 64 lanes as a scalar operand -- no LDS traffic and no VGPRs for sources.
+//    The mixed sweep's four wavefronts of a work-group share their sinks and split the
+//    sources of the group's part, adding their sums in LDS (one partial per work-group).
 //  * LDS-tile sweep: the classic i-block x j-tile scheme, sources staged in LDS per tile
 //    (kept as an autotuner alternative).
 //  * split sweep for sweeps with few sinks (the star and giants against the whole disk):
@@ -637,6 +639,14 @@ __device__ __forceinline__ void kahan_add(double& sum, double& comp, double x)
     sum = t;
 }
 
+// A work-group's four wavefronts share 64 x SPT sinks (SPT per work-item) and split every
+// 64-source chunk of the group's part into four 16-source slices, wavefront w taking slice
+// w: they read the same source lines through the scalar cache at about the same time, and
+// the work-group writes one partial sum per sink (a quarter of the part sums that one
+// wavefront per part would write: 25 instead of 75 MB per launch on the 65533 x 65539
+// test-particle sweep, for 2 % more time).  Each wavefront keeps FP32 sums over its slices
+// of four consecutive chunks (64 pair terms per sink), Kahan-adds them into FP64, and the
+// four FP64 sums meet in LDS in wavefront order.
 template <int SPT, bool OVERLAP, bool EV, bool KAHAN>
 __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__ r, const d4* __restrict__ v,
                                                           d4* __restrict__ a, d4* __restrict__ partial,
@@ -647,16 +657,20 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
                                                           int n_fp64_src, PairTest P, EventSink E,
                                                           Ctl* __restrict__ ctl)
 {
-    const int nt = blockDim.x;
+    constexpr int NS = 64 * SPT;               // sinks per work-group
+    __shared__ double wsum[3][3][NS];          // wavefronts 1-3: x, y, z sums of every sink
+    const int lane = threadIdx.x & 63;
+    // wave-uniform, so that the source index stays scalar and sources arrive by scalar loads
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int p = blockIdx.y;
-    const int base = sw.sink0 + blockIdx.x * nt * SPT + threadIdx.x;
+    const int base = sw.sink0 + blockIdx.x * NS + lane;
     int ii[SPT];
     bool live[SPT];
     double xi[SPT], yi[SPT], zi[SPT], Ri[SPT], ax[SPT], ay[SPT], az[SPT], cx[SPT], cy[SPT], cz[SPT];
     float cut2[SPT];
 #pragma unroll
     for (int s = 0; s < SPT; s++) {
-        ii[s] = base + s * nt;
+        ii[s] = base + s * 64;
         const bool in = ii[s] < sw.sink1;
         const int k = in ? ii[s] : sw.sink0;
         const d4 q = r[k];
@@ -679,9 +693,9 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
             cut2[s] = (float)fmin(cut * cut, 3.0e38);
         }
     }
-    // FP64 head (the star and giant planets), processed by part 0
+    // FP64 head (the star and giant planets), processed by wavefront 0 of part 0
     const int jsplit = min(sw.src1, max(sw.src0, n_fp64_src));
-    if (p == 0) {
+    if (p == 0 && w == 0) {
         for (int j = sw.src0; j < jsplit; ++j) {
             const bool act = sw.gate[j] >= 0;
             const d4 q = r[j];
@@ -693,8 +707,7 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
     }
     int jlo, jhi;
     part_range(jsplit, sw.src1, parts, p, CHUNK, jlo, jhi);
-    for (int j0 = jlo; j0 < jhi; j0 += CHUNK) {
-        const int j1 = min(jhi, j0 + CHUNK);
+    for (int g0 = jlo; g0 < jhi; g0 += 4 * CHUNK) {
         float sx[SPT], sy[SPT], sz[SPT], kx[SPT], ky[SPT], kz[SPT], mn[SPT];
 #pragma unroll
         for (int s = 0; s < SPT; s++) {
@@ -702,31 +715,36 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
             kx[s] = ky[s] = kz[s] = 0.0f;
             mn[s] = 3.0e38f;
         }
+        for (int c = 0; c < 4; c++) {
+            const int j0 = g0 + c * CHUNK + w * (CHUNK / 4);
+            const int j1 = min(jhi, j0 + CHUNK / 4);
 #pragma unroll 4
-        for (int j = j0; j < j1; ++j) {
-            // one 32-byte scalar load; the low word of w is the FP32 mass (0 if inactive)
-            const d4 q = reinterpret_cast<const d4*>(src)[j];
-            const float qm = __uint_as_float((unsigned)((unsigned long long)__double_as_longlong(q.w) & 0xffffffffull));
+            for (int j = j0; j < j1; ++j) {
+                // one 32-byte scalar load; the low word of w is the FP32 mass (0 if inactive)
+                const d4 q = reinterpret_cast<const d4*>(src)[j];
+                const float qm =
+                    __uint_as_float((unsigned)((unsigned long long)__double_as_longlong(q.w) & 0xffffffffull));
 #pragma unroll
-            for (int s = 0; s < SPT; s++) {
-                const float dx = (float)(q.x - xi[s]);
-                const float dy = (float)(q.y - yi[s]);
-                const float dz = (float)(q.z - zi[s]);
-                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-                const float inv = __builtin_amdgcn_rsqf(r2);
-                float sc = (qm * inv) * (inv * inv);
-                if (OVERLAP) sc = (j == ii[s]) ? 0.0f : sc;
-                if (KAHAN) {
-                    float y, t;
-                    y = __builtin_fmaf(sc, dx, -kx[s]); t = sx[s] + y; kx[s] = (t - sx[s]) - y; sx[s] = t;
-                    y = __builtin_fmaf(sc, dy, -ky[s]); t = sy[s] + y; ky[s] = (t - sy[s]) - y; sy[s] = t;
-                    y = __builtin_fmaf(sc, dz, -kz[s]); t = sz[s] + y; kz[s] = (t - sz[s]) - y; sz[s] = t;
-                } else {
-                    sx[s] = __builtin_fmaf(sc, dx, sx[s]);
-                    sy[s] = __builtin_fmaf(sc, dy, sy[s]);
-                    sz[s] = __builtin_fmaf(sc, dz, sz[s]);
+                for (int s = 0; s < SPT; s++) {
+                    const float dx = (float)(q.x - xi[s]);
+                    const float dy = (float)(q.y - yi[s]);
+                    const float dz = (float)(q.z - zi[s]);
+                    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    float sc = (qm * inv) * (inv * inv);
+                    if (OVERLAP) sc = (j == ii[s]) ? 0.0f : sc;
+                    if (KAHAN) {
+                        float y, t;
+                        y = __builtin_fmaf(sc, dx, -kx[s]); t = sx[s] + y; kx[s] = (t - sx[s]) - y; sx[s] = t;
+                        y = __builtin_fmaf(sc, dy, -ky[s]); t = sy[s] + y; ky[s] = (t - sy[s]) - y; sy[s] = t;
+                        y = __builtin_fmaf(sc, dz, -kz[s]); t = sz[s] + y; kz[s] = (t - sz[s]) - y; sz[s] = t;
+                    } else {
+                        sx[s] = __builtin_fmaf(sc, dx, sx[s]);
+                        sy[s] = __builtin_fmaf(sc, dy, sy[s]);
+                        sz[s] = __builtin_fmaf(sc, dz, sz[s]);
+                    }
+                    if (EV) mn[s] = fminf(mn[s], (OVERLAP && j == ii[s]) ? 3.0e38f : r2);
                 }
-                if (EV) mn[s] = fminf(mn[s], (OVERLAP && j == ii[s]) ? 3.0e38f : r2);
             }
         }
 #pragma unroll
@@ -735,22 +753,45 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
             kahan_add(ay[s], cy[s], (double)sy[s]);
             kahan_add(az[s], cz[s], (double)sz[s]);
             if (EV && live[s] && mn[s] < cut2[s]) {
-                // exact FP64 re-evaluation of the reference predicate for this chunk
-                for (int j = j0; j < j1; ++j) {
-                    if (sw.gate[j] < 0 || j == ii[s]) continue;
-                    const d4 q = r[j];
-                    const double dx = q.x - xi[s];
-                    const double dy = q.y - yi[s];
-                    const double dz = q.z - zi[s];
-                    const double r2 = (dx * dx + dy * dy) + dz * dz;
-                    pair_test(P, E, ctl, ii[s], j, sqrt(r2), Ri[s], radius[j], r, v, mass, id);
+                // exact FP64 re-evaluation of the reference predicate for this wavefront's chunk
+                for (int c = 0; c < 4; c++) {
+                    const int j0 = g0 + c * CHUNK + w * (CHUNK / 4);
+                    const int j1 = min(jhi, j0 + CHUNK / 4);
+                    for (int j = j0; j < j1; ++j) {
+                        if (sw.gate[j] < 0 || j == ii[s]) continue;
+                        const d4 q = r[j];
+                        const double dx = q.x - xi[s];
+                        const double dy = q.y - yi[s];
+                        const double dz = q.z - zi[s];
+                        const double r2 = (dx * dx + dy * dy) + dz * dz;
+                        pair_test(P, E, ctl, ii[s], j, sqrt(r2), Ri[s], radius[j], r, v, mass, id);
+                    }
                 }
             }
         }
     }
+    // the four wavefronts' sums (each with its compensation) added in wavefront order
+    if (w > 0) {
 #pragma unroll
-    for (int s = 0; s < SPT; s++)
-        if (ii[s] < sw.sink1) store_acc(a, partial, parts, p, sw, ii[s], live[s], ax[s], ay[s], az[s]);
+        for (int s = 0; s < SPT; s++) {
+            wsum[w - 1][0][s * 64 + lane] = ax[s] - cx[s];
+            wsum[w - 1][1][s * 64 + lane] = ay[s] - cy[s];
+            wsum[w - 1][2][s * 64 + lane] = az[s] - cz[s];
+        }
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+        for (int s = 0; s < SPT; s++) {
+            double x = ax[s] - cx[s], y = ay[s] - cy[s], z = az[s] - cz[s];
+            for (int k = 0; k < 3; k++) {
+                x += wsum[k][0][s * 64 + lane];
+                y += wsum[k][1][s * 64 + lane];
+                z += wsum[k][2][s * 64 + lane];
+            }
+            if (ii[s] < sw.sink1) store_acc(a, partial, parts, p, sw, ii[s], live[s], x, y, z);
+        }
+    }
 }
 
 // Sums the parts of a split sweep in part order (deterministic).
@@ -972,6 +1013,9 @@ hipError_t dispatch_tile(const Sweep& sw, const GravBuffers& B, const GravConfig
     const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
     const bool ev = P.enabled != 0;
     if (C.precision == RED_PREC_MIXED) {
+        // four wavefronts per work-group on 64 x SPT sinks
+        b = dim3(256);
+        g = dim3((nsink + 64 * SPT - 1) / (64 * SPT), parts);
         if (C.kahan_pairs) {
             if (ov) return ev ? go_mixed<SPT, true, true, true>(g, b, sw, B, C, parts, P, E, s)
                               : go_mixed<SPT, true, false, true>(g, b, sw, B, C, parts, P, E, s);
@@ -1012,13 +1056,17 @@ int sweep_parts(const Sweep& sw, const GravConfig& C, size_t partial_elems)
     if (sequential) return 1;
     int parts = C.source_parts;
     if (parts <= 0) {
-        // aim for about 24 wavefronts per SIMD (256 CUs x 4 SIMDs), three rounds of resident
-        // waves: measured on the 65533 x 65539 test-particle sweep (mixed, 2 sinks per
-        // work-item), 16 parts 1.70 ms, 32 parts 1.60, 48 parts 1.57, 64 parts 1.57, 96 1.58
+        // aim for about 24 wavefronts per SIMD (256 CUs x 4 SIMDs) for one wavefront per
+        // part, 32 for the mixed sweep's four per part: measured on the 65533 x 65539
+        // test-particle sweep (mixed, 2 sinks per work-item), one wavefront per part: 16
+        // parts 1.70 ms, 32 parts 1.60, 48 parts 1.57, 64 1.57; four: 12 parts 1.605 ms,
+        // 16 parts 1.597, 24 parts 1.63
         const int spt = C.spt > 0 ? C.spt : 1;
-        const int waves = (nsink + 64 * spt - 1) / (64 * spt);
-        parts = (24 * 1024 + waves - 1) / waves;
-        parts = std::min(parts, std::max(1, nsrc / 512));
+        const int groups = (nsink + 64 * spt - 1) / (64 * spt);
+        const int per_part = C.precision == RED_PREC_MIXED ? 4 : 1;
+        const int target = C.precision == RED_PREC_MIXED ? 32 * 1024 : 24 * 1024;
+        parts = (target + groups * per_part - 1) / (groups * per_part);
+        parts = std::min(parts, std::max(1, nsrc / (512 * per_part)));
     }
     parts = std::max(1, std::min(parts, 64));
     while (parts > 1 && (size_t)parts * nsink > partial_elems) parts--;
