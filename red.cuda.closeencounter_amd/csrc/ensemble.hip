@@ -1507,9 +1507,9 @@ extern "C" {
 int red_ensemble_destroy(red_ensemble* E)
 {
     if (!E) return RED_OK;
-    SetupLock lk;
-    destroy_graph(E);
     if (E->stream) (void)hipStreamSynchronize(E->stream);
+    destroy_graph(E);
+    SetupLock lk;
     for (void* p : {(void*)E->yr, (void*)E->yv, (void*)E->outr, (void*)E->outv, (void*)E->disk_of, (void*)E->err_on,
                     (void*)E->d_ctl, (void*)E->d_scr, (void*)E->d_map, (void*)E->d_ones, (void*)E->d_cscratch, (void*)E->d_rows, (void*)E->d_tiles,
                     (void*)E->d_mass,

@@ -85,10 +85,13 @@ def test_small_disk_full_length_bit_exact(oracle_mod, native_mod, hill):
     assert len(got[0]) > 20
 
 
-def test_fused_loop_multi_tile_sweep(oracle_mod, native_mod):
+@pytest.mark.parametrize("n_pl", [1200, 1600])
+def test_fused_loop_multi_tile_sweep(oracle_mod, native_mod, n_pl):
     """A sweep that needs more than one LDS tile of sources in the fused loop: the star and
-    two giants against 1203 sources (few-sinks mode, 512 sources per tile)."""
-    b = generate.to_internal(generate.disk_small(n_pl=1200, seed=17))
+    two giants against 1203 sources (few-sinks mode, 512 sources per tile), and against 1603
+    sources, more than the 1536 pair terms of one tile (9609 interacting pairs per
+    evaluation, within the fused loop's limit)."""
+    b = generate.to_internal(generate.disk_small(n_pl=n_pl, seed=17))
     print(f"case fused multi-tile n={b.n}", flush=True)
     kw = dict(integrator=2, adaptive=False, dt_day=0.5, radii_enhance_factor=1.0)
     prm = loop_prm(0.5, 40, 10, radii_enhance_factor=1.0)
