@@ -807,6 +807,12 @@ struct red_ensemble {
     red_event* d_kept = nullptr;
     int kept_cap = 0;
     int dedup_blocks = 1;
+    // a batch's event passes left for the time the next batch runs: the disks, their records
+    // (pointers into h_events) and the batch's step log
+    std::vector<int> def_list;
+    std::vector<std::vector<const red_event*>> def_rec;
+    std::vector<int> def_log;
+    int def_k = 0;
     std::vector<int> inactive;               // inactive bodies in each disk's slot (until compaction)
     StageArgs* d_rows = nullptr;             // the stage combinations' arguments ([0] y_(n+1))
     int* d_ones = nullptr;                   // padded storage: every body gates as active
@@ -1371,71 +1377,93 @@ int hold_mode_of(const red_ensemble* E, int d)
     return E->evc[d].event[events::CNT_LAST_CLEAR] >= 8 ? 3 : 1;
 }
 
-// The checks after every accepted step of a batch (main.cu:225-255), member by member in
-// step order on up to host_threads threads: each step's records are those its accepted
-// attempt recorded (the superstep is in the record's attempt field).  A step the device did
-// not hold changes nothing on the device (no merge; a compaction, if due, keeps every body in
-// place); a held disk's last step may merge, then compact.  Merges go to the device in one
-// launch, the compactions follow.
+// The checks after disk d's accepted steps of a batch (main.cu:225-255), in step order: each
+// step's records are those its accepted attempt recorded (the superstep is in the record's
+// attempt field, R holds the disk's records of the batch).  A step the device did not hold
+// changes nothing on the device (no merge; a compaction, if due, keeps every body in place);
+// a held disk's last step may merge (ups) and then compact (*comp).
+int disk_steps_host(red_ensemble* E, int d, std::vector<const red_event*>& R, const int* log, int K,
+                    std::vector<BodyUpdate>& ups, char* comp, std::string& msg)
+{
+    const int nd = E->nd;
+    std::stable_sort(R.begin(), R.end(), [](const red_event* a, const red_event* b) {
+        return (a->attempt & 0xffff) < (b->attempt & 0xffff);
+    });
+    int last = -1;
+    for (int s = 0; s < K; s++)
+        if (log[s * nd + d] >= 2) last = s;
+    size_t q = 0;
+    for (int s = 0; s <= last; s++) {
+        const size_t q0 = q;
+        while (q < R.size() && (R[q]->attempt & 0xffff) == s) q++;
+        if (log[s * nd + d] < 2) continue;   // a rejected attempt's records are dropped
+        E->pending[d].assign(R.begin() + q0, R.begin() + q);
+        E->pending_attempt[d] = log[s * nd + d] - 2;
+        std::vector<BodyUpdate> u;
+        bool c = false;
+        const int rc = after_step_host(E, d, u, &c, msg);
+        if (rc) return rc;
+        const bool held = s == last && E->phase[d] == PH_HELD;
+        if (!u.empty() && !held) { msg = "a merge in a step the device did not hold"; return RED_ERR_ARG; }
+        if (!u.empty()) ups = std::move(u);
+        if (c) {
+            if (count_inactive(E, d) == 0) {
+                E->evc[d].set(events::CNT_LAST_CLEAR, 0);   // the rebuild keeps every body in place
+            } else if (held) {
+                *comp = 1;
+            } else {
+                msg = "a compaction in a step the device did not hold";
+                return RED_ERR_ARG;
+            }
+        }
+    }
+    return RED_OK;
+}
+
+// After a batch: the disks whose state the host changes before they step again -- held by the
+// device, or starting their next step with an ejection scan -- have their event passes now,
+// then every merge's device update in one launch and the compactions.  The other disks'
+// passes only add records and counts on the host: they are kept for after_batch_deferred,
+// which runs while the next batch is on the device.
 int after_batch(red_ensemble* E, int ne)
 {
     const int nd = E->nd, K = E->batch;
-    const int* log = E->hc.steplog;
     const auto t_ev = std::chrono::steady_clock::now();
-    std::vector<int> list;
-    for (int d = 0; d < nd; d++)
-        for (int s = 0; s < K; s++)
-            if (log[s * nd + d] >= 2) { list.push_back(d); break; }
-    if (list.empty()) return RED_OK;
-    // records to their disks, each disk's in superstep order
-    std::vector<std::vector<const red_event*>> rec(nd);
+    E->def_log.assign(E->hc.steplog, E->hc.steplog + (size_t)K * nd);
+    E->def_k = K;
+    const int* log = E->def_log.data();
+    std::vector<int> now;
+    E->def_list.clear();
+    for (int d = 0; d < nd; d++) {
+        bool stepped = false;
+        for (int s = 0; s < K && !stepped; s++) stepped = log[s * nd + d] >= 2;
+        if (!stepped) continue;
+        if (E->phase[d] == PH_HELD || (E->scan_on && E->need_scan[d])) now.push_back(d);
+        else E->def_list.push_back(d);
+    }
+    // records to their disks
+    for (int d = 0; d < nd; d++) E->def_rec[d].clear();
     for (int q = 0; q < ne; q++) {
         const red_event& e = E->h_events[q];
         if (E->S.clear_events_each_attempt && e.stage == 0) continue;   // f0's records, cleared with the attempt
-        rec[e.pad].push_back(&e);
+        E->def_rec[e.pad].push_back(&e);
+    }
+    if (now.empty()) {
+        E->event_s += seconds_since(t_ev);
+        return RED_OK;
     }
     std::vector<std::vector<BodyUpdate>> ups(nd);
     std::vector<int> rcs(nd, RED_OK);
     std::vector<std::string> msgs(nd);
     std::vector<char> comp(nd, 0);
-    for_disks(ne >= 1024 ? E->host_threads : 1, list, [&](int d) {
-        std::vector<const red_event*>& R = rec[d];
-        std::stable_sort(R.begin(), R.end(), [](const red_event* a, const red_event* b) {
-            return (a->attempt & 0xffff) < (b->attempt & 0xffff);
-        });
-        int last = -1;
-        for (int s = 0; s < K; s++)
-            if (log[s * nd + d] >= 2) last = s;
-        size_t q = 0;
-        for (int s = 0; s <= last; s++) {
-            const size_t q0 = q;
-            while (q < R.size() && (R[q]->attempt & 0xffff) == s) q++;
-            if (log[s * nd + d] < 2) continue;   // a rejected attempt's records are dropped
-            E->pending[d].assign(R.begin() + q0, R.begin() + q);
-            E->pending_attempt[d] = log[s * nd + d] - 2;
-            std::vector<BodyUpdate> u;
-            bool c = false;
-            const int rc = after_step_host(E, d, u, &c, msgs[d]);
-            if (rc) { rcs[d] = rc; return; }
-            const bool held = s == last && E->phase[d] == PH_HELD;
-            if (!u.empty() && !held) { rcs[d] = RED_ERR_ARG; msgs[d] = "a merge in a step the device did not hold"; return; }
-            if (!u.empty()) ups[d] = std::move(u);
-            if (c) {
-                if (count_inactive(E, d) == 0) {
-                    E->evc[d].set(events::CNT_LAST_CLEAR, 0);   // the rebuild keeps every body in place
-                } else if (held) {
-                    comp[d] = 1;
-                } else {
-                    rcs[d] = RED_ERR_ARG;
-                    msgs[d] = "a compaction in a step the device did not hold";
-                    return;
-                }
-            }
-        }
+    size_t records = 0;
+    for (int d : now) records += E->def_rec[d].size();
+    for_disks(records >= 1024 ? E->host_threads : 1, now, [&](int d) {
+        rcs[d] = disk_steps_host(E, d, E->def_rec[d], log, K, ups[d], &comp[d], msgs[d]);
     });
     std::vector<BodyUpdate> all;
     std::vector<int> merged;
-    for (int d : list) {
+    for (int d : now) {
         if (rcs[d]) return E->fail(rcs[d], "disk " + std::to_string(d) + ": " + msgs[d]);
         if (!ups[d].empty()) {
             merged.push_back(d);
@@ -1473,6 +1501,31 @@ int after_batch(red_ensemble* E, int ne)
     if (rc) return rc;
     for (int d : compacted) E->inactive[d] = count_inactive(E, d);
     E->rebuild_s += seconds_since(t_rb);
+    return RED_OK;
+}
+
+// The event passes after_batch left: host records and counts only, run while the next batch
+// is on the device (they read the batch's records and step log from host copies).
+int after_batch_deferred(red_ensemble* E)
+{
+    if (E->def_list.empty()) return RED_OK;
+    const auto t_ev = std::chrono::steady_clock::now();
+    const int nd = E->nd;
+    std::vector<std::vector<BodyUpdate>> ups(nd);
+    std::vector<int> rcs(nd, RED_OK);
+    std::vector<std::string> msgs(nd);
+    std::vector<char> comp(nd, 0);
+    size_t records = 0;
+    for (int d : E->def_list) records += E->def_rec[d].size();
+    for_disks(records >= 1024 ? E->host_threads : 1, E->def_list, [&](int d) {
+        rcs[d] = disk_steps_host(E, d, E->def_rec[d], E->def_log.data(), E->def_k, ups[d], &comp[d], msgs[d]);
+    });
+    for (int d : E->def_list) {
+        if (!rcs[d] && (!ups[d].empty() || comp[d])) { rcs[d] = RED_ERR_ARG; msgs[d] = "a merge or compaction left for later"; }
+        if (rcs[d]) return E->fail(rcs[d], "disk " + std::to_string(d) + ": " + msgs[d]);
+    }
+    E->def_list.clear();
+    E->event_s += seconds_since(t_ev);
     return RED_OK;
 }
 
@@ -1677,6 +1730,7 @@ int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* 
     for (int d = 0; d < E->nd; d++) E->inactive[d] = count_inactive(E, d);
     E->evc.assign(E->nd, events::Counters());
     E->pending.assign(E->nd, {});
+    E->def_rec.assign(E->nd, {});
     E->pending_attempt.assign(E->nd, 0);
     E->log.assign(E->nd, {});
     E->compact_due.assign(E->nd, 0);
@@ -1726,6 +1780,11 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
     E->event_s = 0.0;
     E->rebuild_s = 0.0;
     const int nd = E->nd;
+    {
+        // passes a failed call left
+        const int rc = after_batch_deferred(E);
+        if (rc) return rc;
+    }
     // a disk done against an earlier t_stop continues (a run to t1, then to t2, steps as one
     // run to t2); a disk left in a retry by max_attempts resumes it
     for (int d = 0; d < nd; d++)
@@ -1777,6 +1836,11 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         if (E->pair_tests) ENS_HIP(E, hipMemsetAsync(E->d_table, 0xff, E->table_bytes, E->stream));
         ENS_HIP(E, run_batch(E));
         ENS_HIP(E, hipMemcpyAsync(E->h_ctl, E->d_ctl, E->ctl_all, hipMemcpyDeviceToHost, E->stream));
+        {
+            // the previous batch's remaining event passes, while this batch runs
+            const int rc = after_batch_deferred(E);
+            if (rc) return rc;
+        }
         ENS_HIP(E, hipStreamSynchronize(E->stream));
         unpack_ctl(E);
         for (int s = 0; s < E->batch; s++) {
@@ -1803,6 +1867,10 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         if (ne > 0)
             ENS_HIP(E, stream_copy(E->stream, E->h_events, E->d_kept, sizeof(red_event) * ne, hipMemcpyDeviceToHost));
         const int rc = after_batch(E, ne);
+        if (rc) return rc;
+    }
+    {
+        const int rc = after_batch_deferred(E);
         if (rc) return rc;
     }
     // apply the last accepted results so that the state buffers hold y
