@@ -60,8 +60,79 @@ struct EnsDiskPair {
     double rmax;                   // largest body radius (screen)
     double mmax;                   // largest non-star mass (screen)
     int n_star;                    // bodies [b0, b0 + n_star) are stars: no Hill test with them
-    int b0;                        // the disk's first body
+    int b0, b1;                    // the disk's bodies
 };
+
+// ---- the attempt's pair records, deduplicated on the device ---------------------------------
+// The host keeps, of each step's records, one per (survivor, merger) pair: the one of
+// smallest t, the first of equal times in canonical order (stage, sink, source), placed where
+// the pair first occurs in that order (create_sp_events, pp_disk.cu:1111-1149).  Within one
+// attempt a pair's records differ only in their stage, so three passes over the attempt's
+// raw records through a pair table (keyed by superstep, survivor and merger) find each pair's
+// smallest time, the smallest stage at that time (the kept record) and its smallest stage
+// (where it first occurs); the kept records go to the batch's list with that first stage in
+// the attempt field (superstep | first << 16), and the host only orders them by (first stage,
+// sink, source).  In close-encounter mode every record is kept (no dedupe there).
+struct EnsDedup {
+    unsigned long long* key;    // pair table, 2^bits slots; ~0: empty
+    unsigned long long* tmin;   // smallest time of the pair's records (order-preserving bits)
+    unsigned int* first;        // smallest stage of the pair's records
+    unsigned int* kept;         // smallest stage among its records at tmin
+    int bits;
+    const red_event* raw;       // the attempt's raw records
+    int raw_cap;
+    red_event* out;             // the batch's kept records
+    int out_cap;
+    const EnsDiskPair* dp;      // each disk's first body
+};
+
+constexpr unsigned long long EMPTY_KEY = ~0ull;
+
+__device__ __forceinline__ unsigned long long time_key(double t)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(t);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// superstep (6 bits) | survivor (29 bits) | merger (29 bits), global body indices; never
+// ~0, as survivor and merger differ
+__device__ __forceinline__ unsigned long long pair_key(const EnsDedup& D, const red_event& e, int s)
+{
+    const unsigned long long b0 = (unsigned long long)D.dp[e.pad].b0;
+    return ((unsigned long long)s << 58) | ((b0 + (unsigned long long)e.idx1) << 29) | (b0 + (unsigned long long)e.idx2);
+}
+
+__device__ __forceinline__ unsigned int key_hash(unsigned long long k, int bits)
+{
+    return (unsigned int)((k * 0x9E3779B97F4A7C15ull) >> (64 - bits));
+}
+
+// The pair's slot, claimed if new; -1 when the table is full (every probe sequence ends).
+__device__ int pair_slot(const EnsDedup& D, unsigned long long k)
+{
+    const unsigned int mask = (1u << D.bits) - 1u;
+    const unsigned int h = key_hash(k, D.bits);
+    for (unsigned int p = 0; p <= mask; p++) {
+        const unsigned int i = (h + p) & mask;
+        const unsigned long long old = atomicCAS(&D.key[i], EMPTY_KEY, k);
+        if (old == EMPTY_KEY || old == k) return (int)i;
+    }
+    return -1;
+}
+
+// The pair's slot after k_dedup_insert (a read-only probe); -1 if it is not there.
+__device__ int pair_find(const EnsDedup& D, unsigned long long k)
+{
+    const unsigned int mask = (1u << D.bits) - 1u;
+    const unsigned int h = key_hash(k, D.bits);
+    for (unsigned int p = 0; p <= mask; p++) {
+        const unsigned int i = (h + p) & mask;
+        const unsigned long long q = D.key[i];
+        if (q == k) return (int)i;
+        if (q == EMPTY_KEY) return -1;
+    }
+    return -1;
+}
 
 // What a force launch needs for the pair tests of every disk.
 struct EnsPair {
@@ -140,9 +211,22 @@ __device__ void ens_pair_test(const EnsPair& P, int disk, int i, int j, double d
     } else {
         name = RED_EV_CLOSE_ENCOUNTER;
     }
-    atomicAdd(&P.rec[disk], 1);
-    if (name == RED_EV_COLLISION) atomicAdd(&P.rec_col[disk], 1);
-    const int k = atomicAdd(P.count, 1);
+    // one set of atomics per wavefront for the lanes that record here together (the lanes of a
+    // wavefront share the disk: a tile holds one disk's sinks)
+    const unsigned long long act = __ballot(1);
+    const unsigned long long colm = __ballot(name == RED_EV_COLLISION);
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)act) - 1;
+    const int rank = __popcll(act & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (lane == leader) {
+        const int cnt = __popcll(act);
+        atomicAdd(&P.rec[disk], cnt);
+        const int nc = __popcll(colm);
+        if (nc) atomicAdd(&P.rec_col[disk], nc);
+        base = atomicAdd(P.count, cnt);
+    }
+    const int k = __shfl(base, leader) + rank;
     if (k >= P.cap) return;   // counted but dropped: the host reports an overflow
     red_event* e = P.buf + k;
     e->name = name;
@@ -172,11 +256,17 @@ struct EnsEpi {
     int next, next_form;     // stage state formed after this evaluation (0: none)
     int np1, np1_form;       // y_(n+1) formed after it (np1 != 0)
     const d4* cur_a;         // this evaluation's accelerations, whose values are in registers
+    // the ejection / hit-centrum predicate on y_(n+1) (pp_disk.cu:72-142), with np1: one flag
+    // per disk when an active non-star body meets it
+    int probe_on;
+    double ej, hc;
+    const int *id, *type;
+    int* probe;
 };
 
 // One row for body i; the term that is this evaluation's acceleration comes from `cur`.
 template <int FORM>
-__device__ __forceinline__ void epi_row_f(const StageArgs& S, int i, double dt, const d4* cur_a, d4 cur)
+__device__ __forceinline__ d4 epi_row_f(const StageArgs& S, int i, double dt, const d4* cur_a, d4 cur)
 {
     const d4 yr = S.yr[i];
     const d4 yv = S.yv[i];
@@ -186,23 +276,28 @@ __device__ __forceinline__ void epi_row_f(const StageArgs& S, int i, double dt, 
     ovv.w = 0.0;
     S.outr[i] = orr;
     S.outv[i] = ovv;
+    return orr;
 }
 
-__device__ __forceinline__ void epi_row(int form, const StageArgs& S, int i, double dt, const d4* cur_a, d4 cur)
+__device__ __forceinline__ d4 epi_row(int form, const StageArgs& S, int i, double dt, const d4* cur_a, d4 cur)
 {
     switch (form) {
-    case SF_AXPY: epi_row_f<SF_AXPY>(S, i, dt, cur_a, cur); break;
-    case SF_DTSUM: epi_row_f<SF_DTSUM>(S, i, dt, cur_a, cur); break;
-    case SF_ACCUM: epi_row_f<SF_ACCUM>(S, i, dt, cur_a, cur); break;
-    case SF_RK4NP1: epi_row_f<SF_RK4NP1>(S, i, dt, cur_a, cur); break;
-    default: epi_row_f<SF_RKF78NP1>(S, i, dt, cur_a, cur); break;
+    case SF_AXPY: return epi_row_f<SF_AXPY>(S, i, dt, cur_a, cur);
+    case SF_DTSUM: return epi_row_f<SF_DTSUM>(S, i, dt, cur_a, cur);
+    case SF_ACCUM: return epi_row_f<SF_ACCUM>(S, i, dt, cur_a, cur);
+    case SF_RK4NP1: return epi_row_f<SF_RK4NP1>(S, i, dt, cur_a, cur);
+    default: return epi_row_f<SF_RKF78NP1>(S, i, dt, cur_a, cur);
     }
 }
 
-__device__ __forceinline__ void ens_epilogue(const EnsEpi& X, int i, double dt, d4 cur)
+__device__ __forceinline__ void ens_epilogue(const EnsEpi& X, int i, int disk, double dt, d4 cur)
 {
-    if (X.next > 0) epi_row(X.next_form, X.rows[X.next], i, dt, X.cur_a, cur);
-    if (X.np1) epi_row(X.np1_form, X.rows[0], i, dt, X.cur_a, cur);
+    if (X.next > 0) (void)epi_row(X.next_form, X.rows[X.next], i, dt, X.cur_a, cur);
+    if (X.np1) {
+        const d4 o = epi_row(X.np1_form, X.rows[0], i, dt, X.cur_a, cur);
+        if (X.probe_on && X.id[i] > 0 && X.type[i] != 0 && ejection_name(o, X.ej, X.hc) != RED_EV_NONE)
+            atomicOr(&X.probe[disk], 1);
+    }
 }
 
 // Disks in the attempt (bit 0); stage 0's forces only for disks starting a new step (bit 1):
@@ -404,22 +499,7 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
             }
         }
     }
-    if (epi && ei >= 0) ens_epilogue(X, ei, P.disk_dt[T.disk], ecur);
-}
-
-// The ejection / hit-centrum predicate on each attempt's y_(n+1): one flag per disk in the
-// attempt (bit 0) when an active non-star body meets it.
-__global__ __launch_bounds__(256) void k_ens_probe(const d4* __restrict__ outr, const int* __restrict__ id,
-                                                   const int* __restrict__ type, const int* __restrict__ disk_of,
-                                                   const int* __restrict__ flags, int n, double ej, double hc,
-                                                   int* __restrict__ probe)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int d = disk_of[i];
-    if (d < 0 || !(flags[d] & 1)) return;
-    if (!(id[i] > 0 && type[i] != 0)) return;
-    if (ejection_name(outr[i], ej, hc) != RED_EV_NONE) atomicOr(&probe[d], 1);
+    if (epi && ei >= 0) ens_epilogue(X, ei, T.disk, P.disk_dt[T.disk], ecur);
 }
 
 // y <- y_(n+1) for the bodies of disks whose last attempt was accepted (bit 2)
@@ -539,41 +619,35 @@ __global__ __launch_bounds__(256) void k_ens_begin(EnsCtl C, EnsScratch X, int s
     C.steplog[s * nd + d] = 0;
 }
 
-// The controller after superstep s's attempt (integrator.cu:255, int_rungekutta8.cu:776-797,
-// int_rungekutta4.cu:246-273 and the fixed-step loops), with the host controller's
-// arithmetic (tools::controller_update: the same operations and step_pow).  An accepted step
-// is held for the host when its records need a merge (collisions), or may lead to a
-// compaction that moves bodies (hold_mode, set by the host: bit 0 the slot holds inactive
-// bodies, so any record may complete the 8 events; bit 1 the 8 are already reached).
-template <bool ADAPTIVE>
-__global__ __launch_bounds__(256) void k_ens_control(EnsCtl C, EnsScratch X, int s, int nd, double lambda,
-                                                     double tol, double exponent, int accept_on_event,
-                                                     int ce_mode, int half_cap)
+struct EnsCtlArgs {
+    double lambda, tol, exponent;
+    int adaptive, accept_on_event, ce_mode, half_cap;
+};
+
+// The controller of disk d after superstep s's attempt (integrator.cu:255,
+// int_rungekutta8.cu:776-797, int_rungekutta4.cu:246-273 and the fixed-step loops), with the
+// host controller's arithmetic (tools::controller_update: the same operations and step_pow).
+// An accepted step is held for the host when its records need a merge (collisions), or may
+// lead to a compaction that moves bodies (hold_mode, set by the host: bit 0 the slot holds
+// inactive bodies, so any record may complete the 8 events; bit 1 the 8 are already reached).
+__device__ void control_disk(const EnsCtl& C, const EnsScratch& X, int s, int nd, int d, const EnsCtlArgs& A)
 {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d == 0) {
-        // the attempt's raw records are in the kept list now (k_dedup_emit): the raw list
-        // starts empty for the next attempt
-        C.w->raw_max = max(C.w->raw_max, C.w->ev_count);
-        C.w->ev_count = 0;
-        if (C.w->kept_count > half_cap) C.w->stop = 1;
-    }
-    if (d >= nd || !(X.flags[d] & 1)) return;
+    if (!(X.flags[d] & 1)) return;
     bool accept = true, counted = true;
-    if (ADAPTIVE) {
+    if (A.adaptive) {
         if (X.nan[d]) {
             atomicMin(&C.w->nan_disk, d);
             C.phase[d] = PH_HELD;
             return;
         }
         const double m = __longlong_as_double((long long)X.err[d]);
-        const double max_err = fabs(C.dt_try[d] * lambda * m);
-        C.dt_try[d] = C.dt_try[d] * (0.9 * step_pow(tol / max_err, exponent));
+        const double max_err = fabs(C.dt_try[d] * A.lambda * m);
+        C.dt_try[d] = C.dt_try[d] * (0.9 * step_pow(A.tol / max_err, A.exponent));
         C.last_err[d] = max_err;
-        accept = !(max_err > tol);
+        accept = !(max_err > A.tol);
         // RK4 and Cash-Karp accept an attempt that recorded pair events and leave the loop
         // before counting it (int_rungekutta4.cu:255-264)
-        if (accept_on_event && X.rec[d] > 0) {
+        if (A.accept_on_event && X.rec[d] > 0) {
             if (C.dt_try[d] < C.dt_did[d]) C.dt_try[d] = C.dt_did[d];
             accept = true;
             counted = false;
@@ -591,7 +665,7 @@ __global__ __launch_bounds__(256) void k_ens_control(EnsCtl C, EnsScratch X, int
         C.need_scan[d] = X.probe[d] ? 1 : 0;
         C.steplog[s * nd + d] = 2 + stamp;
         const int hm = C.hold_mode[d];
-        const bool hold = (!ce_mode && X.rec_col[d] > 0) || (hm & 2) || ((hm & 1) && X.rec[d] > 0);
+        const bool hold = (!A.ce_mode && X.rec_col[d] > 0) || (hm & 2) || ((hm & 1) && X.rec[d] > 0);
         C.phase[d] = hold ? PH_HELD : PH_NEW;
     } else {
         C.phase[d] = PH_RETRY;
@@ -603,75 +677,20 @@ __global__ __launch_bounds__(256) void k_ens_control(EnsCtl C, EnsScratch X, int
     }
 }
 
-// ---- the attempt's pair records, deduplicated on the device ---------------------------------
-// The host keeps, of each step's records, one per (survivor, merger) pair: the one of
-// smallest t, the first of equal times in canonical order (stage, sink, source), placed where
-// the pair first occurs in that order (create_sp_events, pp_disk.cu:1111-1149).  Within one
-// attempt a pair's records differ only in their stage, so three passes over the attempt's
-// raw records through a pair table (keyed by superstep, survivor and merger) find each pair's
-// smallest time, the smallest stage at that time (the kept record) and its smallest stage
-// (where it first occurs); the kept records go to the batch's list with that first stage in
-// the attempt field (superstep | first << 16), and the host only orders them by (first stage,
-// sink, source).  In close-encounter mode every record is kept (no dedupe there).
-struct EnsDedup {
-    unsigned long long* key;    // pair table, 2^bits slots; ~0: empty
-    unsigned long long* tmin;   // smallest time of the pair's records (order-preserving bits)
-    unsigned int* first;        // smallest stage of the pair's records
-    unsigned int* kept;         // smallest stage among its records at tmin
-    int bits;
-    const red_event* raw;       // the attempt's raw records
-    int raw_cap;
-    red_event* out;             // the batch's kept records
-    int out_cap;
-    const EnsDiskPair* dp;      // each disk's first body
-};
-
-constexpr unsigned long long EMPTY_KEY = ~0ull;
-
-__device__ __forceinline__ unsigned long long time_key(double t)
+// The attempt's raw records are in the kept list once the attempt ends: the raw list starts
+// empty for the next one; a batch whose kept records passed half the capacity idles on.
+__device__ void control_words(const EnsCtl& C, int kept, const EnsCtlArgs& A)
 {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(t);
-    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+    C.w->raw_max = max(C.w->raw_max, C.w->ev_count);
+    C.w->ev_count = 0;
+    if (kept > A.half_cap) C.w->stop = 1;
 }
 
-// superstep (6 bits) | survivor (29 bits) | merger (29 bits), global body indices; never
-// ~0, as survivor and merger differ
-__device__ __forceinline__ unsigned long long pair_key(const EnsDedup& D, const red_event& e, int s)
+__global__ __launch_bounds__(256) void k_ens_control(EnsCtl C, EnsScratch X, int s, int nd, EnsCtlArgs A)
 {
-    const unsigned long long b0 = (unsigned long long)D.dp[e.pad].b0;
-    return ((unsigned long long)s << 58) | ((b0 + (unsigned long long)e.idx1) << 29) | (b0 + (unsigned long long)e.idx2);
-}
-
-__device__ __forceinline__ unsigned int key_hash(unsigned long long k, int bits)
-{
-    return (unsigned int)((k * 0x9E3779B97F4A7C15ull) >> (64 - bits));
-}
-
-// The pair's slot, claimed if new; -1 when the table is full (every probe sequence ends).
-__device__ int pair_slot(const EnsDedup& D, unsigned long long k)
-{
-    const unsigned int mask = (1u << D.bits) - 1u;
-    const unsigned int h = key_hash(k, D.bits);
-    for (unsigned int p = 0; p <= mask; p++) {
-        const unsigned int i = (h + p) & mask;
-        const unsigned long long old = atomicCAS(&D.key[i], EMPTY_KEY, k);
-        if (old == EMPTY_KEY || old == k) return (int)i;
-    }
-    return -1;
-}
-
-// The pair's slot after k_dedup_insert (a read-only probe); -1 if it is not there.
-__device__ int pair_find(const EnsDedup& D, unsigned long long k)
-{
-    const unsigned int mask = (1u << D.bits) - 1u;
-    const unsigned int h = key_hash(k, D.bits);
-    for (unsigned int p = 0; p <= mask; p++) {
-        const unsigned int i = (h + p) & mask;
-        const unsigned long long q = D.key[i];
-        if (q == k) return (int)i;
-        if (q == EMPTY_KEY) return -1;
-    }
-    return -1;
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d == 0) control_words(C, C.w->kept_count, A);
+    if (d < nd) control_disk(C, X, s, nd, d, A);
 }
 
 __global__ __launch_bounds__(256) void k_dedup_insert(EnsDedup D, EnsWords* w, int s)
@@ -696,6 +715,7 @@ __global__ __launch_bounds__(256) void k_dedup_mark(EnsDedup D, EnsWords* w, int
     }
 }
 
+// The kept records into the batch's list.
 template <bool ALL>
 __global__ __launch_bounds__(256) void k_dedup_emit(EnsDedup D, EnsWords* w, int s)
 {
@@ -927,6 +947,7 @@ EnsDiskPair pair_consts(red_ensemble* E, int d)
     P.mmax = mns;
     P.n_star = E->cnt[d][0];
     P.b0 = E->b0[d];
+    P.b1 = E->b1[d];
     return P;
 }
 
@@ -997,6 +1018,10 @@ hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v, int s
     X.rows = E->d_rows;
     X.cur_a = E->acc[stage];
     X.next = 0; X.next_form = 0; X.np1 = 0; X.np1_form = 0;
+    X.probe_on = 0;
+    X.ej = E->prm.ejection; X.hc = E->prm.hit_centrum;
+    X.id = E->d_id; X.type = E->d_type;
+    X.probe = E->scr.probe;
     if (epilogue) {
         const Scheme& S = E->S;
         const bool adaptive = E->prm.adaptive && S.err_form >= 0;
@@ -1008,6 +1033,7 @@ hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v, int s
         if (stage == S.n_main - 1) {
             X.np1 = 1;
             X.np1_form = S.np1.form;
+            X.probe_on = E->scan_on ? 1 : 0;
         }
     }
     // RKF7(8) clears the records at every attempt start (int_rungekutta8.cu:730-733), so f0's
@@ -1026,12 +1052,14 @@ hipError_t ens_force(red_ensemble* E, int stage, const d4* r, const d4* v, int s
 hipError_t ens_superstep(red_ensemble* E, int s)
 {
     const Scheme& S = E->S;
-    const dim3 gb((E->n + 255) / 256), bb(256), gd((E->nd + 255) / 256);
+    const dim3 bb(256), gd((E->nd + 255) / 256);
     hipLaunchKernelGGL(k_ens_begin, gd, bb, 0, E->stream, E->dc, E->scr, s, E->nd, E->scan_on ? 1 : 0);
     RED_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_ens_accept, gb, bb, 0, E->stream, E->yr, E->yv, E->outr, E->outv, E->disk_of, E->d_flags, E->n);
+    hipLaunchKernelGGL(k_ens_accept, dim3((E->n + 255) / 256), bb, 0, E->stream, E->yr, E->yv, E->outr, E->outv,
+                       E->disk_of, E->d_flags, E->n);
     RED_HIP_TRY(hipGetLastError());
-    // each force launch forms the next stage state (and y_(n+1)) in its epilogue
+    // each force launch forms the next stage state (and y_(n+1), with the ejection predicate)
+    // in its epilogue
     RED_HIP_TRY(ens_force(E, 0, E->yr, E->yv, s));
     for (int r = 1; r < S.n_main; r++) RED_HIP_TRY(ens_force(E, r, E->sr[r], E->sv[r], s));
     const bool adaptive = E->prm.adaptive && S.err_form >= 0;
@@ -1042,35 +1070,29 @@ hipError_t ens_superstep(red_ensemble* E, int s)
         }
         RED_HIP_TRY(launch_err(S.err_form, ens_args(E, S.err, nullptr, nullptr), E->stream));
     }
-    if (E->scan_on) {
-        hipLaunchKernelGGL(k_ens_probe, gb, bb, 0, E->stream, E->outr, E->d_id, E->d_type, E->disk_of, E->d_flags,
-                           E->n, E->prm.ejection, E->prm.hit_centrum, E->scr.probe);
-        RED_HIP_TRY(hipGetLastError());
-    }
+    EnsCtlArgs A;
+    A.lambda = S.lambda; A.tol = E->prm.tolerance; A.exponent = S.exponent;
+    A.adaptive = adaptive ? 1 : 0;
+    A.accept_on_event = S.accept_on_event ? 1 : 0;
+    A.ce_mode = E->prm.close_encounter ? 1 : 0;
+    A.half_cap = E->ev_cap / 2;
     if (E->pair_tests) {
         // the attempt's records, one per pair (all of them in close-encounter mode), into the
         // batch's list
         const dim3 gq(E->dedup_blocks);
-        EnsWords* w = E->dc.w;
         if (!E->prm.close_encounter) {
-            hipLaunchKernelGGL(k_dedup_insert, gq, bb, 0, E->stream, E->dedup, w, s);
+            hipLaunchKernelGGL(k_dedup_insert, gq, bb, 0, E->stream, E->dedup, E->dc.w, s);
             RED_HIP_TRY(hipGetLastError());
-            hipLaunchKernelGGL(k_dedup_mark, gq, bb, 0, E->stream, E->dedup, w, s);
+            hipLaunchKernelGGL(k_dedup_mark, gq, bb, 0, E->stream, E->dedup, E->dc.w, s);
             RED_HIP_TRY(hipGetLastError());
-            hipLaunchKernelGGL(k_dedup_emit<false>, gq, bb, 0, E->stream, E->dedup, w, s);
+            hipLaunchKernelGGL(k_dedup_emit<false>, gq, bb, 0, E->stream, E->dedup, E->dc.w, s);
         } else {
-            hipLaunchKernelGGL(k_dedup_emit<true>, gq, bb, 0, E->stream, E->dedup, w, s);
+            hipLaunchKernelGGL(k_dedup_emit<true>, gq, bb, 0, E->stream, E->dedup, E->dc.w, s);
         }
         RED_HIP_TRY(hipGetLastError());
     }
-    const int half = E->ev_cap / 2;
-    const int ce = E->prm.close_encounter ? 1 : 0, aoe = S.accept_on_event ? 1 : 0;
-    if (adaptive)
-        hipLaunchKernelGGL(k_ens_control<true>, gd, bb, 0, E->stream, E->dc, E->scr, s, E->nd, S.lambda,
-                           E->prm.tolerance, S.exponent, aoe, ce, half);
-    else
-        hipLaunchKernelGGL(k_ens_control<false>, gd, bb, 0, E->stream, E->dc, E->scr, s, E->nd, S.lambda,
-                           E->prm.tolerance, S.exponent, aoe, ce, half);
+    // one work-item per disk (64 per work-group: the controllers spread over more CUs)
+    hipLaunchKernelGGL(k_ens_control, dim3((E->nd + 63) / 64), dim3(64), 0, E->stream, E->dc, E->scr, s, E->nd, A);
     return hipGetLastError();
 }
 

@@ -27,8 +27,10 @@
 
 #if defined(__HIPCC__)
 #define STEP_POW_FN __host__ __device__ inline
+#define STEP_POW_UNROLL _Pragma("unroll")
 #else
 #define STEP_POW_FN inline
+#define STEP_POW_UNROLL
 #endif
 
 namespace red {
@@ -116,8 +118,53 @@ STEP_POW_FN dd div(dd a, dd b)
     return add(fast_two_sum(q1, q2), dd{q3, 0.0});
 }
 
-// 1 / n as a double-double (n a small integer)
-STEP_POW_FN dd recip(double n) { return div(dd{1.0, 0.0}, dd{n, 0.0}); }
+// 1/(2k+1) (k = 0..24) and 1/i! (i = 0..15) as double-doubles: hi is the double nearest the
+// value, lo the double nearest the rest (gen_step_pow_constants.py prints them).
+constexpr double INV_ODD[25][2] = {
+    {0x1.0000000000000p+0, 0x0.0p+0},
+    {0x1.5555555555555p-2, 0x1.5555555555555p-56},
+    {0x1.999999999999ap-3, -0x1.999999999999ap-57},
+    {0x1.2492492492492p-3, 0x1.2492492492492p-57},
+    {0x1.c71c71c71c71cp-4, 0x1.c71c71c71c71cp-58},
+    {0x1.745d1745d1746p-4, -0x1.745d1745d1746p-59},
+    {0x1.3b13b13b13b14p-4, -0x1.3b13b13b13b14p-58},
+    {0x1.1111111111111p-4, 0x1.1111111111111p-60},
+    {0x1.e1e1e1e1e1e1ep-5, 0x1.e1e1e1e1e1e1ep-61},
+    {0x1.af286bca1af28p-5, 0x1.af286bca1af28p-59},
+    {0x1.8618618618618p-5, 0x1.8618618618618p-59},
+    {0x1.642c8590b2164p-5, 0x1.642c8590b2164p-60},
+    {0x1.47ae147ae147bp-5, -0x1.eb851eb851eb8p-61},
+    {0x1.2f684bda12f68p-5, 0x1.2f684bda12f68p-59},
+    {0x1.1a7b9611a7b96p-5, 0x1.1a7b9611a7b96p-61},
+    {0x1.0842108421084p-5, 0x1.0842108421084p-60},
+    {0x1.f07c1f07c1f08p-6, -0x1.f07c1f07c1f08p-61},
+    {0x1.d41d41d41d41dp-6, 0x1.0750750750750p-60},
+    {0x1.bacf914c1bad0p-6, -0x1.bacf914c1bad0p-60},
+    {0x1.a41a41a41a41ap-6, 0x1.0690690690690p-60},
+    {0x1.8f9c18f9c18fap-6, -0x1.f3831f3831f38p-61},
+    {0x1.7d05f417d05f4p-6, 0x1.7d05f417d05f4p-62},
+    {0x1.6c16c16c16c17p-6, -0x1.f49f49f49f49fp-61},
+    {0x1.5c9882b931057p-6, 0x1.310572620ae4cp-61},
+    {0x1.4e5e0a72f0539p-6, 0x1.e0a72f0539783p-60},
+};
+constexpr double INV_FACT[16][2] = {
+    {0x1.0000000000000p+0, 0x0.0p+0},
+    {0x1.0000000000000p+0, 0x0.0p+0},
+    {0x1.0000000000000p-1, 0x0.0p+0},
+    {0x1.5555555555555p-3, 0x1.5555555555555p-57},
+    {0x1.5555555555555p-5, 0x1.5555555555555p-59},
+    {0x1.1111111111111p-7, 0x1.1111111111111p-63},
+    {0x1.6c16c16c16c17p-10, -0x1.f49f49f49f49fp-65},
+    {0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-73},
+    {0x1.a01a01a01a01ap-16, 0x1.a01a01a01a01ap-76},
+    {0x1.71de3a556c734p-19, -0x1.c154f8ddc6c00p-73},
+    {0x1.27e4fb7789f5cp-22, 0x1.cbbc05b4fa99ap-76},
+    {0x1.ae64567f544e4p-26, -0x1.c062e06d1f209p-80},
+    {0x1.1eed8eff8d898p-29, -0x1.2aec959e14c06p-83},
+    {0x1.6124613a86d09p-33, 0x1.f28e0cc748ebep-87},
+    {0x1.93974a8c07c9dp-37, 0x1.05d6f8a2efd1fp-92},
+    {0x1.ae7f3e733b81fp-41, 0x1.1d8656b0ee8cbp-97},
+};
 
 constexpr double LN2_HI = 0x1.62e42fefa39efp-1;    // ln 2 = LN2_HI + LN2_LO to 2^-108
 constexpr double LN2_LO = 0x1.abc9e3b39803fp-56;
@@ -147,8 +194,9 @@ STEP_POW_FN dd log_dd(double x)
     const dd s2 = mul(s, s);
     // atanh(s) / s = sum_k s^(2k) / (2k + 1): s2 <= 0.0295, so 24 terms reach 2^-120
     constexpr int K = 24;
-    dd p = recip(2.0 * K + 1.0);
-    for (int k = K - 1; k >= 0; k--) p = add(mul(p, s2), recip(2.0 * k + 1.0));
+    dd p = dd{INV_ODD[K][0], INV_ODD[K][1]};
+    STEP_POW_UNROLL
+    for (int k = K - 1; k >= 0; k--) p = add(mul(p, s2), dd{INV_ODD[k][0], INV_ODD[k][1]});
     const dd logm = mul_d(mul(s, p), 2.0);
     return add(k_ln2((double)e), logm);
 }
@@ -162,13 +210,9 @@ STEP_POW_FN double exp_dd_rounded(dd z)
     r.lo = r.lo * 0.0625;
     // exp(r/16) = sum_{i<=15} (r/16)^i / i!, |r/16| <= 0.0217
     constexpr int N = 15;
-    double fact = 1.0;
-    for (int i = 2; i <= N; i++) fact = fact * i;   // 15! is exact in a double
-    dd p = div(dd{1.0, 0.0}, dd{fact, 0.0});
-    for (int i = N - 1; i >= 0; i--) {
-        fact = fact / (i + 1);                      // i!, exact
-        p = add(mul(p, r), div(dd{1.0, 0.0}, dd{fact, 0.0}));
-    }
+    dd p = dd{INV_FACT[N][0], INV_FACT[N][1]};
+    STEP_POW_UNROLL
+    for (int i = N - 1; i >= 0; i--) p = add(mul(p, r), dd{INV_FACT[i][0], INV_FACT[i][1]});
     for (int q = 0; q < 4; q++) p = mul(p, p);      // (exp(r/16))^16
     // p is normalised (fast_two_sum), so p.hi is p rounded to nearest; 2^k scales exactly
     // while the result stays normal, which y <= 1/2 guarantees
