@@ -311,7 +311,8 @@ int red_ensemble_disk_events(red_ensemble* ens, int disk, red_event* buf, int ca
 /* out[3] = tried failed passed of one disk; its time and trial step. */
 int red_ensemble_disk_info(red_ensemble* ens, int disk, int out[3], double* t, double* dt_try);
 /* Picks the force-launch shape (threads per block x source tile, each in {64,128,256}) by
- * timing `reps` force evaluations per shape; ms[9] receives the timings (block-major). */
+ * timing `reps` attempts' force launches per shape (every stage, each with its pair-test
+ * rule, on the current state); ms[9] receives the timings (block-major). */
 int red_ensemble_autotune(red_ensemble* ens, int reps, int* best_block, int* best_jtile, double ms[9]);
 /* Supersteps per graph launch, 1..64 (default 8).  Each disk's controller runs on the device
  * (the reference's, int_rungekutta8.cu:704-797 and siblings, with step_pow below), so a
