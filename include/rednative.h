@@ -285,11 +285,14 @@ int red_copy_to_host(red_ctx* ctx, void* host, const void* dev, unsigned long lo
  * would have alone.  FP64, barycentric frame. */
 typedef struct red_ensemble red_ensemble;
 typedef struct red_ensemble_info {
-    int    supersteps;            /* attempts launched (one graph replay each)      */
+    int    supersteps;            /* supersteps in which some disk attempted        */
     int    active_disks;          /* disks still integrating after the call         */
     double seconds;               /* wall time of the call                          */
     double event_seconds;         /* of which host event passes and ejection scans  */
     double rebuild_seconds;       /* of which slot compactions                      */
+    int    batches;               /* batches launched (one graph replay each)       */
+    int    full_batches;          /* batches whose later supersteps idled: their
+                                     kept pair records passed half the capacity     */
 } red_ensemble_info;
 int red_ensemble_create(int n_disks, const red_bodies* disks, const red_params* params, red_ensemble** out);
 int red_ensemble_destroy(red_ensemble* ens);

@@ -1797,7 +1797,7 @@ int red_ensemble_set_time(red_ensemble* E, double t)
 int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_ensemble_info* info)
 {
     if (!E) return RED_ERR_ARG;
-    int attempts = 0;
+    int attempts = 0, batches = 0, full_batches = 0;
     const auto t_begin = std::chrono::steady_clock::now();
     E->event_s = 0.0;
     E->rebuild_s = 0.0;
@@ -1865,6 +1865,8 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         }
         ENS_HIP(E, hipStreamSynchronize(E->stream));
         unpack_ctl(E);
+        batches++;
+        full_batches += w.stop != 0;
         for (int s = 0; s < E->batch; s++) {
             bool any = false;
             for (int d = 0; d < nd && !any; d++) any = E->hc.steplog[s * nd + d] != 0;
@@ -1914,6 +1916,8 @@ int red_ensemble_advance(red_ensemble* E, double t_stop, int max_attempts, red_e
         info->seconds = seconds_since(t_begin);
         info->event_seconds = E->event_s;
         info->rebuild_seconds = E->rebuild_s;
+        info->batches = batches;
+        info->full_batches = full_batches;
     }
     return RED_OK;
 }

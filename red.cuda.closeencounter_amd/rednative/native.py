@@ -153,7 +153,8 @@ MLP_FEATURES = 12
 
 class RedEnsembleInfo(ctypes.Structure):
     _fields_ = [("supersteps", ctypes.c_int), ("active_disks", ctypes.c_int), ("seconds", ctypes.c_double),
-                ("event_seconds", ctypes.c_double), ("rebuild_seconds", ctypes.c_double)]
+                ("event_seconds", ctypes.c_double), ("rebuild_seconds", ctypes.c_double), ("batches", ctypes.c_int),
+                ("full_batches", ctypes.c_int)]
 
 _lib = None
 
@@ -577,7 +578,8 @@ class Ensemble:
         info = RedEnsembleInfo()
         self._chk(self.L.red_ensemble_advance(self.h, float(t_stop), int(max_attempts), ctypes.byref(info)))
         return dict(supersteps=info.supersteps, active_disks=info.active_disks, seconds=info.seconds,
-                    event_seconds=info.event_seconds, rebuild_seconds=info.rebuild_seconds)
+                    event_seconds=info.event_seconds, rebuild_seconds=info.rebuild_seconds, batches=info.batches,
+                    full_batches=info.full_batches)
 
     def counts(self, disk):
         c = np.zeros(7, dtype=np.int32)

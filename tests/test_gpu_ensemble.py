@@ -282,3 +282,37 @@ def test_ensemble_batch_sizes_agree(native_mod, case):
             assert_bitwise(v0, v1, f"disk {k} v")
             assert np.array_equal(b0["id"], b1["id"]) and c0 == c1 and i0 == i1, k
             events_equal(e0, e1, f"disk {k} events")
+
+
+def test_ensemble_small_record_capacity(native_mod):
+    """With a record capacity small enough that a batch's kept records pass half of it, the
+    batch's later supersteps idle and the next batch resumes them: members still equal those
+    of a run with one superstep per batch and ample capacity (and the batch size is checked)."""
+    from test_gpu_events import planted_disk
+    kw = EVENT_CASES["ck_hill"]
+    disks = [planted_disk(seed=41), planted_disk(seed=44)] + small_ensemble(n=2, seed=412)
+    for b in disks[2:]:
+        b.radius[0] = 1e-7
+    # 25-95 kept records per superstep (a few hundred raw ones per attempt): a batch of 16
+    # supersteps passes half of a 400-record capacity after a few of them
+    t_stop = 20.0 * 0.01720209895
+    print("case ensemble small record capacity", flush=True)
+    ref = native_mod.Ensemble(disks, event_capacity=200000, batch=1, **kw)
+    ref.advance(t_stop)
+    small = native_mod.Ensemble(disks, event_capacity=400, batch=16, **kw)
+    for bad in (0, 65):
+        with pytest.raises(native_mod.RedError):
+            small.set_batch(bad)
+    info = small.advance(t_stop)
+    assert info["active_disks"] == 0
+    print("batches", info["batches"], "full", info["full_batches"], "supersteps", info["supersteps"], flush=True)
+    assert info["full_batches"] > 0
+    from test_gpu_events import events_equal
+    n_records = 0
+    for k in range(len(disks)):
+        assert_bitwise(small.state(k)[0], ref.state(k)[0], f"disk {k} r")
+        assert small.counters(k) == ref.counters(k) and small.disk_info(k) == ref.disk_info(k), k
+        e_small, e_ref = small.events(k), ref.events(k)
+        events_equal(e_small, e_ref, f"disk {k} events")
+        n_records += len(e_ref)
+    print("records", n_records, flush=True)
