@@ -477,7 +477,8 @@ def ensemble_section(native, days, sample, use_graph=True):
     rules = dict(radii_enhance_factor=1.0, encounter_hill_factor=3.0)
     kw = dict(integrator=4, adaptive=True, tolerance=1e-11, dt_day=0.001, ejection=100.0, **rules)
     t_stop = days * GAUSS
-    E = native.Ensemble(disks, use_graph=use_graph, **kw)
+    batch = 8   # supersteps per graph launch
+    E = native.Ensemble(disks, use_graph=use_graph, batch=batch, **kw)
     block, jtile, _ = E.autotune(reps=3)
     t0 = time.perf_counter()
     info = E.advance(t_stop)
@@ -509,6 +510,7 @@ def ensemble_section(native, days, sample, use_graph=True):
             "speedup": rate_ens / rate_one, "supersteps": info["supersteps"],
             "ms_per_superstep": 1e3 * t_ens / max(info["supersteps"], 1), "seconds": t_ens, "events": ev,
             "host_event_seconds": info["event_seconds"], "host_rebuild_seconds": info["rebuild_seconds"],
+            "batches": info["batches"], "supersteps_per_batch": batch,
             "autotuned_block": block, "autotuned_jtile": jtile}
 
 

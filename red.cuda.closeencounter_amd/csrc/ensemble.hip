@@ -144,7 +144,7 @@ struct EnsPair {
     double c_stage;
     int stage;
     int superstep;                 // the record's attempt field until the host stamps it
-    red_event* buf;                // records of the batch, all disks
+    red_event* buf;                // the attempt's raw records, all disks
     int cap;
     int* count;
     int* rec;                      // records of this attempt per disk, and of them collisions
