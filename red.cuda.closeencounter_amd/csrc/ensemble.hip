@@ -1140,12 +1140,16 @@ int upload_tiles(red_ensemble* E)
         destroy_graph(E);
         E->tiles_block = E->block;
     }
-    if ((int)tiles.size() > E->tiles_cap) {
+    // room for a quarter more tiles than needed; a table more than twice that (left by a
+    // smaller launch shape, whose tiles hold fewer sinks) shrinks back, since every empty tile
+    // is still a work-group of every force launch
+    const int want = (int)tiles.size() + (int)tiles.size() / 4 + 64;
+    if ((int)tiles.size() > E->tiles_cap || E->tiles_cap > 2 * want) {
         SetupLock lk;
         (void)hipStreamSynchronize(E->stream);
         if (E->d_tiles) (void)hipFree(E->d_tiles);
         E->d_tiles = nullptr;
-        E->tiles_cap = (int)tiles.size() + (int)tiles.size() / 4 + 64;
+        E->tiles_cap = want;
         ENS_HIP(E, dmalloc(&E->d_tiles, E->tiles_cap));
         destroy_graph(E);
     }
