@@ -383,11 +383,10 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
             for (int k = (int)threadIdx.x - p0; k < nj; k += (int)blockDim.x - p0) {
                 double tx, ty, tz;
                 const bool use = P.gate[j0 + k] >= 0 && (j0 + k) != i;
-                const double d = fp64_term(r[j0 + k], p.x, p.y, p.z, use, tx, ty, tz);
+                (void)fp64_term(r[j0 + k], p.x, p.y, p.z, use, tx, ty, tz);
                 t0[k] = tx;
                 t1[k] = ty;
                 t2[k] = tz;
-                if (EV && use && live && j0 + k < i) ens_pair_test(P, T.disk, i, j0 + k, d, Ri, r, v, mass, radius, id);
             }
         };
         double acc = 0.0;
@@ -454,12 +453,10 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
                     if (k < nj) {
                         double tx, ty, tz;
                         const bool use = qa[m] >= 0 && (j0 + k) != i;
-                        const double d = fp64_term(q[m], p.x, p.y, p.z, use, tx, ty, tz);
+                        (void)fp64_term(q[m], p.x, p.y, p.z, use, tx, ty, tz);
                         term[ch & 1][0][k] = tx;
                         term[ch & 1][1][k] = ty;
                         term[ch & 1][2][k] = tz;
-                        if (EV && use && live && j0 + k < i)
-                            ens_pair_test(P, T.disk, i, j0 + k, d, Ri, r, v, mass, radius, id);
                     }
                 }
             };
@@ -486,6 +483,17 @@ __global__ __launch_bounds__(256) void k_ens_force(const EnsTile* __restrict__ t
                 __syncthreads();
                 if (c < 3) consume(ch);
                 __syncthreads();
+            }
+        }
+        if (EV && live) {
+            // the sink's pair tests (sources j < i only: i > j in the disk's order) after its
+            // sums, out of the producers' loops so that their registers stay those of the
+            // sums; d is the term's distance, formed by the same operations
+            for (int j = T.src0 + (int)threadIdx.x; j < min(T.src1, i); j += blockDim.x) {
+                if (P.gate[j] < 0) continue;
+                double tx, ty, tz;
+                const double d = fp64_term(r[j], p.x, p.y, p.z, true, tx, ty, tz);
+                ens_pair_test(P, T.disk, i, j, d, Ri, r, v, mass, radius, id);
             }
         }
         if (c < 3) reinterpret_cast<double*>(a + i)[c] = live ? acc : 0.0;

@@ -730,6 +730,8 @@ __global__ __launch_bounds__(256) void k_grav_mixed_bcast(const d4* __restrict__
                     const float dy = (float)(q.y - yi[s]);
                     const float dz = (float)(q.z - zi[s]);
                     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                    // one v_rsq_f32 (half rate: two issue slots) and three multiplies; an
+                    // added v_rcp_f32 for r^-2 in place of inv * inv measured 11 % slower
                     const float inv = __builtin_amdgcn_rsqf(r2);
                     float sc = (qm * inv) * (inv * inv);
                     if (OVERLAP) sc = (j == ii[s]) ? 0.0f : sc;
