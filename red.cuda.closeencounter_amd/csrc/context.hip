@@ -80,6 +80,7 @@ struct red_ctx {
     // the proposer's pair features run on their own stream beside the step's f0
     hipStream_t learn_side = nullptr;
     hipEvent_t ev_learn_fork = nullptr, ev_learn_join = nullptr;
+    bool learn_on_side = false;           // this step's pair features went to learn_side
     LearnPlan learn_plan_cur{};
     SrcMixed* d_src_mixed = nullptr;
     d4* d_partial = nullptr;
@@ -798,7 +799,7 @@ int learn_launch(red_ctx* c, bool propose, double eta_n)
     A.prev_log_max_pert = c->prev_log_max_pert;
     A.have_prev = c->have_prev_feat ? 1 : 0;
     LearnBuffers B{c->d_learn_part, c->d_learn_counter, c->h_prop_out_dev};
-    CTX_HIP(c, hipStreamWaitEvent(c->stream, c->ev_learn_join, 0));   // the pair launch of learn_pairs
+    if (c->learn_on_side) CTX_HIP(c, hipStreamWaitEvent(c->stream, c->ev_learn_join, 0));   // learn_pairs' launch
     CTX_HIP(c, timed(c, RED_PROF_LEARN, [&] {
         return launch_learn_tail(propose ? c->prop_net : nullptr, A, c->learn_plan_cur, c->d_r[c->cur], c->d_a[0],
                                  c->d_id, c->nSI(), c->nMassive(), c->n, B, c->d_ctl, propose ? 1 : 0, eta_n,
@@ -808,20 +809,27 @@ int learn_launch(red_ctx* c, bool propose, double eta_n)
 }
 
 // At the start of a step, before f0: the pair features need only y_n, so their launch runs
-// on the proposer's stream beside the force evaluation (learn_launch joins it).
+// on the proposer's stream beside the force evaluation (learn_launch joins it) when that
+// evaluation is long enough to hide it; a small disk's runs in line, ahead of f0 (there the
+// fork and join between the streams cost more than the launch itself).
+constexpr long LEARN_SIDE_MIN_PAIRS = 1L << 22;
+
 int learn_pairs(red_ctx* c)
 {
     int rc = learn_buffers(c);
     if (rc) return rc;
     c->learn_plan_cur = learn_plan(c->nSI(), c->nMassive(), c->n);
     LearnBuffers B{c->d_learn_part, c->d_learn_counter, c->h_prop_out_dev};
-    CTX_HIP(c, hipEventRecord(c->ev_learn_fork, c->stream));
-    CTX_HIP(c, hipStreamWaitEvent(c->learn_side, c->ev_learn_fork, 0));
+    c->learn_on_side = (long)c->nMassive() * (long)c->n >= LEARN_SIDE_MIN_PAIRS;
+    hipStream_t s = c->learn_on_side ? c->learn_side : c->stream;
+    if (c->learn_on_side) {
+        CTX_HIP(c, hipEventRecord(c->ev_learn_fork, c->stream));
+        CTX_HIP(c, hipStreamWaitEvent(c->learn_side, c->ev_learn_fork, 0));
+    }
     CTX_HIP(c, timed(c, RED_PROF_LEARN, [&] {
-        return launch_learn_pairs(c->learn_plan_cur, c->d_r[c->cur], c->d_id, c->nSI(), c->nMassive(), c->n, B,
-                                  c->learn_side);
-    }, c->learn_side));
-    CTX_HIP(c, hipEventRecord(c->ev_learn_join, c->learn_side));
+        return launch_learn_pairs(c->learn_plan_cur, c->d_r[c->cur], c->d_id, c->nSI(), c->nMassive(), c->n, B, s);
+    }, s));
+    if (c->learn_on_side) CTX_HIP(c, hipEventRecord(c->ev_learn_join, c->learn_side));
     return RED_OK;
 }
 

@@ -1,7 +1,7 @@
 """The learned step-size proposer in the integration loop.
 
 Gain: the network shipped with the package (trained by rednative.train_proposer on step
-records written by the library, 1,058,708 records; see rednative/data/README) proposes
+records written by the library, 1,121,692 records; see rednative/data/README) proposes
 every first trial step on disks it never saw -- the 256-body planetary disk of seed 101
 and the 16,384-body power-law disk of seed 202 -- while the embedded RKF7(8) error test
 still accepts or rejects every attempt.  Against the reference controller
