@@ -11,7 +11,8 @@ Reported: pair-interactions/s of the mixed-precision path (FP64 differences, FP3
 r^-3, compensated FP64 sums), simulated years per hour, the relative energy change
 over the timed steps, the same figures for the plain FP64 path on the same state,
 the roofline position of the dominant kernel (the test-particle force sweep) from
-HIP events around its launches, and the CPU oracle's pair rate on the host.  Further
+HIP events around its launches, beside the ALU ceiling of its pair sequence measured on the
+same GPU (red_valu_rates), and the CPU oracle's pair rate on the host.  Further
 sections (each also alone with --only): the learned step-size proposer on the 16384-body
 disk, the 512-disk ensemble, the 256-body disk's 10^4 fixed RK4 steps through the fused
 device loop and step by step (small_disk), and, only with --only drift, the energy drift
@@ -709,6 +710,8 @@ def main():
     ap.add_argument("--hbm-summary", default=os.path.join(ROOT, "profiles", "pmc_stage_err_eject_n131072.json"))
     ap.add_argument("--drift-summary", default=os.path.join(ROOT, "profiles", "drift_mixed_n131072_tol1e-11.json"))
     ap.add_argument("--no-bound", action="store_true", help="skip the section at tolerance 1e-11")
+    ap.add_argument("--valu-rates", action="store_true",
+                    help="print the vector-ALU issue rates of the GPU (red_valu_rates) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -721,6 +724,9 @@ def main():
 
     from rednative import generate, native
     native.load_library()
+    if args.valu_rates:
+        print(json.dumps({"valu_rates": native.valu_rates()}))
+        return
     if args.only == "proposer":
         print(json.dumps({"proposer": proposer_section(native, args.proposer_years, not args.no_graph)}))
         return
@@ -770,6 +776,10 @@ def main():
         ni64_ms, ni64_n = fp64["prof"]["ni"]
         achieved64 = FLOP_PER_PAIR * ni_pairs / ((ni64_ms / max(ni64_n, 1)) * 1e-3) / 1e12
     traffic, traffic_src = load_traffic(args.pmc_summary)
+    # the ALU ceiling of the sweep's own pair sequence on this GPU (sources in cache, no
+    # memory traffic), measured here beside it
+    rates = native.valu_rates()
+    ni_pairs_per_s = ni_pairs / avg_ni_s
 
     acc = None if args.no_fp64 or args.no_aux else accuracy_section(native, bodies)
     refw = None if args.no_fp64 or args.no_aux else reference_workload_section(native)
@@ -818,7 +828,14 @@ def main():
                      "flop_per_pair": FLOP_PER_PAIR,
                      "duration_source": "HIP events on the context stream around every launch, in a "
                                         "second pass over the same number of steps without graphs "
-                                        "(the timed pass replays graphs with no events)"},
+                                        "(the timed pass replays graphs with no events)",
+                     "issue_ceiling": {"pairs_per_s": rates["mixed_pairs_per_s"],
+                                       "frac": ni_pairs_per_s / rates["mixed_pairs_per_s"],
+                                       "lane_instr_per_s": {k: v for k, v in rates.items()
+                                                            if k != "mixed_pairs_per_s"},
+                                       "source": "red_valu_rates: the sweep's pair sequence (the same "
+                                                 "132 vector instructions per 8 pairs) with its sources "
+                                                 "in cache, and single-kind instruction chains"}},
         "roofline_hbm": hbm_rooflines(mixed, bodies, args.hbm_summary),
         "at_bound_tolerance": bound,
         "profile_ms": {k: v[0] for k, v in mixed["prof"].items() if v[1]},

@@ -329,6 +329,16 @@ int red_ensemble_set_batch(red_ensemble* ens, int supersteps);
  * (int_rungekutta8.cu:777, int_rungekutta4.cu:253), whose last bit depends on the platform. */
 int red_step_pow(const double* x, const double* y, double* out, int n, int on_device, int device);
 
+/* ---- calibration --------------------------------------------------------------------
+ * No counterpart in the reference: a measurement for the force sweeps' rooflines.  Vector-ALU
+ * issue rates of the device from kernels of independent instruction chains with no memory
+ * traffic: out[0..5] receive lane-instructions per second of FP32 FMA, FP32 multiply, FP64
+ * add, f64 <-> f32 conversion, v_rsq_f32, and of FP32 FMAs and FP64 adds issued by different
+ * wavefronts of each SIMD; out[6] receives pairs per second of the mixed-precision sweep's
+ * pair sequence with its sources in cache.  Allocates and frees its own buffers. */
+#define RED_VALU_RATES 7
+int red_valu_rates(int device, double* out, int n_out);
+
 /* ---- learned step-size proposer -----------------------------------------------------
  * Not in the reference, whose only step control is the elementary controller
  * dt <- 0.9 dt (tol/err)^(1/order) (int_rungekutta8.cu:776-777 and its siblings).

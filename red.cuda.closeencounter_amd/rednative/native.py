@@ -130,6 +130,7 @@ SIGNATURES = {
     "red_ensemble_autotune": (_I, [_P, _I, _IP, _IP, _DP]),
     "red_ensemble_set_batch": (_I, [_P, _I]),
     "red_step_pow": (_I, [_P, _P, _P, _I, _I, _I]),
+    "red_valu_rates": (_I, [_I, _P, _I]),
     "red_mlp_param_count": (_I, []),
     "red_mlp_create": (_I, [_I, _P, ctypes.POINTER(_P)]),
     "red_mlp_destroy": (_I, [_P]),
@@ -648,6 +649,22 @@ def step_pow(x, y, on_device=False, device=0):
     if rc:
         raise RedError(rc, "red_step_pow failed")
     return out
+
+
+VALU_RATE_NAMES = ("fma32", "mul32", "add64", "cvt", "rsq32", "fp32_fp64_waves")
+
+
+def valu_rates(device=0):
+    """Vector-ALU issue rates (red_valu_rates): lane-instructions per second per kind, and
+    pairs per second of the mixed sweep's pair sequence with its sources in cache."""
+    L = load_library()
+    out = np.zeros(len(VALU_RATE_NAMES) + 1)
+    rc = L.red_valu_rates(int(device), _ptr(out), int(out.size))
+    if rc:
+        raise RedError(rc, "red_valu_rates failed")
+    d = {k: float(v) for k, v in zip(VALU_RATE_NAMES, out)}
+    d["mixed_pairs_per_s"] = float(out[-1])
+    return d
 
 
 class Mlp:
