@@ -815,7 +815,8 @@ def main():
                 {"pairs_per_s": s64["pairs_per_s"], "sim_yr_per_hr": s64["sim_yr_per_hr"],
                  "ms_per_step": s64["ms_per_step"], "dE_over_E_timed": s64["dE_over_E"],
                  "steps": args.fp64_steps, "ni_sweep_tflops_20flop": achieved64,
-                 "ni_sweep_frac_fp64_peak": achieved64 / PEAK_FP64_TFLOPS},
+                 "ni_sweep_frac_fp64_peak": achieved64 / PEAK_FP64_TFLOPS,
+                 "ni_sweep_frac_fp64_pair_ceiling": achieved64 * 1e12 / FLOP_PER_PAIR / rates["fp64_pairs_per_s"]},
         "mixed_vs_fp64": acc,
         "reference_workload": refw,
         "speedup_sim_yr_per_hr_vs_fp64": sm["sim_yr_per_hr"] / s64["sim_yr_per_hr"] if s64 else None,
@@ -832,7 +833,7 @@ def main():
                      "issue_ceiling": {"pairs_per_s": rates["mixed_pairs_per_s"],
                                        "frac": ni_pairs_per_s / rates["mixed_pairs_per_s"],
                                        "lane_instr_per_s": {k: v for k, v in rates.items()
-                                                            if k != "mixed_pairs_per_s"},
+                                                            if not k.endswith("pairs_per_s")},
                                        "source": "red_valu_rates: the sweep's pair sequence (the same "
                                                  "132 vector instructions per 8 pairs) with its sources "
                                                  "in cache, and single-kind instruction chains"}},

@@ -100,8 +100,11 @@ typedef struct red_params {
     int    sinks_per_thread;      /* 0 = default; strategy 3: sinks per block
                                      (4, 8 or 16)                                   */
     int    strategy;              /* 0 = auto, 1 = scalar-broadcast, 2 = LDS tiles,
-                                     3 = FP64 producer/consumer (sequential sums;
-                                     the FP64 default)                              */
+                                     3 = FP64 sequential sums (the FP64 default):
+                                     producer/consumer, one work-item per sink for
+                                     disjoint sweeps of >= 49152 sinks;
+                                     4 = as 3 with one work-item per sink for every
+                                     disjoint sweep                                 */
     int    fp64_split;            /* FP64: let sums split across source ranges for
                                      occupancy (faster, not bit-identical to the
                                      reference's sequential order); 0 = sequential  */
@@ -335,8 +338,9 @@ int red_step_pow(const double* x, const double* y, double* out, int n, int on_de
  * traffic: out[0..5] receive lane-instructions per second of FP32 FMA, FP32 multiply, FP64
  * add, f64 <-> f32 conversion, v_rsq_f32, and of FP32 FMAs and FP64 adds issued by different
  * wavefronts of each SIMD; out[6] receives pairs per second of the mixed-precision sweep's
- * pair sequence with its sources in cache.  Allocates and frees its own buffers. */
-#define RED_VALU_RATES 7
+ * pair sequence with its sources in cache, out[7] the same for the FP64 sweep's pair term
+ * added in ascending j.  Allocates and frees its own buffers. */
+#define RED_VALU_RATES 8
 int red_valu_rates(int device, double* out, int n_out);
 
 /* ---- learned step-size proposer -----------------------------------------------------

@@ -83,6 +83,8 @@ struct red_ctx {
     bool learn_on_side = false;           // this step's pair features went to learn_side
     LearnPlan learn_plan_cur{};
     SrcMixed* d_src_mixed = nullptr;
+    d4* d_src_fp64 = nullptr;             // FP64 one-per-sink sweeps: packed sources
+    TpsChunk* d_tps_chunk = nullptr;      // and their chunks
     d4* d_partial = nullptr;
     size_t partial_elems = 0;
     std::vector<void*> user_buffers;      // red_device_alloc
@@ -278,6 +280,8 @@ hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const
     B.mass = c->d_mass; B.radius = c->d_radius; B.id = c->d_id; B.ctl = c->d_ctl;
     B.scratch = c->d_scratch; B.scratch_bytes = c->scratch_bytes;
     B.src_mixed = c->d_src_mixed;
+    B.src_fp64 = c->d_src_fp64;
+    B.tps_chunk = c->d_tps_chunk;
     B.partial = c->d_partial;
     B.partial_elems = c->partial_elems;
     B.seg = c->d_seg;
@@ -288,6 +292,10 @@ hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const
         RED_HIP_TRY(timed(c, RED_PROF_PACK, [&] { return launch_pack_mixed(r, c->padded ? c->d_ones : c->d_id, c->d_src_mixed, c->nMassive(), c->stream); }));
     Sweep sw[3];
     sweeps_of(c, sw);
+    if (sweep_takes_tps(sw[1], c->gcfg, B) || sweep_takes_tps(sw[2], c->gcfg, B))
+        RED_HIP_TRY(timed(c, RED_PROF_PACK, [&] {
+            return launch_pack_fp64(r, sw[2].gate, c->d_src_fp64, c->d_tps_chunk, c->nMassive(), c->stream);
+        }));
     // The three sweeps write disjoint rows of a and append events atomically (the list is
     // sorted afterwards), so the SI sweep -- few sinks, long serial sums -- can run on the
     // side stream while the NSI and NI sweeps fill the chip, unless two of them would share
@@ -488,7 +496,7 @@ int red_ctx_destroy(red_ctx* c)
     (void)hipFree(c->d_mass); (void)hipFree(c->d_radius); (void)hipFree(c->d_id); (void)hipFree(c->d_type);
     (void)hipFree(c->d_ones);
     (void)hipFree(c->d_ctl); (void)hipFree(c->d_events); (void)hipFree(c->d_events_hc); (void)hipFree(c->d_scratch);
-    (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy); (void)hipFree(c->d_energy_part);
+    (void)hipFree(c->d_src_mixed); (void)hipFree(c->d_src_fp64); (void)hipFree(c->d_tps_chunk); (void)hipFree(c->d_updates); (void)hipFree(c->d_energy); (void)hipFree(c->d_energy_part);
     (void)hipFree(c->d_partial); (void)hipFree(c->d_seg);
     (void)hipFree(c->d_learn_part); (void)hipFree(c->d_learn_counter);
     (void)hipFree(c->d_fused_rows); (void)hipFree(c->d_fused_out);
@@ -522,7 +530,7 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     if (p->integrator < RED_EULER || p->integrator > RED_RKF78) { delete c; return RED_ERR_UNSUPPORTED; }
     if (p->frame_center != RED_FRAME_BARY && p->frame_center != RED_FRAME_ASTRO) { delete c; return RED_ERR_ARG; }
     if (p->precision != RED_PREC_FP64 && p->precision != RED_PREC_MIXED) { delete c; return RED_ERR_ARG; }
-    if (p->strategy < 0 || p->strategy > 3) { delete c; return RED_ERR_ARG; }
+    if (p->strategy < 0 || p->strategy > 4) { delete c; return RED_ERR_ARG; }
     if (!p->inner_steps && (p->radii_enhance_factor > 0.0 || p->encounter_hill_factor > 0.0)) { delete c; return RED_ERR_UNSUPPORTED; }
     if (n > 0 && (!b->id || !b->type || !b->mass || !b->radius || !b->density || !b->cd || !b->r || !b->v)) {
         delete c;
@@ -623,6 +631,10 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
     CTX_HIP(c, dmalloc(&c->d_events, c->ev_cap));
     CTX_HIP(c, dmalloc(&c->d_events_hc, std::max(1, n)));
     if (p->precision == RED_PREC_MIXED) CTX_HIP(c, dmalloc(&c->d_src_mixed, cap));
+    if (p->precision == RED_PREC_FP64 && !p->fp64_split && c->gcfg.strategy >= 3) {
+        CTX_HIP(c, dmalloc(&c->d_src_fp64, cap));
+        CTX_HIP(c, dmalloc(&c->d_tps_chunk, (cap + TPS_CHUNK - 1) / TPS_CHUNK));
+    }
     if (p->precision == RED_PREC_MIXED || p->fp64_split) {
         // per-part sums: up to 64 parts of the largest sweep, capped at 1 GiB
         c->partial_elems = std::min((size_t)64 * cap, ((size_t)1 << 30) / sizeof(d4));
@@ -644,7 +656,7 @@ int red_ctx_create(const red_bodies* b, const red_params* p, red_ctx** out)
         if (need > limit) need = 0;   // too large: those sweeps use the broadcast path
         if (need > 0) CTX_HIP(c, hipMalloc(&c->d_scratch, need));
         c->scratch_bytes = need;
-        if (need > 0 && !(p->precision == RED_PREC_FP64 && !p->fp64_split && c->gcfg.strategy == 3)) {
+        if (need > 0 && !(p->precision == RED_PREC_FP64 && !p->fp64_split && c->gcfg.strategy >= 3)) {
             c->seg_elems = rows * SPLIT_SEG_MAX;
             CTX_HIP(c, hipMalloc(&c->d_seg, sizeof(double) * c->seg_elems));
         }

@@ -34,6 +34,7 @@
 // in FP64 for them, so event lists equal the FP64 path's for a given state.
 #include <algorithm>
 
+#include "fp64_core.hpp"
 #include "red_internal.hpp"
 #include "stage_forms.hpp"
 
@@ -96,46 +97,6 @@ __device__ __forceinline__ void pair_test(const PairTest& P, const EventSink& E,
     if (collide || hill) append_pair_event(P, E, ctl, i, j, d, collide, r, v, mass, id);
 }
 
-
-// IEEE square root and quotient for operands well inside the normal range.  For such
-// operands the compiler's correctly rounded sequences reduce to these cores: the square
-// root's range scaling is by 2^0, its special-value select keeps the core result, the
-// quotient's v_div_scale returns its operands unchanged with no scale flag (exponent gap
-// < 768, no denormal numerator, reciprocal or quotient), v_div_fmas is then a plain fused
-// multiply-add and v_div_fixup passes a normal quotient through.  So the results are
-// bit-identical, with about ten fewer instructions per pair.  Callers take this path only
-// when every lane's operands lie in [2^-300, 2^300] (in_core_range); otherwise they use
-// sqrt() and the division operator.
-__device__ __forceinline__ bool in_core_range(double x)
-{
-    return (unsigned)(__double2hiint(x) - 0x2D300000) < 0x25800000u;   // 2^-300 <= x < 2^300
-}
-
-__device__ __forceinline__ double sqrt_core(double x)
-{
-    const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y;
-    double h = y * 0.5;
-    const double r = fma(-h, g, 0.5);
-    g = fma(g, r, g);
-    h = fma(h, r, h);
-    double e = fma(-g, g, x);
-    g = fma(e, h, g);
-    e = fma(-g, g, x);
-    return fma(e, h, g);
-}
-
-__device__ __forceinline__ double div_core(double num, double den)
-{
-    double rc = __builtin_amdgcn_rcp(den);
-    double e = fma(-den, rc, 1.0);
-    rc = fma(rc, e, rc);
-    e = fma(-den, rc, 1.0);
-    rc = fma(rc, e, rc);
-    const double q = num * rc;
-    const double rem = fma(-den, q, num);
-    return fma(rem, rc, q);
-}
 
 // Distance below which pair_test can report an event for sink i, so that farther pairs skip
 // it without changing any result: the collision reach with the largest body radius, and the
@@ -577,6 +538,168 @@ hipError_t go_fp64_seq(int np, int sg, const Sweep& sw, const GravBuffers& B, co
     if (sg >= 16) return go_fp64_seq_sg<OV, EV, 16>(np, sw, B, P, E, s);
     if (sg >= 8) return go_fp64_seq_sg<OV, EV, 8>(np, sw, B, P, E, s);
     return go_fp64_seq_sg<OV, EV, 4>(np, sw, B, P, E, s);
+}
+
+// ---------------------------------------------------------------------------------------
+// FP64 sweep with one work-item per sink: the reference's own loop (pp_disk.cu:217-296),
+// each sink's sums formed in ascending j by its work-item, for sweeps whose sink and source
+// ranges are disjoint (NSI x SI, NI x massive) and whose sinks fill the chip (65536 sinks are
+// one wavefront per SIMD).  Sources reach the work-group through LDS, chunk by chunk, from a
+// packed copy (k_pack_fp64) in which an inactive source (id < 0) sits at (1e30, 0, 0)
+// with mass 0: its term is +0 (a zero quotient times the difference), which leaves every sum
+// as the reference's skip of that source does (the sums start at +0 and x + (+-0) == x), so
+// the loop has no per-source test.  Groups of four sources give independent terms, so one
+// wavefront keeps the FP64 pipeline busy while its three sums wait on their last addition.
+// The terms use the IEEE square-root and quotient cores: a chunk whose packed masses are not
+// all 0 or inside [2^-300, 2^300], whose coordinates with the sink's could push r^2 to 2^300,
+// or in which some r^2 falls below 2^-300 is formed again from its starting sums with sqrt()
+// and the division operator.  Pair tests: a chunk whose smallest r^2 is below the sink's
+// screen (squared, with margin) is swept again over the real positions and ids for the
+// reference predicate with the exact distance.
+constexpr int TPS_MIN_SINKS = 49152;      // automatic choice from this many sinks
+constexpr double TPS_PARK = 1e30;         // where the packed copy parks inactive sources
+
+__global__ __launch_bounds__(TPS_CHUNK) void k_pack_fp64(const d4* __restrict__ r, const int* __restrict__ gate,
+                                                         d4* __restrict__ out, TpsChunk* __restrict__ chunks, int n)
+{
+    __shared__ double wmax[TPS_CHUNK / 64];
+    const int j = blockIdx.x * TPS_CHUNK + threadIdx.x;
+    double b = 0.0;
+    int ok = 1;
+    if (j < n) {
+        const d4 q = r[j];
+        const d4 o = gate[j] >= 0 ? q : d4{TPS_PARK, 0.0, 0.0, 0.0};
+        out[j] = o;
+        b = fmax(fabs(o.x), fmax(fabs(o.y), fabs(o.z)));
+        ok = o.w == 0.0 || in_core_range(o.w);
+    }
+    ok = __syncthreads_and(ok);
+    for (int off = 32; off > 0; off >>= 1) b = fmax(b, __shfl_xor(b, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < TPS_CHUNK / 64; w++) b = fmax(b, wmax[w]);
+        chunks[blockIdx.x] = TpsChunk{b, ok, 0};
+    }
+}
+
+__device__ __forceinline__ void tps_term(const d4& q, const d4& p, double& ax, double& ay, double& az, double& lo)
+{
+    const double dx = q.x - p.x;
+    const double dy = q.y - p.y;
+    const double dz = q.z - p.z;
+    const double r2 = (dx * dx + dy * dy) + dz * dz;
+    const double d = sqrt_core(r2);
+    const double sc = div_core(q.w, d * r2);
+    ax += sc * dx;
+    ay += sc * dy;
+    az += sc * dz;
+    lo = fmin(lo, r2);
+}
+
+// One wavefront per SIMD: the occupancy hint lets the scheduler interleave a group's four
+// terms (with the default target it may form them one after another: 8.2 against 6.6 ms on a
+// 65536 x 65536 sweep).  A work-group is TPS_CHUNK work-items.
+template <bool EV>
+__global__ __launch_bounds__(TPS_CHUNK) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_grav_fp64_tps(
+    const d4* __restrict__ r, const d4* __restrict__ v,
+                                                       d4* __restrict__ a, const double* __restrict__ mass,
+                                                       const double* __restrict__ radius,
+                                                       const int* __restrict__ id, const d4* __restrict__ src,
+                                                       const TpsChunk* __restrict__ chunk, Sweep sw, PairTest P,
+                                                       EventSink E, Ctl* __restrict__ ctl)
+{
+    constexpr double LO = 0x1p-300, COORD = 0x1p149;   // |dx|, |dy|, |dz| < 2^149: r^2 < 2^300
+    const int i = sw.sink0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = i < sw.sink1;
+    const int ic = in ? i : sw.sink0;
+    const d4 p = r[ic];
+    const double pb = fmax(fabs(p.x), fmax(fabs(p.y), fabs(p.z)));
+    const bool live = in && sw.gate[ic] > 0;
+    const double Ri = EV ? radius[ic] : 0.0;
+    const double scr = EV && live ? pair_screen(P, Ri, mass[ic], p) : -1.0;
+    // d < scr implies r^2 < scr^2 (1 + 2^-51): the squared screen with a margin
+    const double scr2 = scr > 0.0 ? scr * scr * 1.0001 : -1.0;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    // Sources pass through LDS, a chunk per work-item each: the next chunk's loads are issued
+    // when a chunk starts and stored when it ends (freshly written positions come from memory,
+    // about a microsecond, which scalar loads in the loop would wait for group by group)
+    __shared__ d4 stage[2][TPS_CHUNK];
+    const int t = threadIdx.x;
+    const int nch = (sw.src1 - sw.src0 + TPS_CHUNK - 1) / TPS_CHUNK;
+    if (nch > 0 && sw.src0 + t < sw.src1) stage[0][t] = src[sw.src0 + t];
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        const int c0 = sw.src0 + c * TPS_CHUNK;
+        const int c1 = min(sw.src1, c0 + TPS_CHUNK);
+        const int nc = c1 - c0;
+        const d4* S = stage[c & 1];
+        const int jn = c0 + TPS_CHUNK + t;
+        const bool has_next = c + 1 < nch && jn < sw.src1;
+        d4 next{0.0, 0.0, 0.0, 0.0};
+        if (has_next) next = src[jn];
+        const TpsChunk ch = chunk[c0 / TPS_CHUNK];
+        const double bx = ax, by = ay, bz = az;
+        double lo = 0x1p300;        // smallest r^2 of the chunk
+        // two groups of four per pass: the second group's reads are in flight while the first
+        // group's terms are formed, and the next pass's first group while the second's are
+        int k = 0;
+        if (nc >= 8) {
+            d4 a0 = S[0], a1 = S[1], a2 = S[2], a3 = S[3];
+            for (; k + 8 <= nc; k += 8) {
+                const d4 b0 = S[k + 4], b1 = S[k + 5], b2 = S[k + 6], b3 = S[k + 7];
+                tps_term(a0, p, ax, ay, az, lo);
+                tps_term(a1, p, ax, ay, az, lo);
+                tps_term(a2, p, ax, ay, az, lo);
+                tps_term(a3, p, ax, ay, az, lo);
+                if (k + 12 <= nc) {
+                    a0 = S[k + 8];
+                    a1 = S[k + 9];
+                    a2 = S[k + 10];
+                    a3 = S[k + 11];
+                }
+                tps_term(b0, p, ax, ay, az, lo);
+                tps_term(b1, p, ax, ay, az, lo);
+                tps_term(b2, p, ax, ay, az, lo);
+                tps_term(b3, p, ax, ay, az, lo);
+            }
+        }
+        for (; k < nc; ++k) tps_term(S[k], p, ax, ay, az, lo);
+        if (!ch.mass_ok || !__all(lo >= LO && pb + ch.bound < COORD)) {
+            // outside the cores' range somewhere in the wavefront: the chunk again, general
+            ax = bx;
+            ay = by;
+            az = bz;
+            for (k = 0; k < nc; ++k) {
+                const d4 q = S[k];
+                const double dx = q.x - p.x;
+                const double dy = q.y - p.y;
+                const double dz = q.z - p.z;
+                const double r2 = (dx * dx + dy * dy) + dz * dz;
+                const double d = sqrt(r2);
+                const double sc = q.w / (d * r2);
+                ax += sc * dx;
+                ay += sc * dy;
+                az += sc * dz;
+            }
+        }
+        if (EV && __any(lo < scr2)) {
+            for (int j = c0; j < c1; ++j) {
+                if (sw.gate[j] < 0) continue;
+                const d4 q = r[j];
+                const double dx = q.x - p.x;
+                const double dy = q.y - p.y;
+                const double dz = q.z - p.z;
+                const double r2 = (dx * dx + dy * dy) + dz * dz;
+                const double d = sqrt(r2);
+                if (live && d < scr) pair_test(P, E, ctl, i, j, d, Ri, radius[j], r, v, mass, id);
+            }
+        }
+        // the buffer being filled was last read in the previous chunk, before its barrier
+        if (has_next) stage[(c + 1) & 1][t] = next;
+        __syncthreads();
+    }
+    if (in) a[i] = d4{live ? ax : 0.0, live ? ay : 0.0, live ? az : 0.0, 0.0};
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1083,8 +1206,32 @@ bool sweep_wants_split(const Sweep& sw, const GravConfig& C)
     // FP64 sequential sums over many sources for a handful of sinks: the producer/consumer
     // sweep would run a few blocks, so the terms are formed by the whole chip and each
     // (sink, component) row is added by a wave of its own
-    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3) return nsink <= 64 && nsrc >= 4096;
+    if (C.precision == RED_PREC_FP64 && !C.fp64_split && (C.strategy == 3 || C.strategy == 4))
+        return nsink <= 64 && nsrc >= 4096;
     return nsink <= C.split_max_sinks && nsrc >= 1024;
+}
+
+// One work-item per sink where the sink and source ranges are disjoint and the sources start
+// at body 0 (the packed copy's chunks): asked for (strategy 4), or chosen when the sinks fill
+// the chip and no producer/consumer shape was asked for.
+bool sweep_takes_tps(const Sweep& sw, const GravConfig& C, const GravBuffers& B)
+{
+    const int nsink = sw.sink1 - sw.sink0;
+    const int nsrc = sw.src1 - sw.src0;
+    if (!B.src_fp64 || !B.tps_chunk || nsink <= 0 || nsrc <= 0 || nsrc >= SEQ_MAX_SOURCES || C.astro) return false;
+    if (!(C.precision == RED_PREC_FP64 && !C.fp64_split && (C.strategy == 3 || C.strategy == 4))) return false;
+    if (sweep_wants_split(sw, C)) return false;
+    const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
+    if (ov || sw.src0 != 0) return false;
+    return C.strategy == 4 || (C.seq_sinks <= 0 && C.block_seq <= 0 && nsink >= TPS_MIN_SINKS);
+}
+
+hipError_t launch_pack_fp64(const d4* r, const int* gate, d4* out, TpsChunk* chunks, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_fp64, dim3((n + TPS_CHUNK - 1) / TPS_CHUNK), dim3(TPS_CHUNK), 0, s, r, gate, out, chunks,
+                       n);
+    return hipGetLastError();
 }
 
 hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig& C, const PairTest& P,
@@ -1104,10 +1251,20 @@ hipError_t launch_sweep(const Sweep& sw, const GravBuffers& B, const GravConfig&
         return hipGetLastError();
     }
     const bool use_split = sweep_wants_split(sw, C) && split_scratch_bytes(sw) <= B.scratch_bytes;
-    if (C.precision == RED_PREC_FP64 && !C.fp64_split && C.strategy == 3 && nsrc > 0 && nsrc < SEQ_MAX_SOURCES &&
-        !use_split) {
+    if (C.precision == RED_PREC_FP64 && !C.fp64_split && (C.strategy == 3 || C.strategy == 4) && nsrc > 0 &&
+        nsrc < SEQ_MAX_SOURCES && !use_split) {
         const bool ov = sw.src0 < sw.sink1 && sw.sink0 < sw.src1;
         const bool ev = P.enabled != 0;
+        if (sweep_takes_tps(sw, C, B)) {
+            const dim3 g((nsink + TPS_CHUNK - 1) / TPS_CHUNK);
+            if (ev)
+                hipLaunchKernelGGL(k_grav_fp64_tps<true>, g, dim3(TPS_CHUNK), 0, s, B.r, B.v, B.a, B.mass, B.radius, B.id,
+                                   B.src_fp64, B.tps_chunk, sw, P, E, B.ctl);
+            else
+                hipLaunchKernelGGL(k_grav_fp64_tps<false>, g, dim3(TPS_CHUNK), 0, s, B.r, B.v, B.a, B.mass, B.radius,
+                                   B.id, B.src_fp64, B.tps_chunk, sw, P, E, B.ctl);
+            return hipGetLastError();
+        }
         // sinks per block (chunk length 1024 / sg) and producer waves per block: as asked for,
         // else long chunks for few sinks (the serial chain per sink is the critical path), and
         // 16 sinks (fewer consumer additions per term) once that still gives 3 blocks per CU;

@@ -85,6 +85,16 @@ struct alignas(32) SrcMixed {
 // ---- launchers implemented in the .hip files (host side) ----------------------------
 namespace red {
 
+// Per source chunk of the one-work-item-per-sink FP64 sweep (TPS_CHUNK packed sources):
+// the largest |coordinate| of the chunk's packed sources and whether every packed mass is 0
+// or inside the square-root / quotient cores' range [2^-300, 2^300].
+constexpr int TPS_CHUNK = 256;
+struct TpsChunk {
+    double bound;
+    int mass_ok;
+    int pad;
+};
+
 struct GravBuffers {
     const d4* r;                   // stage positions (w = mass)
     const d4* v;                   // stage velocities (only read for event records)
@@ -96,6 +106,8 @@ struct GravBuffers {
     void* scratch;                 // split-sweep scratch
     size_t scratch_bytes;
     SrcMixed* src_mixed;           // packed sources for the mixed sweep (cap entries)
+    d4* src_fp64 = nullptr;        // packed sources for the FP64 one-per-sink sweep (cap
+    TpsChunk* tps_chunk = nullptr; // entries, inactive ones parked) and their chunks
     d4* partial;                   // per-part sums of source-partitioned sweeps
     size_t partial_elems;
     double* seg;                   // segment sums of split sweeps (own buffer: a split
@@ -131,6 +143,10 @@ bool sweep_wants_split(const Sweep& sw, const GravConfig& C);
 constexpr int SPLIT_SEG_MAX = 64;   // segments per row of a free-order split sum
 // Packs SrcMixed records for bodies [0, n).
 hipError_t launch_pack_mixed(const d4* r, const int* id, SrcMixed* out, int n, hipStream_t s);
+// whether launch_sweep sweeps it with one work-item per sink (then the FP64 sources [0, n)
+// must have been packed by launch_pack_fp64 for this evaluation)
+bool sweep_takes_tps(const Sweep& sw, const GravConfig& C, const GravBuffers& B);
+hipError_t launch_pack_fp64(const d4* r, const int* gate, d4* out, TpsChunk* chunks, int n, hipStream_t s);
 
 // Stage-sum forms (see kern_stages.hip).
 enum StageForm { SF_AXPY = 0, SF_DTSUM = 1, SF_ACCUM = 2, SF_RK4NP1 = 3, SF_RKF78NP1 = 4 };

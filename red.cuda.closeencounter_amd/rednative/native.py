@@ -656,14 +656,15 @@ VALU_RATE_NAMES = ("fma32", "mul32", "add64", "cvt", "rsq32", "fp32_fp64_waves")
 
 def valu_rates(device=0):
     """Vector-ALU issue rates (red_valu_rates): lane-instructions per second per kind, and
-    pairs per second of the mixed sweep's pair sequence with its sources in cache."""
+    pairs per second of the mixed and the FP64 sweeps' pair sequences with sources in cache."""
     L = load_library()
-    out = np.zeros(len(VALU_RATE_NAMES) + 1)
+    out = np.zeros(len(VALU_RATE_NAMES) + 2)
     rc = L.red_valu_rates(int(device), _ptr(out), int(out.size))
     if rc:
         raise RedError(rc, "red_valu_rates failed")
     d = {k: float(v) for k, v in zip(VALU_RATE_NAMES, out)}
-    d["mixed_pairs_per_s"] = float(out[-1])
+    d["mixed_pairs_per_s"] = float(out[-2])
+    d["fp64_pairs_per_s"] = float(out[-1])
     return d
 
 

@@ -81,13 +81,15 @@ def lockstep(o, d, steps, check_rebuild=True):
     return n_events, co
 
 
-@pytest.mark.parametrize("integ,adaptive,tol,dt", [(2, False, 0.0, 0.5), (4, True, 1e-10, 0.5), (3, True, 1e-10, 0.5)])
-def test_driver_loop_events_bit_exact(oracle_mod, native_mod, integ, adaptive, tol, dt):
-    print(f"case loop integ={integ} adaptive={adaptive}", flush=True)
+@pytest.mark.parametrize("integ,adaptive,tol,dt,strategy", [(2, False, 0.0, 0.5, 0), (4, True, 1e-10, 0.5, 0),
+                                                            (3, True, 1e-10, 0.5, 0), (4, True, 1e-10, 0.5, 4)])
+def test_driver_loop_events_bit_exact(oracle_mod, native_mod, integ, adaptive, tol, dt, strategy):
+    # strategy 4: the NSI x SI and NI x massive sweeps with one work-item per sink
+    print(f"case loop integ={integ} adaptive={adaptive} strategy={strategy}", flush=True)
     b = planted_disk()
     kw = dict(integrator=integ, adaptive=adaptive, tolerance=tol, dt_day=dt, **PRM)
     o = oracle_mod.Oracle(b, **kw)
-    d = native_mod.Disk(b, event_capacity=20000, **kw)
+    d = native_mod.Disk(b, event_capacity=20000, strategy=strategy, **kw)
     n_events, cnt = lockstep(o, d, 40)
     print("events", n_events, cnt)
     assert cnt["n_collision"] > 0 and cnt["n_ejection"] == 2 and cnt["n_hit_centrum"] == 2
@@ -139,13 +141,15 @@ def test_mixed_close_encounter_mode_matches_fp64(oracle_mod, native_mod):
     events_equal(sp, dmx.sp_events(), "mixed sp events")
 
 
-def test_hill_close_encounters_reported(oracle_mod, native_mod):
+@pytest.mark.parametrize("strategy", [0, 4])
+def test_hill_close_encounters_reported(oracle_mod, native_mod, strategy):
     # pairs within 10 mutual Hill radii in the ring (pairs with the star are never tested)
+    print(f"case hill strategy={strategy}", flush=True)
     b = planted_disk(seed=44)
     kw = dict(integrator=2, dt_day=0.5, hit_centrum=0.0, ejection=0.0, radii_enhance_factor=1.0,
               encounter_hill_factor=10.0)
     o = oracle_mod.Oracle(b, **kw)
-    d = native_mod.Disk(b, event_capacity=20000, **kw)
+    d = native_mod.Disk(b, event_capacity=20000, strategy=strategy, **kw)
     n_events, cnt = lockstep(o, d, 20)
     assert cnt["n_close_encounter"] > 0
 

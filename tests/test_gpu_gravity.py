@@ -53,10 +53,11 @@ def test_fp64_typed_disk_bit_exact(oracle_mod, native_mod):
     assert_bitwise(d.accel(1.0, b.r, b.v), o.accel(1.0, b.r, b.v), "accel")
 
 
-@pytest.mark.parametrize("strategy", [1, 3])
+@pytest.mark.parametrize("strategy", [1, 3, 4])
 def test_fp64_split_path_bit_exact(oracle_mod, native_mod, strategy):
     # 3 SI sinks against 6003 sources and 3000 NI sinks: with the broadcast strategy the SI
-    # sweep takes the split path; the producer/consumer strategy takes every sweep
+    # sweep takes the split path; the producer/consumer strategy takes every sweep; strategy 4
+    # sweeps the NSI and NI sinks with one work-item per sink
     print(f"case disk_mixed 3+3000+3000 strategy={strategy}", flush=True)
     b = generate.to_internal(generate.disk_mixed(n_pl=3000, n_tp=3000, seed=5))
     o, d = _pair(oracle_mod, native_mod, b, strategy=strategy)
@@ -232,3 +233,51 @@ def test_fp64_extreme_scales_bit_exact(oracle_mod, native_mod):
         d = native_mod.Disk(b, sinks_per_thread=spt, block=block, strategy=3)
         assert_bitwise(d.accel(0.0, b.r, b.v), a_o, f"accel sg={spt}")
         d.close()
+
+
+def test_fp64_one_per_sink_inactive_sources(oracle_mod, native_mod):
+    """One work-item per sink (strategy 4) with inactive sources (a giant and planetesimals,
+    id < 0, mass and position kept) in the groups of four that the sweep takes at once, and
+    the same sweeps through the producer/consumer kernels: both equal the oracle."""
+    b = generate.to_internal(generate.disk_mixed(n_pl=2500, n_tp=3001, seed=9))
+    b.ids[[1, 10, 11, 12, 13, 200, 1999]] *= -1
+    print("case one-per-sink inactive sources", flush=True)
+    o = oracle_mod.Oracle(b)
+    a_o = o.accel(0.25, b.r, b.v)
+    for strategy in (4, 3):
+        d = native_mod.Disk(b, strategy=strategy)
+        assert_bitwise(d.accel(0.25, b.r, b.v), a_o, f"accel strategy={strategy}")
+        d.close()
+
+
+def test_fp64_one_per_sink_extreme_scales(oracle_mod, native_mod):
+    """One work-item per sink across and around 2^-300 .. 2^300: massive sources and test
+    particles at the same scales, source masses from 1e-95 to 1e20, so that chunks leave the
+    cores' range through r^2 and through masses and are formed again with sqrt() and the
+    division operator.  Equals the oracle bit for bit."""
+    from rednative import files
+    rng = np.random.default_rng(78)
+    groups = [(1e-45, 40), (1e-30, 40), (1.0, 40), (1.5e45, 40)]
+    r_src, r_tp = [np.array([3.0, 0.0, 0.0])], []
+    for scale, k in groups:
+        r_src += list(rng.uniform(-1.0, 1.0, (k, 3)) * scale)
+        r_tp += list(rng.uniform(-1.0, 1.0, (k, 3)) * scale)
+    r = np.array(r_src + r_tp)
+    n, ns = len(r), len(r_src)
+    m = np.zeros(n)
+    m[:ns] = np.exp(rng.uniform(np.log(1e-95), np.log(1e20), ns))
+    m[0] = 1.0
+    types = np.full(n, 6, dtype=np.int32)
+    types[0] = 0
+    types[1:ns] = 3
+    counts = [1, 0, 0, ns - 1, 0, 0, n - ns]
+    b = files.BodySet(counts, np.arange(1, n + 1, dtype=np.int32), [f"b{k}" for k in range(n)], types,
+                      np.zeros(n), m, np.full(n, 1e-300), np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int32),
+                      np.zeros(n), r, np.zeros((n, 3)))
+    print(f"case one-per-sink extreme scales n={n}", flush=True)
+    o = oracle_mod.Oracle(b)
+    a_o = o.accel(0.0, b.r, b.v)
+    assert np.all(np.isfinite(a_o))
+    d = native_mod.Disk(b, strategy=4)
+    assert_bitwise(d.accel(0.0, b.r, b.v), a_o, "accel")
+    d.close()
