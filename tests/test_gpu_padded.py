@@ -40,12 +40,13 @@ def test_padded_accel_bit_exact(oracle_mod, native_mod, strategy, kw):
 
 def test_padded_split_and_classes_bit_exact(oracle_mod, native_mod):
     # 3 SI sinks against 6003 sources (the split path with the broadcast strategy), NSI and NI
-    # sinks, inactive bodies in every class
+    # sinks, inactive bodies in every class; strategy 4 sweeps the NSI and NI sinks with one
+    # work-item per sink (every source packed as it is: padded storage parks none)
     print("case padded disk_mixed 3+3000+3000", flush=True)
     b = generate.to_internal(generate.disk_mixed(n_pl=3000, n_tp=3000, seed=5))
     b = _inactive(b, [1, 10, 2000, 4000, 5000])
     o = oracle_mod.Oracle(b, padded=True)
-    for strategy in (1, 3):
+    for strategy in (1, 3, 4):
         d = native_mod.Disk(b, padded_storage=True, strategy=strategy)
         assert_bitwise(d.accel(2.0, b.r, b.v), o.accel(2.0, b.r, b.v), f"accel strategy {strategy}")
         d.close()
