@@ -294,7 +294,8 @@ hipError_t eval_forces(red_ctx* c, int stage, double c_stage, const d4* r, const
     sweeps_of(c, sw);
     if (sweep_takes_tps(sw[1], c->gcfg, B) || sweep_takes_tps(sw[2], c->gcfg, B))
         RED_HIP_TRY(timed(c, RED_PROF_PACK, [&] {
-            return launch_pack_fp64(r, sw[2].gate, c->d_src_fp64, c->d_tps_chunk, c->nMassive(), c->stream);
+            return launch_pack_fp64(r, sw[2].gate, c->d_radius, c->d_src_fp64, c->d_tps_chunk, c->nMassive(),
+                                    c->stream);
         }));
     // The three sweeps write disjoint rows of a and append events atomically (the list is
     // sorted afterwards), so the SI sweep -- few sinks, long serial sums -- can run on the

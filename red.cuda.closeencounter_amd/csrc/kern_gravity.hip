@@ -102,15 +102,17 @@ __device__ __forceinline__ void pair_test(const PairTest& P, const EventSink& E,
 // it without changing any result: the collision reach with the largest body radius, and the
 // Hill reach with the largest non-star mass, using |r_j| <= |r_i| + d (which turns
 // d < f c (|r_i| + |r_j|) / 2 into d < f c |r_i| / (1 - f c / 2)); 0.1 % margin for rounding.
+__device__ __forceinline__ double pair_screen_hill(const PairTest& P, double mi, const d4& qi)
+{
+    if (!(P.hill_f > 0.0)) return 0.0;
+    const double fc = P.hill_f * cbrt((mi + P.mmax_ns) / P.three_mstar);
+    const double hr = sqrt((qi.x * qi.x) + (qi.y * qi.y) + (qi.z * qi.z));
+    return fc < 1.0 ? fc * hr / (1.0 - 0.5 * fc) : 1e300;
+}
+
 __device__ __forceinline__ double pair_screen(const PairTest& P, double Ri, double mi, const d4& qi)
 {
-    double cut = P.radii_f * (Ri + P.rmax_all);
-    if (P.hill_f > 0.0) {
-        const double fc = P.hill_f * cbrt((mi + P.mmax_ns) / P.three_mstar);
-        const double hr = sqrt((qi.x * qi.x) + (qi.y * qi.y) + (qi.z * qi.z));
-        cut = fc < 1.0 ? fmax(cut, fc * hr / (1.0 - 0.5 * fc)) : 1e300;
-    }
-    return cut * 1.001;
+    return fmax(P.radii_f * (Ri + P.rmax_all), pair_screen_hill(P, mi, qi)) * 1.001;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -554,32 +556,45 @@ hipError_t go_fp64_seq(int np, int sg, const Sweep& sw, const GravBuffers& B, co
 // all 0 or inside [2^-300, 2^300], whose coordinates with the sink's could push r^2 to 2^300,
 // or in which some r^2 falls below 2^-300 is formed again from its starting sums with sqrt()
 // and the division operator.  Pair tests: a chunk whose smallest r^2 is below the sink's
-// screen (squared, with margin) is swept again over the real positions and ids for the
-// reference predicate with the exact distance.
+// screen for that chunk (collision reach with the chunk's largest radius, Hill reach with the
+// largest non-star mass; squared, with margin) is swept again over the real positions and
+// ids for the reference predicate with the exact distance.
 constexpr int TPS_MIN_SINKS = 49152;      // automatic choice from this many sinks
 constexpr double TPS_PARK = 1e30;         // where the packed copy parks inactive sources
 
 __global__ __launch_bounds__(TPS_CHUNK) void k_pack_fp64(const d4* __restrict__ r, const int* __restrict__ gate,
-                                                         d4* __restrict__ out, TpsChunk* __restrict__ chunks, int n)
+                                                         const double* __restrict__ radius, d4* __restrict__ out,
+                                                         TpsChunk* __restrict__ chunks, int n)
 {
-    __shared__ double wmax[TPS_CHUNK / 64];
+    __shared__ double wmax[2][TPS_CHUNK / 64];
     const int j = blockIdx.x * TPS_CHUNK + threadIdx.x;
-    double b = 0.0;
+    double b = 0.0, rm = 0.0;
     int ok = 1;
     if (j < n) {
         const d4 q = r[j];
-        const d4 o = gate[j] >= 0 ? q : d4{TPS_PARK, 0.0, 0.0, 0.0};
+        const bool act = gate[j] >= 0;
+        const d4 o = act ? q : d4{TPS_PARK, 0.0, 0.0, 0.0};
         out[j] = o;
         b = fmax(fabs(o.x), fmax(fabs(o.y), fabs(o.z)));
+        rm = act ? radius[j] : 0.0;
         ok = o.w == 0.0 || in_core_range(o.w);
     }
     ok = __syncthreads_and(ok);
-    for (int off = 32; off > 0; off >>= 1) b = fmax(b, __shfl_xor(b, off));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = b;
+    for (int off = 32; off > 0; off >>= 1) {
+        b = fmax(b, __shfl_xor(b, off));
+        rm = fmax(rm, __shfl_xor(rm, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wmax[0][threadIdx.x >> 6] = b;
+        wmax[1][threadIdx.x >> 6] = rm;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < TPS_CHUNK / 64; w++) b = fmax(b, wmax[w]);
-        chunks[blockIdx.x] = TpsChunk{b, ok, 0};
+        for (int w = 1; w < TPS_CHUNK / 64; w++) {
+            b = fmax(b, wmax[0][w]);
+            rm = fmax(rm, wmax[1][w]);
+        }
+        chunks[blockIdx.x] = TpsChunk{b, rm, ok, 0};
     }
 }
 
@@ -617,9 +632,9 @@ __global__ __launch_bounds__(TPS_CHUNK) __attribute__((amdgpu_waves_per_eu(1, 1)
     const double pb = fmax(fabs(p.x), fmax(fabs(p.y), fabs(p.z)));
     const bool live = in && sw.gate[ic] > 0;
     const double Ri = EV ? radius[ic] : 0.0;
-    const double scr = EV && live ? pair_screen(P, Ri, mass[ic], p) : -1.0;
-    // d < scr implies r^2 < scr^2 (1 + 2^-51): the squared screen with a margin
-    const double scr2 = scr > 0.0 ? scr * scr * 1.0001 : -1.0;
+    // the screen's Hill reach for this sink; its collision reach is taken per chunk, with the
+    // chunk's largest radius (a disk's star is in one chunk, not in every sink's screen)
+    const double hill = EV && live ? pair_screen_hill(P, mass[ic], p) : 0.0;
     double ax = 0.0, ay = 0.0, az = 0.0;
     // Sources pass through LDS, a chunk per work-item each: the next chunk's loads are issued
     // when a chunk starts and stored when it ends (freshly written positions come from memory,
@@ -639,6 +654,9 @@ __global__ __launch_bounds__(TPS_CHUNK) __attribute__((amdgpu_waves_per_eu(1, 1)
         d4 next{0.0, 0.0, 0.0, 0.0};
         if (has_next) next = src[jn];
         const TpsChunk ch = chunk[c0 / TPS_CHUNK];
+        const double scr = EV && live ? fmax(P.radii_f * (Ri + ch.rmax), hill) * 1.001 : -1.0;
+        // d < scr implies r^2 < scr^2 (1 + 2^-51): the squared screen with a margin
+        const double scr2 = scr > 0.0 ? scr * scr * 1.0001 : -1.0;
         const double bx = ax, by = ay, bz = az;
         double lo = 0x1p300;        // smallest r^2 of the chunk
         // two groups of four per pass: the second group's reads are in flight while the first
@@ -1226,11 +1244,12 @@ bool sweep_takes_tps(const Sweep& sw, const GravConfig& C, const GravBuffers& B)
     return C.strategy == 4 || (C.seq_sinks <= 0 && C.block_seq <= 0 && nsink >= TPS_MIN_SINKS);
 }
 
-hipError_t launch_pack_fp64(const d4* r, const int* gate, d4* out, TpsChunk* chunks, int n, hipStream_t s)
+hipError_t launch_pack_fp64(const d4* r, const int* gate, const double* radius, d4* out, TpsChunk* chunks, int n,
+                            hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pack_fp64, dim3((n + TPS_CHUNK - 1) / TPS_CHUNK), dim3(TPS_CHUNK), 0, s, r, gate, out, chunks,
-                       n);
+    hipLaunchKernelGGL(k_pack_fp64, dim3((n + TPS_CHUNK - 1) / TPS_CHUNK), dim3(TPS_CHUNK), 0, s, r, gate, radius, out,
+                       chunks, n);
     return hipGetLastError();
 }
 

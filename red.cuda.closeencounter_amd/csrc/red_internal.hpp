@@ -86,11 +86,13 @@ struct alignas(32) SrcMixed {
 namespace red {
 
 // Per source chunk of the one-work-item-per-sink FP64 sweep (TPS_CHUNK packed sources):
-// the largest |coordinate| of the chunk's packed sources and whether every packed mass is 0
-// or inside the square-root / quotient cores' range [2^-300, 2^300].
+// the largest |coordinate| of the chunk's packed sources, the largest radius of its active
+// sources (the pair screen's collision reach) and whether every packed mass is 0 or inside
+// the square-root / quotient cores' range [2^-300, 2^300].
 constexpr int TPS_CHUNK = 256;
 struct TpsChunk {
     double bound;
+    double rmax;
     int mass_ok;
     int pad;
 };
@@ -146,7 +148,8 @@ hipError_t launch_pack_mixed(const d4* r, const int* id, SrcMixed* out, int n, h
 // whether launch_sweep sweeps it with one work-item per sink (then the FP64 sources [0, n)
 // must have been packed by launch_pack_fp64 for this evaluation)
 bool sweep_takes_tps(const Sweep& sw, const GravConfig& C, const GravBuffers& B);
-hipError_t launch_pack_fp64(const d4* r, const int* gate, d4* out, TpsChunk* chunks, int n, hipStream_t s);
+hipError_t launch_pack_fp64(const d4* r, const int* gate, const double* radius, d4* out, TpsChunk* chunks, int n,
+                            hipStream_t s);
 
 // Stage-sum forms (see kern_stages.hip).
 enum StageForm { SF_AXPY = 0, SF_DTSUM = 1, SF_ACCUM = 2, SF_RK4NP1 = 3, SF_RKF78NP1 = 4 };
